@@ -1,0 +1,358 @@
+"""ctypes bindings of libfbse_gpu.so (include/fbse_gpu.h) and
+libfbse_host.so (include/fbse_host.h).
+
+The GPU path has no fallback: if the HIP library is missing or fails to load,
+every call raises.  Arrays follow the reference layouts (filter_mpi/): a state
+is stlen = nspin*ngrid*(is_complex+1) float64 values, complex interleaved.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GPU_LIB = os.path.join(HERE, "libfbse_gpu.so")
+HOST_LIB = os.path.join(HERE, "libfbse_host.so")
+
+FBSE_PREC_FP64 = 0
+FBSE_PREC_MIXED = 1
+
+DP = C.POINTER(C.c_double)
+LP = C.POINTER(C.c_long)
+IP = C.POINTER(C.c_int)
+
+ERRORS = {0: "FBSE_OK", -1: "FBSE_ERR_ARG", -2: "FBSE_ERR_HIP", -3: "FBSE_ERR_NORM_ZERO",
+          -4: "FBSE_ERR_NORM_NAN", -5: "FBSE_ERR_SOLVER", -6: "FBSE_ERR_NOMEM", -7: "FBSE_ERR_UNSUPPORTED"}
+
+
+class FbseError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (ERRORS.get(code, "?"), code, msg))
+        self.code = code
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("nx", C.c_long), ("ny", C.c_long), ("nz", C.c_long), ("nspin", C.c_long),
+                ("is_complex", C.c_int), ("dv", C.c_double), ("ksqr", DP), ("pot_local", DP)]
+
+
+class ProjDesc(C.Structure):
+    _fields_ = [("so", C.c_int), ("nl", C.c_int), ("n_atoms", C.c_long), ("n_gridpts", C.c_long),
+                ("n_gpt", LP), ("jxyz", LP), ("y1", DP), ("proj", DP), ("nl_proj", DP), ("nl_sign", IP),
+                ("n_so_proj", C.c_int), ("n_nl_proj", C.c_int), ("LS", DP)]
+
+
+_gpu = None
+_host = None
+
+# every entry point of include/fbse_gpu.h with its ctypes signature
+GPU_SIGNATURES = {
+    "fbse_create": (C.c_int, [C.POINTER(GridDesc), C.POINTER(ProjDesc), C.c_int, C.POINTER(C.c_void_p)]),
+    "fbse_destroy": (None, [C.c_void_p]),
+    "fbse_last_error": (C.c_char_p, [C.c_void_p]),
+    "fbse_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fbse_synchronize": (C.c_int, [C.c_void_p]),
+    "fbse_apply_h": (C.c_int, [C.c_void_p, DP, DP, C.c_long]),
+    "fbse_apply_h_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long]),
+    "fbse_filter": (C.c_int, [C.c_void_p, DP, C.c_long, C.c_long, C.c_long, DP, DP, C.c_double, C.c_double,
+                              C.c_int]),
+    "fbse_filter_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_long, DP, DP, C.c_double,
+                                  C.c_double, C.c_int]),
+    "fbse_normalize_all": (C.c_int, [C.c_void_p, DP, C.c_long]),
+    "fbse_normalize_all_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long]),
+    "fbse_energy_all": (C.c_int, [C.c_void_p, DP, C.c_long, DP]),
+    "fbse_energy_all_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, DP]),
+    "fbse_ortho": (C.c_int, [C.c_void_p, DP, C.c_long, C.c_double, LP, DP]),
+    "fbse_ortho_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_double, LP, DP]),
+    "fbse_rayleigh_ritz": (C.c_int, [C.c_void_p, DP, C.c_long, DP]),
+    "fbse_rayleigh_ritz_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, DP]),
+    "fbse_sigma": (C.c_int, [C.c_void_p, DP, C.c_long, DP, DP]),
+    "fbse_sigma_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, DP, DP]),
+    "fbse_energy_range": (C.c_int, [C.c_void_p, DP, DP, C.c_long, C.c_double, C.c_double, DP]),
+    "fbse_profile": (C.c_int, [C.c_void_p, C.c_int]),
+    "fbse_profile_read": (C.c_int, [C.c_void_p, C.c_int, LP, DP]),
+    "fbse_set_window": (C.c_int, [C.c_void_p, C.c_int]),
+    "fbse_dev_alloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
+    "fbse_dev_free": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fbse_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    "fbse_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+}
+
+HOST_SIGNATURES = {
+    "fbse_ran_nrc": (C.c_double, [LP]),
+    "fbse_init_psi": (None, [DP, LP, C.c_long, C.c_long, C.c_long, C.c_int, C.c_double]),
+    "fbse_init_filter_states": (None, [DP, C.c_long, C.c_long, LP, C.c_long, C.c_long, C.c_long, C.c_long,
+                                       C.c_int, C.c_double]),
+    "fbse_init_grid_params": (None, [DP, C.c_long, LP, DP, DP, DP, DP]),
+    "fbse_build_grid_ksqr": (None, [DP, DP, DP, DP, LP, DP, DP, DP, C.c_double]),
+    "fbse_samp_points_ashkenazy": (C.c_double, [DP, C.c_long, C.c_int]),
+    "fbse_gen_newton_coeff": (None, [DP, DP, DP, C.c_long, C.c_long, C.c_double, C.c_double, C.c_double,
+                                     C.c_int]),
+    "fbse_set_ene_targets": (None, [DP, C.c_long, C.c_long, C.c_double, C.c_double, C.c_double, C.c_double]),
+    "fbse_def_LS": (None, [DP]),
+}
+
+
+def _load(path, sigs):
+    if not os.path.exists(path):
+        raise RuntimeError("%s is not built: run `python -m filterbse_amd.build` (no CPU fallback exists)" % path)
+    lib = C.CDLL(path)
+    for name, (res, args) in sigs.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+def gpu():
+    global _gpu
+    if _gpu is None:
+        _gpu = _load(GPU_LIB, GPU_SIGNATURES)
+    return _gpu
+
+
+def host():
+    global _host
+    if _host is None:
+        gpu()  # the host library links against the GPU one
+        _host = _load(HOST_LIB, HOST_SIGNATURES)
+    return _host
+
+
+def _dp(a):
+    return a.ctypes.data_as(DP)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def nthreads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+# --------------------------------------------------------------------------
+# host-side inputs
+# --------------------------------------------------------------------------
+def init_filter_states(n_starts, ms, seed, nx, ny, nz, nspin, is_complex, dv):
+    stlen = (is_complex + 1) * nspin * nx * ny * nz
+    psi_rank = np.zeros(n_starts * ms * stlen)
+    s = C.c_long(seed)
+    host().fbse_init_filter_states(_dp(psi_rank), n_starts, ms, C.byref(s), nx, ny, nz, nspin, is_complex, dv)
+    return psi_rank, s.value
+
+
+def init_psi(seed, nx, ny, nz, is_complex, dv):
+    psi = np.zeros(2 * nx * ny * nz)
+    s = C.c_long(seed)
+    host().fbse_init_psi(_dp(psi), C.byref(s), nx, ny, nz, is_complex, dv)
+    return psi, s.value
+
+
+def energy_range_start_states(nx, ny, nz, nspin, is_complex, dv):
+    """The two start vectors of get_energy_range (energy.c:207,228-231,270-273):
+    init_psi per spin component from the fixed seed -874917403."""
+    seed = -874917403
+    ng = nx * ny * nz
+    out = []
+    for _ in range(2):
+        comps = []
+        for _s in range(nspin):
+            p, seed = init_psi(seed, nx, ny, nz, is_complex, dv)
+            comps.append(p if is_complex else p[0::2].copy())
+        out.append(np.concatenate(comps))
+    return out
+
+
+def grid_params(R, n_req, d):
+    R = _f64(R)
+    n = np.array(n_req, dtype=np.int64)
+    d = _f64(d)
+    mins, maxs, dk = np.zeros(3), np.zeros(3), np.zeros(3)
+    host().fbse_init_grid_params(_dp(R), len(R), n.ctypes.data_as(LP), _dp(d), _dp(mins), _dp(maxs), _dp(dk))
+    return n, mins, maxs, dk
+
+
+def grid_ksqr(n, d, mins, dk, ke_max=20.0):
+    n = np.array(n, dtype=np.int64)
+    ksqr = np.zeros(int(np.prod(n)))
+    gx, gy, gz = np.zeros(n[0]), np.zeros(n[1]), np.zeros(n[2])
+    host().fbse_build_grid_ksqr(_dp(ksqr), _dp(gx), _dp(gy), _dp(gz), n.ctypes.data_as(LP), _dp(_f64(d)),
+                                _dp(_f64(mins)), _dp(_f64(dk)), ke_max)
+    return ksqr, gx, gy, gz
+
+
+def leja_points(M):
+    zn = np.zeros(M)
+    host().fbse_samp_points_ashkenazy(_dp(zn), M, nthreads())
+    return zn
+
+
+def newton_coeff(zn, targets, Vmin, dE, dt):
+    zn, targets = _f64(zn), _f64(targets)
+    an = np.zeros(2 * len(targets) * len(zn))
+    host().fbse_gen_newton_coeff(_dp(an), _dp(zn), _dp(targets), len(targets), len(zn), Vmin, dE, dt, nthreads())
+    return an
+
+
+def ene_targets(nVB, nCB, VBmin, VBmax, CBmin, CBmax):
+    t = np.zeros(nVB + nCB)
+    host().fbse_set_ene_targets(_dp(t), nVB, nCB, VBmin, VBmax, CBmin, CBmax)
+    return t
+
+
+def def_LS():
+    ls = np.zeros(72)
+    host().fbse_def_LS(_dp(ls))
+    return ls
+
+
+# --------------------------------------------------------------------------
+# GPU context
+# --------------------------------------------------------------------------
+class Context:
+    """One fbse_ctx: grid, local potential and projector tables on one GPU."""
+
+    def __init__(self, nx, ny, nz, nspin, is_complex, dv, ksqr, pot_local, proj=None, device=-1):
+        L = gpu()
+        self._keep = {}
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.nspin, self.is_complex, self.dv = nspin, is_complex, dv
+        self.ngrid = nx * ny * nz
+        self.stlen = (is_complex + 1) * nspin * self.ngrid
+        ksqr = self._hold("ksqr", _f64(ksqr))
+        pot = self._hold("pot", _f64(pot_local))
+        if pot.size != nspin * self.ngrid:
+            raise ValueError("pot_local must hold nspin*ngrid values (the reference duplicates it per spin)")
+        g = GridDesc(nx, ny, nz, nspin, is_complex, dv, _dp(ksqr), _dp(pot))
+        pd = None
+        if proj is not None:
+            arrs = {}
+            for k, dt in (("n_gpt", np.int64), ("jxyz", np.int64), ("y1", np.float64), ("proj", np.float64),
+                          ("nl_proj", np.float64), ("nl_sign", np.int32), ("LS", np.float64)):
+                arrs[k] = self._hold(k, np.ascontiguousarray(proj[k], dtype=dt))
+            pd = ProjDesc(int(proj["so"]), int(proj["nl"]), int(proj["n_atoms"]), int(proj["n_gridpts"]),
+                          arrs["n_gpt"].ctypes.data_as(LP), arrs["jxyz"].ctypes.data_as(LP), _dp(arrs["y1"]),
+                          _dp(arrs["proj"]), _dp(arrs["nl_proj"]), arrs["nl_sign"].ctypes.data_as(IP),
+                          int(proj["n_so_proj"]), int(proj["n_nl_proj"]), _dp(arrs["LS"]))
+        ctx = C.c_void_p()
+        rc = L.fbse_create(C.byref(g), C.byref(pd) if pd is not None else None, device, C.byref(ctx))
+        if rc != 0:
+            raise FbseError(rc, "fbse_create failed")
+        self.ctx = ctx
+        self._keep.clear()  # the library copied everything
+
+    def _hold(self, k, a):
+        self._keep[k] = a
+        return a
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            gpu().fbse_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise FbseError(rc, gpu().fbse_last_error(self.ctx).decode())
+
+    # host-array entry points -------------------------------------------------
+    def apply_h(self, psi):
+        psi = _f64(psi)
+        out = np.zeros_like(psi)
+        self._check(gpu().fbse_apply_h(self.ctx, _dp(psi), _dp(out), psi.size // self.stlen))
+        return out
+
+    def filter(self, psi_rank, n_starts, ms, an, zn, Vmin, dE_1, precision=FBSE_PREC_FP64):
+        psi_rank = _f64(psi_rank).copy()
+        an, zn = _f64(an), _f64(zn)
+        self._check(gpu().fbse_filter(self.ctx, _dp(psi_rank), n_starts, ms, len(zn), _dp(an), _dp(zn), Vmin,
+                                      dE_1, precision))
+        return psi_rank
+
+    def normalize_all(self, psi):
+        psi = _f64(psi).copy()
+        self._check(gpu().fbse_normalize_all(self.ctx, _dp(psi), psi.size // self.stlen))
+        return psi
+
+    def energies(self, psi):
+        psi = _f64(psi)
+        n = psi.size // self.stlen
+        e = np.zeros(n)
+        self._check(gpu().fbse_energy_all(self.ctx, _dp(psi), n, _dp(e)))
+        return e
+
+    def ortho(self, psi, n_cols, svd_eps=1.0e-10):
+        psi = _f64(psi).copy()
+        rank = C.c_long(0)
+        sv = np.zeros(n_cols)
+        self._check(gpu().fbse_ortho(self.ctx, _dp(psi), n_cols, svd_eps, C.byref(rank), _dp(sv)))
+        return psi, rank.value, sv
+
+    def rayleigh_ritz(self, psi, n):
+        psi = _f64(psi).copy()
+        ev = np.zeros(n)
+        self._check(gpu().fbse_rayleigh_ritz(self.ctx, _dp(psi), n, _dp(ev)))
+        return psi, ev
+
+    def sigma(self, psi, n):
+        psi = _f64(psi)
+        s, e = np.zeros(n), np.zeros(n)
+        self._check(gpu().fbse_sigma(self.ctx, _dp(psi), n, _dp(s), _dp(e)))
+        return s, e
+
+    def energy_range(self, psi0_min, psi0_max, max_iter=1500, nl_lambda_min=0.0, vloc_min=0.0):
+        out = np.zeros(6)
+        self._check(gpu().fbse_energy_range(self.ctx, _dp(_f64(psi0_min)), _dp(_f64(psi0_max)), max_iter,
+                                            nl_lambda_min, vloc_min, _dp(out)))
+        return out
+
+    # device-pointer entry points (integers from torch .data_ptr()) -----------
+    def set_stream(self, stream_ptr):
+        self._check(gpu().fbse_set_stream(self.ctx, C.c_void_p(stream_ptr) if stream_ptr else None))
+
+    def synchronize(self):
+        self._check(gpu().fbse_synchronize(self.ctx))
+
+    def apply_h_dev(self, d_in, d_out, nstates):
+        self._check(gpu().fbse_apply_h_dev(self.ctx, C.c_void_p(d_in), C.c_void_p(d_out), nstates))
+
+    def filter_dev(self, d_psi_rank, n_starts, ms, an, zn, Vmin, dE_1, precision=FBSE_PREC_FP64):
+        an, zn = _f64(an), _f64(zn)
+        self._check(gpu().fbse_filter_dev(self.ctx, C.c_void_p(d_psi_rank), n_starts, ms, len(zn), _dp(an),
+                                          _dp(zn), Vmin, dE_1, precision))
+
+    def normalize_all_dev(self, d_psi, n):
+        self._check(gpu().fbse_normalize_all_dev(self.ctx, C.c_void_p(d_psi), n))
+
+    def ortho_dev(self, d_psi, n_cols, svd_eps=1.0e-10):
+        rank = C.c_long(0)
+        sv = np.zeros(n_cols)
+        self._check(gpu().fbse_ortho_dev(self.ctx, C.c_void_p(d_psi), n_cols, svd_eps, C.byref(rank), _dp(sv)))
+        return rank.value, sv
+
+    def rayleigh_ritz_dev(self, d_psi, n):
+        ev = np.zeros(n)
+        self._check(gpu().fbse_rayleigh_ritz_dev(self.ctx, C.c_void_p(d_psi), n, _dp(ev)))
+        return ev
+
+    def sigma_dev(self, d_psi, n):
+        s, e = np.zeros(n), np.zeros(n)
+        self._check(gpu().fbse_sigma_dev(self.ctx, C.c_void_p(d_psi), n, _dp(s), _dp(e)))
+        return s, e
+
+    def set_window(self, k):
+        self._check(gpu().fbse_set_window(self.ctx, k))
+
+    def profile(self, enable=True):
+        self._check(gpu().fbse_profile(self.ctx, 1 if enable else 0))
+
+    def profile_read(self, kind):
+        n, ms = C.c_long(0), C.c_double(0)
+        self._check(gpu().fbse_profile_read(self.ctx, kind, C.byref(n), C.byref(ms)))
+        return n.value, ms.value

@@ -1,0 +1,1103 @@
+// capi.hip -- C ABI of the gfx950 filter-diagonalization library (include/fbse_gpu.h):
+// context set-up, H application, the Newton filter driver, normalisation,
+// energies, orthogonalisation, Rayleigh-Ritz and sigma_E.
+#include "fbse_internal.h"
+
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+using namespace fbse;
+
+namespace fbse {
+
+int set_err(fbse_ctx *c, int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+static int grow(fbse_ctx *c, void **p, size_t *cur, size_t bytes) {
+  if (bytes <= *cur) return FBSE_OK;
+  if (*p) {
+    FBSE_HIP(c, hipStreamSynchronize(c->stream));
+    FBSE_HIP(c, hipFree(*p));
+    *p = nullptr;
+    *cur = 0;
+  }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return set_err(c, FBSE_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  *cur = bytes;
+  return FBSE_OK;
+}
+int ensure_scratch(fbse_ctx *c, size_t b) { return grow(c, &c->d_scratch, &c->scratch_bytes, b); }
+int ensure_scratch2(fbse_ctx *c, size_t b) { return grow(c, &c->d_scratch2, &c->scratch2_bytes, b); }
+int ensure_red(fbse_ctx *c, size_t b) { return grow(c, (void **)&c->d_red, &c->red_bytes, b); }
+
+static int ensure_host_red(fbse_ctx *c, size_t b) {
+  if (b <= c->h_red_bytes) return FBSE_OK;
+  if (c->h_red) hipHostFree(c->h_red);
+  c->h_red = nullptr;
+  c->h_red_bytes = 0;
+  FBSE_HIP(c, hipHostMalloc((void **)&c->h_red, b, hipHostMallocDefault));
+  c->h_red_bytes = b;
+  return FBSE_OK;
+}
+
+void prof_begin(fbse_ctx *c, int kind, hipEvent_t *ev) {
+  if (!c->profile) return;
+  hipEvent_t e;
+  if (!c->ev_pool.empty()) {
+    e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+  } else {
+    hipEventCreate(&e);
+  }
+  hipEventRecord(e, c->stream);
+  *ev = e;
+  (void)kind;
+}
+void prof_end(fbse_ctx *c, int kind, hipEvent_t ev) {
+  if (!c->profile || !ev) return;
+  hipEvent_t e;
+  if (!c->ev_pool.empty()) {
+    e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+  } else {
+    hipEventCreate(&e);
+  }
+  hipEventRecord(e, c->stream);
+  c->ev_pending.push_back({kind, {ev, e}});
+  if (c->ev_pending.size() > 4096) prof_collect(c);
+}
+void prof_collect(fbse_ctx *c) {
+  if (c->ev_pending.empty()) return;
+  hipStreamSynchronize(c->stream);
+  for (auto &p : c->ev_pending) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, p.second.first, p.second.second);
+    c->timers[p.first].launches += 1;
+    c->timers[p.first].ms += ms;
+    c->timers[4].launches += 1;
+    c->timers[4].ms += ms;
+    c->ev_pool.push_back(p.second.first);
+    c->ev_pool.push_back(p.second.second);
+  }
+  c->ev_pending.clear();
+}
+
+// Radix schedule: 8s first, then 4, 2, 3, 5, then any remaining primes.
+static void factorize(int n, AxisPlan &pl) {
+  pl.n = n;
+  pl.nf = 0;
+  int m = n;
+  const int pref[] = {8, 4, 2, 3, 5};
+  for (int r : pref)
+    while (m % r == 0 && pl.nf < kMaxFactors) {
+      pl.f[pl.nf++] = r;
+      m /= r;
+    }
+  for (int p = 7; m > 1 && p <= n; p += 2)
+    while (m % p == 0 && pl.nf < kMaxFactors) {
+      pl.f[pl.nf++] = p;
+      m /= p;
+    }
+}
+
+}  // namespace fbse
+
+// ---------------------------------------------------------------------------
+// creation / destruction
+// ---------------------------------------------------------------------------
+static int upload(fbse_ctx *c, void **dst, const void *src, size_t bytes) {
+  FBSE_HIP(c, hipMalloc(dst, bytes ? bytes : 16));
+  if (bytes) FBSE_HIP(c, hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+  return FBSE_OK;
+}
+
+template <typename T, typename S> static std::vector<T> cast_vec(const S *p, size_t n) {
+  std::vector<T> v(n);
+  for (size_t i = 0; i < n; ++i) v[i] = (T)p[i];
+  return v;
+}
+
+static int build_projectors(fbse_ctx *c, const fbse_proj_desc *pd) {
+  const long na = pd->n_atoms, stride = pd->n_gridpts;
+  std::vector<int> off(na), cnt(na), jx, sign(na * 5);
+  long npts = 0;
+  for (long a = 0; a < na; ++a) {
+    off[a] = (int)npts;
+    cnt[a] = (int)pd->n_gpt[a];
+    if (pd->n_gpt[a] < 0 || pd->n_gpt[a] > stride)
+      return set_err(c, FBSE_ERR_ARG, "atom %ld: n_gpt %ld outside [0, n_gridpts=%ld]", a, pd->n_gpt[a], stride);
+    npts += pd->n_gpt[a];
+  }
+  if (npts > 0x7fffffffL) return set_err(c, FBSE_ERR_UNSUPPORTED, "projector table too large");
+  jx.resize(npts);
+  std::vector<double> proj(5 * npts, 0.0), nlp(5 * npts, 0.0), y(6 * npts, 0.0);
+  for (long a = 0; a < na; ++a) {
+    for (int p = 0; p < 5; ++p) sign[a * 5 + p] = pd->nl_sign ? pd->nl_sign[a * 5 + p] : 1;
+    for (long g = 0; g < pd->n_gpt[a]; ++g) {
+      const long src = a * stride + g, dst = off[a] + g;
+      const long jxyz = pd->jxyz[src];
+      if (jxyz < 0 || jxyz >= c->ngrid)
+        return set_err(c, FBSE_ERR_ARG, "atom %ld point %ld: jxyz %ld outside the grid", a, g, jxyz);
+      jx[dst] = (int)jxyz;
+      for (int p = 0; p < 5; ++p) {
+        if (pd->so && p < pd->n_so_proj) proj[p * npts + dst] = pd->proj[src * 5 + p];
+        if (pd->nl && p < pd->n_nl_proj) nlp[p * npts + dst] = pd->nl_proj[src * 5 + p];
+      }
+      for (int m = 0; m < 3; ++m) {
+        y[2 * (m * npts + dst)] = pd->y1[src * 6 + 2 * m];
+        y[2 * (m * npts + dst) + 1] = pd->y1[src * 6 + 2 * m + 1];
+      }
+    }
+  }
+  double lsd[72];
+  for (int i = 0; i < 72; ++i) lsd[i] = pd->LS ? pd->LS[i] : 0.0;
+  int rc;
+  auto &P = c->pj;
+  auto &F = c->pj_f;
+  P.n_atoms = F.n_atoms = (int)na;
+  P.npts = F.npts = (int)npts;
+  P.so = F.so = pd->so;
+  P.nl = F.nl = pd->nl;
+  P.n_so = F.n_so = pd->n_so_proj;
+  P.n_nl = F.n_nl = pd->n_nl_proj;
+  if ((rc = upload(c, (void **)&P.a_off, off.data(), na * sizeof(int)))) return rc;
+  if ((rc = upload(c, (void **)&P.a_cnt, cnt.data(), na * sizeof(int)))) return rc;
+  if ((rc = upload(c, (void **)&P.jxyz, jx.data(), npts * sizeof(int)))) return rc;
+  if ((rc = upload(c, (void **)&P.sign, sign.data(), na * 5 * sizeof(int)))) return rc;
+  F.a_off = P.a_off;
+  F.a_cnt = P.a_cnt;
+  F.jxyz = P.jxyz;
+  F.sign = P.sign;
+  if ((rc = upload(c, (void **)&P.proj, proj.data(), proj.size() * 8))) return rc;
+  if ((rc = upload(c, (void **)&P.nlproj, nlp.data(), nlp.size() * 8))) return rc;
+  if ((rc = upload(c, (void **)&P.y1, y.data(), y.size() * 8))) return rc;
+  if ((rc = upload(c, (void **)&P.ls, lsd, sizeof lsd))) return rc;
+  auto pf = cast_vec<float>(proj.data(), proj.size());
+  auto nf = cast_vec<float>(nlp.data(), nlp.size());
+  auto yf = cast_vec<float>(y.data(), y.size());
+  auto lf = cast_vec<float>(lsd, 72);
+  if ((rc = upload(c, (void **)&F.proj, pf.data(), pf.size() * 4))) return rc;
+  if ((rc = upload(c, (void **)&F.nlproj, nf.data(), nf.size() * 4))) return rc;
+  if ((rc = upload(c, (void **)&F.y1, yf.data(), yf.size() * 4))) return rc;
+  if ((rc = upload(c, (void **)&F.ls, lf.data(), lf.size() * 4))) return rc;
+  c->has_proj = (pd->so || pd->nl) && npts > 0;
+  return FBSE_OK;
+}
+
+extern "C" int fbse_create(const fbse_grid_desc *g, const fbse_proj_desc *pd, int device, fbse_ctx **out) {
+  if (!out) return FBSE_ERR_ARG;
+  *out = nullptr;
+  if (!g || g->nx < 1 || g->ny < 1 || g->nz < 1 || (g->nspin != 1 && g->nspin != 2) || !g->ksqr || !g->pot_local)
+    return FBSE_ERR_ARG;
+  if (g->nx > 1024 || g->ny > 1024 || g->nz > 1024) return FBSE_ERR_UNSUPPORTED;
+  if (g->nspin == 2 && !g->is_complex) return FBSE_ERR_ARG;  // spinors are complex (read.c:919-926)
+  if (pd && pd->so && g->nspin != 2) return FBSE_ERR_ARG;   // SO needs spinors (read.c:914-917)
+  fbse_ctx *c = new fbse_ctx();
+  int rc = FBSE_OK;
+  if (device < 0) {
+    if (hipGetDevice(&c->device) != hipSuccess) {
+      delete c;
+      return FBSE_ERR_HIP;
+    }
+  } else {
+    c->device = device;
+  }
+  if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return FBSE_ERR_HIP;
+  }
+  c->stream = c->own_stream;
+  if (rocblas_create_handle(&c->blas) != rocblas_status_success) {
+    fbse_destroy(c);
+    return FBSE_ERR_SOLVER;
+  }
+  rocblas_set_stream(c->blas, c->stream);
+  c->nx = g->nx;
+  c->ny = g->ny;
+  c->nz = g->nz;
+  c->ngrid = g->nx * g->ny * g->nz;
+  c->nspin = g->nspin;
+  c->nsg = c->nspin * c->ngrid;
+  c->is_complex = g->is_complex;
+  c->dv = g->dv;
+
+  // ksqr: detect the separable form of build_grid_ksqr (init.c:336-347) so the
+  // z pass can rebuild k^2 from three 1D tables instead of reading N doubles.
+  {
+    const long nx = c->nx, ny = c->ny, nz = c->nz;
+    std::vector<double> kx(nx), ky(ny), kz(nz);
+    double kmax = 0.0;
+    for (long i = 0; i < c->ngrid; ++i) kmax = std::max(kmax, g->ksqr[i]);
+    for (long i = 0; i < nx; ++i) kx[i] = g->ksqr[i];
+    for (long j = 0; j < ny; ++j) ky[j] = g->ksqr[nx * j];
+    for (long k = 0; k < nz; ++k) kz[k] = g->ksqr[nx * ny * k];
+    bool sep = g->ksqr[0] == 0.0;
+    for (long k = 0; sep && k < nz; ++k)
+      for (long j = 0; sep && j < ny; ++j)
+        for (long i = 0; i < nx; ++i) {
+          double v = kx[i] + ky[j] + kz[k];
+          if (v > kmax) v = kmax;
+          if (v != g->ksqr[nx * (ny * k + j) + i]) {
+            sep = false;
+            break;
+          }
+        }
+    // a capped 1D entry would make the tables wrong; require no cap on the axes
+    for (long i = 0; sep && i < nx; ++i) sep = kx[i] < kmax || i == 0 || kmax == 0.0;
+    c->ksqr_sep = sep ? 1 : 0;
+    c->kcap = kmax;
+    if ((rc = upload(c, (void **)&c->d_kx, kx.data(), nx * 8)) || (rc = upload(c, (void **)&c->d_ky, ky.data(), ny * 8)) ||
+        (rc = upload(c, (void **)&c->d_kz, kz.data(), nz * 8)) ||
+        (rc = upload(c, (void **)&c->d_ksqr, g->ksqr, c->ngrid * 8))) {
+      fbse_destroy(c);
+      return rc;
+    }
+    auto fx = cast_vec<float>(kx.data(), nx), fy = cast_vec<float>(ky.data(), ny), fz = cast_vec<float>(kz.data(), nz);
+    auto fk = cast_vec<float>(g->ksqr, c->ngrid);
+    if ((rc = upload(c, (void **)&c->d_kx_f, fx.data(), nx * 4)) || (rc = upload(c, (void **)&c->d_ky_f, fy.data(), ny * 4)) ||
+        (rc = upload(c, (void **)&c->d_kz_f, fz.data(), nz * 4)) ||
+        (rc = upload(c, (void **)&c->d_ksqr_f, fk.data(), c->ngrid * 4))) {
+      fbse_destroy(c);
+      return rc;
+    }
+  }
+  {
+    auto vf = cast_vec<float>(g->pot_local, c->nsg);
+    if ((rc = upload(c, (void **)&c->d_vloc, g->pot_local, c->nsg * 8)) ||
+        (rc = upload(c, (void **)&c->d_vloc_f, vf.data(), c->nsg * 4))) {
+      fbse_destroy(c);
+      return rc;
+    }
+  }
+  const long dims[3] = {c->nx, c->ny, c->nz};
+  for (int ax = 0; ax < 3; ++ax) {
+    const int n = (int)dims[ax];
+    factorize(n, c->plan[ax]);
+    // tile shape: x rows fill ~2048 elements; y/z take 16 consecutive inner
+    // indices (256-byte rows for FP64)
+    if (ax == 0) c->plan[ax].lines = std::max(1, std::min(64, 2048 / n));
+    else c->plan[ax].lines = n > 256 ? 8 : 16;
+    std::vector<double2> tw(n);
+    std::vector<float2> twf(n);
+    for (int t = 0; t < n; ++t) {
+      const long double ang = -2.0L * 3.141592653589793238462643383279502884L * (long double)t / (long double)n;
+      tw[t] = make_double2((double)cosl(ang), (double)sinl(ang));
+      twf[t] = make_float2((float)cosl(ang), (float)sinl(ang));
+    }
+    if ((rc = upload(c, (void **)&c->d_tw[ax], tw.data(), n * sizeof(double2))) ||
+        (rc = upload(c, (void **)&c->d_tw_f[ax], twf.data(), n * sizeof(float2)))) {
+      fbse_destroy(c);
+      return rc;
+    }
+  }
+  if (pd && (pd->so || pd->nl)) {
+    if ((rc = build_projectors(c, pd))) {
+      std::string msg = c->err;
+      fprintf(stderr, "fbse_create: %s\n", msg.c_str());
+      fbse_destroy(c);
+      return rc;
+    }
+  }
+  *out = c;
+  return FBSE_OK;
+}
+
+extern "C" void fbse_destroy(fbse_ctx *c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  void *ptrs[] = {c->d_vloc, c->d_vloc_f, c->d_kx, c->d_ky, c->d_kz, c->d_ksqr, c->d_kx_f, c->d_ky_f, c->d_kz_f,
+                  c->d_ksqr_f, c->d_scratch, c->d_scratch2, c->d_red, c->pj.a_off, c->pj.a_cnt, c->pj.jxyz,
+                  c->pj.sign, c->pj.proj, c->pj.nlproj, c->pj.y1, c->pj.ls, c->pj_f.proj, c->pj_f.nlproj,
+                  c->pj_f.y1, c->pj_f.ls};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  for (int a = 0; a < 3; ++a) {
+    if (c->d_tw[a]) hipFree(c->d_tw[a]);
+    if (c->d_tw_f[a]) hipFree(c->d_tw_f[a]);
+  }
+  if (c->h_red) hipHostFree(c->h_red);
+  for (auto e : c->ev_pool) hipEventDestroy(e);
+  for (auto &p : c->ev_pending) {
+    hipEventDestroy(p.second.first);
+    hipEventDestroy(p.second.second);
+  }
+  if (c->blas) rocblas_destroy_handle(c->blas);
+  if (c->own_stream) hipStreamDestroy(c->own_stream);
+  delete c;
+}
+
+extern "C" const char *fbse_last_error(const fbse_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int fbse_set_stream(fbse_ctx *c, void *stream) {
+  if (!c) return FBSE_ERR_ARG;
+  c->stream = stream ? (hipStream_t)stream : c->own_stream;
+  rocblas_set_stream(c->blas, c->stream);
+  return FBSE_OK;
+}
+extern "C" int fbse_synchronize(fbse_ctx *c) {
+  if (!c) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipStreamSynchronize(c->stream));
+  return FBSE_OK;
+}
+extern "C" int fbse_profile(fbse_ctx *c, int enable) {
+  if (!c) return FBSE_ERR_ARG;
+  prof_collect(c);
+  c->profile = enable;
+  for (auto &t : c->timers) t = Timer{};
+  return FBSE_OK;
+}
+extern "C" int fbse_profile_read(fbse_ctx *c, int kind, long *launches, double *total_ms) {
+  if (!c || kind < 0 || kind > 4) return FBSE_ERR_ARG;
+  prof_collect(c);
+  if (launches) *launches = c->timers[kind].launches;
+  if (total_ms) *total_ms = c->timers[kind].ms;
+  return FBSE_OK;
+}
+extern "C" int fbse_set_window(fbse_ctx *c, int w) {
+  if (!c || w < 0 || w > 63) return FBSE_ERR_ARG;
+  c->window = w;
+  return FBSE_OK;
+}
+extern "C" int fbse_dev_alloc(fbse_ctx *c, size_t bytes, void **p) {
+  if (!c || !p) return FBSE_ERR_ARG;
+  hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+  if (e != hipSuccess) return set_err(c, FBSE_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  return FBSE_OK;
+}
+extern "C" int fbse_dev_free(fbse_ctx *c, void *p) {
+  if (!c) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipStreamSynchronize(c->stream));
+  if (p) FBSE_HIP(c, hipFree(p));
+  return FBSE_OK;
+}
+extern "C" int fbse_memcpy_h2d(fbse_ctx *c, void *d, const void *h, size_t b) {
+  if (!c) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, c->stream));
+  FBSE_HIP(c, hipStreamSynchronize(c->stream));
+  return FBSE_OK;
+}
+extern "C" int fbse_memcpy_d2h(fbse_ctx *c, void *h, const void *d, size_t b) {
+  if (!c) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipMemcpyAsync(h, d, b, hipMemcpyDeviceToHost, c->stream));
+  FBSE_HIP(c, hipStreamSynchronize(c->stream));
+  return FBSE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// H application in the reference state layout
+// ---------------------------------------------------------------------------
+static long stlen_of(const fbse_ctx *c) { return c->nsg * (c->is_complex + 1); }
+
+// Max states per H chunk: bounded so the scratch stays below ~8 GiB.
+static long h_chunk(const fbse_ctx *c) {
+  const size_t per = (size_t)c->nsg * sizeof(double2);
+  long ch = (long)((8ull << 30) / (3 * per));
+  return std::max(1L, std::min(ch, 4096L));
+}
+
+static int apply_h_dev_impl(fbse_ctx *c, const double *d_psi, double *d_hpsi, long nstates) {
+  int rc;
+  const long ch = h_chunk(c);
+  const size_t per = (size_t)c->nsg * sizeof(double2);
+  if (c->is_complex) {
+    if ((rc = ensure_scratch(c, (size_t)std::min(ch, nstates) * per))) return rc;
+    for (long s0 = 0; s0 < nstates; s0 += ch) {
+      const long ns = std::min(ch, nstates - s0);
+      const double2 *in = reinterpret_cast<const double2 *>(d_psi) + s0 * c->nsg;
+      double2 *out = reinterpret_cast<double2 *>(d_hpsi) + s0 * c->nsg;
+      if ((rc = apply_h_cplx<double>(c, in, out, (double2 *)c->d_scratch, ns, 1.0, 0.0))) return rc;
+    }
+  } else {
+    const long chp = std::max(2L, ch & ~1L);
+    const long np = (std::min(chp, nstates) + 1) / 2;
+    if ((rc = ensure_scratch(c, (size_t)np * per * 2))) return rc;
+    if ((rc = ensure_scratch2(c, (size_t)np * per))) return rc;
+    double2 *packed = (double2 *)c->d_scratch;
+    double2 *work = packed + np * c->nsg;
+    double2 *res = (double2 *)c->d_scratch2;
+    for (long s0 = 0; s0 < nstates; s0 += chp) {
+      const long ns = std::min(chp, nstates - s0);
+      const long nps = (ns + 1) / 2;
+      if ((rc = pack_real(c, d_psi + s0 * c->nsg, c->nsg, ns, packed))) return rc;
+      if ((rc = apply_h_cplx<double>(c, packed, res, work, nps, 1.0, 0.0))) return rc;
+      if ((rc = unpack_real(c, res, ns, d_hpsi + s0 * c->nsg, c->nsg))) return rc;
+    }
+  }
+  return FBSE_OK;
+}
+
+extern "C" int fbse_apply_h_dev(fbse_ctx *c, const double *d_psi, double *d_hpsi, long nstates) {
+  if (!c || nstates < 0 || (nstates && (!d_psi || !d_hpsi))) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipSetDevice(c->device));
+  return apply_h_dev_impl(c, d_psi, d_hpsi, nstates);
+}
+
+// host-pointer wrapper: stage through device buffers in chunks
+template <typename F>
+static int host_staged(fbse_ctx *c, long nstates, long per_state_doubles, F &&fn) {
+  const long ch = std::max(1L, std::min(nstates, h_chunk(c)));
+  double *d_a = nullptr, *d_b = nullptr;
+  const size_t bytes = (size_t)ch * per_state_doubles * sizeof(double);
+  FBSE_HIP(c, hipMalloc(&d_a, bytes));
+  if (hipMalloc(&d_b, bytes) != hipSuccess) {
+    hipFree(d_a);
+    return set_err(c, FBSE_ERR_NOMEM, "staging buffer");
+  }
+  int rc = FBSE_OK;
+  for (long s0 = 0; s0 < nstates && rc == FBSE_OK; s0 += ch) {
+    const long ns = std::min(ch, nstates - s0);
+    rc = fn(s0, ns, d_a, d_b);
+  }
+  hipStreamSynchronize(c->stream);
+  hipFree(d_a);
+  hipFree(d_b);
+  return rc;
+}
+
+extern "C" int fbse_apply_h(fbse_ctx *c, const double *psi, double *hpsi, long nstates) {
+  if (!c || nstates < 0 || (nstates && (!psi || !hpsi))) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipSetDevice(c->device));
+  const long sl = stlen_of(c);
+  return host_staged(c, nstates, sl, [&](long s0, long ns, double *da, double *db) -> int {
+    FBSE_HIP(c, hipMemcpyAsync(da, psi + s0 * sl, ns * sl * 8, hipMemcpyHostToDevice, c->stream));
+    int rc = apply_h_dev_impl(c, da, db, ns);
+    if (rc) return rc;
+    FBSE_HIP(c, hipMemcpyAsync(hpsi + s0 * sl, db, ns * sl * 8, hipMemcpyDeviceToHost, c->stream));
+    FBSE_HIP(c, hipStreamSynchronize(c->stream));
+    return FBSE_OK;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// normalize_all / energy_all / sigma
+// ---------------------------------------------------------------------------
+static int normalize_dev_impl(fbse_ctx *c, double *d_psi, long n) {
+  if (n == 0) return FBSE_OK;
+  const long sl = stlen_of(c);
+  int rc;
+  double *d_n = nullptr;
+  FBSE_HIP(c, hipMalloc(&d_n, n * sizeof(double)));
+  if ((rc = state_norms(c, d_psi, n, sl, d_n))) {
+    hipFree(d_n);
+    return rc;
+  }
+  std::vector<double> h(n);
+  FBSE_HIP(c, hipMemcpyAsync(h.data(), d_n, n * 8, hipMemcpyDeviceToHost, c->stream));
+  FBSE_HIP(c, hipStreamSynchronize(c->stream));
+  for (long s = 0; s < n; ++s) {
+    double norm = h[s] * c->dv;
+    if (norm == 0.0) {
+      hipFree(d_n);
+      return set_err(c, FBSE_ERR_NORM_ZERO, "Norm is %.2g! exiting! (state %ld)", norm, s);
+    }
+    if (std::isnan(norm)) {
+      hipFree(d_n);
+      return set_err(c, FBSE_ERR_NORM_NAN, "Norm is NAN! exiting! (state %ld)", s);
+    }
+    h[s] = 1.0 / sqrt(norm);
+  }
+  FBSE_HIP(c, hipMemcpyAsync(d_n, h.data(), n * 8, hipMemcpyHostToDevice, c->stream));
+  rc = state_scale(c, d_psi, n, sl, d_n);
+  hipStreamSynchronize(c->stream);
+  hipFree(d_n);
+  return rc;
+}
+extern "C" int fbse_normalize_all_dev(fbse_ctx *c, double *d_psi, long n) {
+  if (!c || n < 0) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipSetDevice(c->device));
+  return normalize_dev_impl(c, d_psi, n);
+}
+extern "C" int fbse_normalize_all(fbse_ctx *c, double *psi, long n) {
+  if (!c || n < 0) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipSetDevice(c->device));
+  const long sl = stlen_of(c);
+  return host_staged(c, n, sl, [&](long s0, long ns, double *da, double *) -> int {
+    FBSE_HIP(c, hipMemcpyAsync(da, psi + s0 * sl, ns * sl * 8, hipMemcpyHostToDevice, c->stream));
+    int rc = normalize_dev_impl(c, da, ns);
+    if (rc) return rc;
+    FBSE_HIP(c, hipMemcpyAsync(psi + s0 * sl, da, ns * sl * 8, hipMemcpyDeviceToHost, c->stream));
+    FBSE_HIP(c, hipStreamSynchronize(c->stream));
+    return FBSE_OK;
+  });
+}
+
+// energies (and optionally residual norms) of n device states, chunked
+static int energies_dev_impl(fbse_ctx *c, const double *d_psi, long n, double *ene, double *sig) {
+  const long sl = stlen_of(c);
+  const long ch = std::max(1L, std::min(n, h_chunk(c)));
+  double *d_h = nullptr, *d_e = nullptr, *d_r = nullptr;
+  FBSE_HIP(c, hipMalloc(&d_h, (size_t)ch * sl * 8));
+  FBSE_HIP(c, hipMalloc(&d_e, (size_t)n * 8));
+  FBSE_HIP(c, hipMalloc(&d_r, (size_t)n * 8));
+  int rc = FBSE_OK;
+  std::vector<double> he(n), hr(n);
+  for (long s0 = 0; s0 < n && !rc; s0 += ch) {
+    const long ns = std::min(ch, n - s0);
+    rc = apply_h_dev_impl(c, d_psi + s0 * sl, d_h, ns);
+    if (!rc) rc = state_dots(c, d_psi + s0 * sl, d_h, ns, sl, d_e + s0);
+    if (!rc && sig) {
+      // eps = <H> (unscaled dot * dv) on device: scale the dots first
+      std::vector<double> tmp(ns);
+      hipMemcpyAsync(tmp.data(), d_e + s0, ns * 8, hipMemcpyDeviceToHost, c->stream);
+      hipStreamSynchronize(c->stream);
+      for (long s = 0; s < ns; ++s) tmp[s] *= c->dv;
+      hipMemcpyAsync(d_r + s0, tmp.data(), ns * 8, hipMemcpyHostToDevice, c->stream);
+      rc = state_resid(c, d_psi + s0 * sl, d_h, d_r + s0, ns, sl, d_r + s0);
+      if (!rc) {
+        // state_resid reads eps from d_r and overwrites it element-wise after the
+        // ordered final reduction; the final kernel runs after the partial one.
+      }
+    }
+  }
+  if (!rc) {
+    FBSE_HIP(c, hipMemcpyAsync(he.data(), d_e, n * 8, hipMemcpyDeviceToHost, c->stream));
+    if (sig) FBSE_HIP(c, hipMemcpyAsync(hr.data(), d_r, n * 8, hipMemcpyDeviceToHost, c->stream));
+    FBSE_HIP(c, hipStreamSynchronize(c->stream));
+    for (long s = 0; s < n; ++s) {
+      if (ene) ene[s] = he[s] * c->dv;
+      if (sig) sig[s] = sqrt(fabs(hr[s] * c->dv));
+    }
+  }
+  hipStreamSynchronize(c->stream);
+  hipFree(d_h);
+  hipFree(d_e);
+  hipFree(d_r);
+  return rc;
+}
+extern "C" int fbse_energy_all_dev(fbse_ctx *c, const double *d_psi, long n, double *ene) {
+  if (!c || n < 0 || !ene) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipSetDevice(c->device));
+  return energies_dev_impl(c, d_psi, n, ene, nullptr);
+}
+extern "C" int fbse_sigma_dev(fbse_ctx *c, const double *d_psi, long n, double *sigma, double *eval) {
+  if (!c || n < 0 || !sigma) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipSetDevice(c->device));
+  std::vector<double> e(n);
+  int rc = energies_dev_impl(c, d_psi, n, e.data(), sigma);
+  if (!rc && eval) memcpy(eval, e.data(), n * 8);
+  return rc;
+}
+template <typename F> static int with_device_copy(fbse_ctx *c, const double *h, long n, F &&fn) {
+  const long sl = stlen_of(c);
+  double *d = nullptr;
+  FBSE_HIP(c, hipMalloc(&d, (size_t)std::max(1L, n) * sl * 8));
+  hipError_t e = hipMemcpyAsync(d, h, (size_t)n * sl * 8, hipMemcpyHostToDevice, c->stream);
+  int rc = e == hipSuccess ? fn(d) : set_err(c, FBSE_ERR_HIP, "copy: %s", hipGetErrorString(e));
+  hipStreamSynchronize(c->stream);
+  hipFree(d);
+  return rc;
+}
+extern "C" int fbse_energy_all(fbse_ctx *c, const double *psi, long n, double *ene) {
+  if (!c || n < 0 || !ene) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipSetDevice(c->device));
+  return with_device_copy(c, psi, n, [&](double *d) { return energies_dev_impl(c, d, n, ene, nullptr); });
+}
+extern "C" int fbse_sigma(fbse_ctx *c, const double *psi, long n, double *sigma, double *eval) {
+  if (!c || n < 0 || !sigma) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipSetDevice(c->device));
+  return with_device_copy(c, psi, n, [&](double *d) { return fbse_sigma_dev(c, d, n, sigma, eval); });
+}
+
+// ---------------------------------------------------------------------------
+// Newton filter
+// ---------------------------------------------------------------------------
+template <typename T>
+static int filter_impl(fbse_ctx *c, double *d_psi_rank, long n_starts, long ms, long M, const double *an,
+                       const double *zn, double Vmin, double dE_1) {
+  using C = cplx_t<T>;
+  const int packed = !c->is_complex;
+  const long n_items = packed ? (n_starts + 1) / 2 : n_starts;
+  const long nsg = c->nsg;
+  const long sl = stlen_of(c);
+  const long slot_elems = n_items * nsg;
+  const size_t slot_bytes = (size_t)slot_elems * sizeof(C);
+  int rc;
+  // window depth: forced, or as deep as ~40% of free memory allows (<= 32)
+  int K = c->window;
+  if (K <= 0) {
+    size_t fr = 0, tot = 0;
+    hipMemGetInfo(&fr, &tot);
+    long kk = (long)((0.4 * (double)fr) / (double)slot_bytes) - 2;
+    K = (int)std::max(1L, std::min(32L, kk));
+  }
+  if (K > 63) K = 63;
+  const int R = K + 1;
+  // ring (R slots) + one work slot + the double-precision staging for loading
+  if ((rc = ensure_scratch(c, slot_bytes * (size_t)(R + 1)))) return rc;
+  C *ring = (C *)c->d_scratch;
+  C *work = ring + (size_t)R * slot_elems;
+  double *d_an = nullptr;
+  FBSE_HIP(c, hipMalloc(&d_an, (size_t)ms * M * 2 * sizeof(double)));
+  FBSE_HIP(c, hipMemcpyAsync(d_an, an, (size_t)ms * M * 2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+
+  // start states -> ring slot 0 (packed pairs for real states)
+  {
+    double2 *stage;
+    if constexpr (sizeof(T) == 8) stage = (double2 *)ring;
+    else {
+      if ((rc = ensure_scratch2(c, (size_t)slot_elems * sizeof(double2)))) return rc;
+      stage = (double2 *)c->d_scratch2;
+    }
+    if (packed) {
+      if ((rc = pack_real(c, d_psi_rank, ms * sl, n_starts, stage))) return rc;
+    } else {
+      FBSE_HIP(c, hipMemcpy2DAsync(stage, nsg * sizeof(double2), d_psi_rank, ms * sl * sizeof(double),
+                                   nsg * sizeof(double2), n_starts, hipMemcpyDeviceToDevice, c->stream));
+    }
+    if constexpr (sizeof(T) == 4) {
+      if ((rc = to_f32(c, stage, (float2 *)ring, slot_elems))) return rc;
+    }
+  }
+  if ((rc = fill_zero(c, d_psi_rank, (size_t)n_starts * ms * sl * sizeof(double)))) return rc;
+
+  long acc_lo = 0;
+  const double c2 = Vmin * dE_1;
+  for (long j = 1; j < M; ++j) {
+    const C *in = ring + (size_t)((j - 1) % R) * slot_elems;
+    C *out = ring + (size_t)(j % R) * slot_elems;
+    const double a0 = -(2.0 + zn[j - 1] + c2);
+    if ((rc = apply_h_cplx<T>(c, in, out, work, n_items, (T)dE_1, (T)a0))) break;
+    if (j + 1 - acc_lo == R || j == M - 1) {
+      if ((rc = accumulate_window<T>(c, ring, R, slot_elems, acc_lo, j + 1, d_an, ms, M, d_psi_rank, n_starts,
+                                     n_items, packed)))
+        break;
+      acc_lo = j + 1;
+    }
+  }
+  if (!rc && M == 1)
+    rc = accumulate_window<T>(c, ring, R, slot_elems, 0, 1, d_an, ms, M, d_psi_rank, n_starts, n_items, packed);
+  hipStreamSynchronize(c->stream);
+  hipFree(d_an);
+  if (!rc) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = set_err(c, FBSE_ERR_HIP, "filter: %s", hipGetErrorString(e));
+  }
+  return rc;
+}
+
+extern "C" int fbse_filter_dev(fbse_ctx *c, double *d_psi_rank, long n_starts, long ms, long M, const double *an,
+                               const double *zn, double Vmin, double dE_1, int precision) {
+  if (!c || n_starts < 0 || ms < 1 || M < 1 || !an || (M > 1 && !zn)) return FBSE_ERR_ARG;
+  if (ms > 16) return set_err(c, FBSE_ERR_UNSUPPORTED, "ms = %ld > 16 energy targets per start", ms);
+  if (n_starts == 0) return FBSE_OK;
+  FBSE_HIP(c, hipSetDevice(c->device));
+  if (precision == FBSE_PREC_FP64) return filter_impl<double>(c, d_psi_rank, n_starts, ms, M, an, zn, Vmin, dE_1);
+  if (precision == FBSE_PREC_MIXED) return filter_impl<float>(c, d_psi_rank, n_starts, ms, M, an, zn, Vmin, dE_1);
+  return FBSE_ERR_ARG;
+}
+extern "C" int fbse_filter(fbse_ctx *c, double *psi_rank, long n_starts, long ms, long M, const double *an,
+                           const double *zn, double Vmin, double dE_1, int precision) {
+  if (!c || !psi_rank) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipSetDevice(c->device));
+  const size_t bytes = (size_t)n_starts * ms * stlen_of(c) * sizeof(double);
+  double *d = nullptr;
+  FBSE_HIP(c, hipMalloc(&d, bytes ? bytes : 16));
+  int rc = FBSE_OK;
+  hipError_t e = hipMemcpyAsync(d, psi_rank, bytes, hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) rc = set_err(c, FBSE_ERR_HIP, "copy: %s", hipGetErrorString(e));
+  if (!rc) rc = fbse_filter_dev(c, d, n_starts, ms, M, an, zn, Vmin, dE_1, precision);
+  if (!rc) {
+    e = hipMemcpyAsync(psi_rank, d, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e != hipSuccess) rc = set_err(c, FBSE_ERR_HIP, "copy: %s", hipGetErrorString(e));
+  }
+  hipStreamSynchronize(c->stream);
+  hipFree(d);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Orthogonalisation and Rayleigh-Ritz (FP64, rocBLAS GEMMs + rocSOLVER)
+// ---------------------------------------------------------------------------
+struct LinAlg {
+  fbse_ctx *c;
+  bool cplx;
+  long m;   // rows (complex elements or reals)
+  size_t es() const { return cplx ? 16 : 8; }
+  // C(n1 x n2) = A^H B   with A (m x n1), B (m x n2), lda = ldb = m
+  int gram(const double *A, long n1, const double *B, long n2, double *Cm, long ldc) {
+    if (cplx) {
+      const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
+      FBSE_BLAS(c, rocblas_zgemm(c->blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, (int)n1,
+                                 (int)n2, (int)m, &one, (const rocblas_double_complex *)A, (int)m,
+                                 (const rocblas_double_complex *)B, (int)m, &zero, (rocblas_double_complex *)Cm,
+                                 (int)ldc));
+    } else {
+      const double one = 1.0, zero = 0.0;
+      FBSE_BLAS(c, rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, (int)n1, (int)n2,
+                                 (int)m, &one, A, (int)m, B, (int)m, &zero, Cm, (int)ldc));
+    }
+    return FBSE_OK;
+  }
+  // A[:, 0:nout] = A[:, 0:nin] * V (nin x nout, ldv), in place by row blocks
+  int rotate(double *A, long nin, const double *V, long ldv, long nout) {
+    long rb = 32768;
+    while (rb > 1024 && (size_t)rb * nout * es() > (1ull << 30)) rb /= 2;
+    int rc = ensure_scratch2(c, (size_t)rb * nout * es());
+    if (rc) return rc;
+    for (long r0 = 0; r0 < m; r0 += rb) {
+      const long nr = std::min(rb, m - r0);
+      if (cplx) {
+        const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
+        FBSE_BLAS(c, rocblas_zgemm(c->blas, rocblas_operation_none, rocblas_operation_none, (int)nr, (int)nout,
+                                   (int)nin, &one, (const rocblas_double_complex *)A + r0, (int)m,
+                                   (const rocblas_double_complex *)V, (int)ldv, &zero,
+                                   (rocblas_double_complex *)c->d_scratch2, (int)nr));
+      } else {
+        const double one = 1.0, zero = 0.0;
+        FBSE_BLAS(c, rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, (int)nr, (int)nout,
+                                   (int)nin, &one, A + r0, (int)m, V, (int)ldv, &zero, (double *)c->d_scratch2,
+                                   (int)nr));
+      }
+      char *dst = (char *)A + (size_t)r0 * es();
+      FBSE_HIP(c, hipMemcpy2DAsync(dst, (size_t)m * es(), c->d_scratch2, (size_t)nr * es(), (size_t)nr * es(), nout,
+                                   hipMemcpyDeviceToDevice, c->stream));
+    }
+    return FBSE_OK;
+  }
+  // eigen-decomposition of the Hermitian n x n matrix H (in place -> vectors)
+  int eigh(double *H, long n, double *d_w) {
+    double *d_e = nullptr;
+    int *d_info = nullptr;
+    FBSE_HIP(c, hipMalloc(&d_e, n * 8));
+    FBSE_HIP(c, hipMalloc(&d_info, sizeof(int)));
+    rocblas_status st;
+    if (cplx)
+      st = rocsolver_zheevd(c->blas, rocblas_evect_original, rocblas_fill_upper, (int)n, (rocblas_double_complex *)H,
+                            (int)n, d_w, d_e, d_info);
+    else
+      st = rocsolver_dsyevd(c->blas, rocblas_evect_original, rocblas_fill_upper, (int)n, H, (int)n, d_w, d_e, d_info);
+    int info = 0;
+    hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    hipStreamSynchronize(c->stream);
+    hipFree(d_e);
+    hipFree(d_info);
+    if (st != rocblas_status_success || info != 0)
+      return set_err(c, FBSE_ERR_SOLVER, "eigensolver failed: status %d info %d (n=%ld)", (int)st, info, n);
+    return FBSE_OK;
+  }
+};
+
+__global__ void hermitize_kernel(double *H, long n, int cplx) {
+  // H <- (H + H^H)/2, column-major
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n * n; idx += (long)gridDim.x * blockDim.x) {
+    const long i = idx % n, j = idx / n;
+    if (i > j) continue;
+    if (cplx) {
+      double2 *A = (double2 *)H;
+      double2 a = A[i + j * n], b = A[j + i * n];
+      double2 r = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+      A[i + j * n] = r;
+      A[j + i * n] = make_double2(r.x, -r.y);
+    } else {
+      const double r = 0.5 * (H[i + j * n] + H[j + i * n]);
+      H[i + j * n] = r;
+      H[j + i * n] = r;
+    }
+  }
+}
+
+// Z[:, k] = W[:, src_k] / sig_k for the selected columns (column-major, n rows)
+__global__ void select_scale_kernel(const double *W, long n, const int *src, const double *inv, long k, double *Z,
+                                    int cplx) {
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n * k; idx += (long)gridDim.x * blockDim.x) {
+    const long i = idx % n, j = idx / n;
+    if (cplx) {
+      double2 v = ((const double2 *)W)[i + (long)src[j] * n];
+      ((double2 *)Z)[i + j * n] = make_double2(v.x * inv[j], v.y * inv[j]);
+    } else {
+      Z[i + j * n] = W[i + (long)src[j] * n] * inv[j];
+    }
+  }
+}
+
+static int ortho_dev_impl(fbse_ctx *c, double *A, long n, double svd_eps, long *rank, double *sing) {
+  LinAlg la{c, c->is_complex != 0, c->nsg};
+  const size_t es = la.es();
+  int rc;
+  double *G = nullptr, *w = nullptr, *Z = nullptr;
+  int *d_src = nullptr;
+  double *d_inv = nullptr;
+  FBSE_HIP(c, hipMalloc(&G, (size_t)n * n * es));
+  FBSE_HIP(c, hipMalloc(&Z, (size_t)n * n * es));
+  FBSE_HIP(c, hipMalloc(&w, (size_t)n * 8));
+  FBSE_HIP(c, hipMalloc(&d_src, (size_t)n * sizeof(int)));
+  FBSE_HIP(c, hipMalloc(&d_inv, (size_t)n * 8));
+  std::vector<double> hw(n);
+  std::vector<int> src;
+  std::vector<double> inv;
+  long k = 0;
+  auto cleanup = [&]() {
+    hipStreamSynchronize(c->stream);
+    hipFree(G);
+    hipFree(Z);
+    hipFree(w);
+    hipFree(d_src);
+    hipFree(d_inv);
+  };
+  // pass 1: G = A^H A -> V ; B = A V
+  if ((rc = la.gram(A, n, A, n, G, n)) || (rc = la.eigh(G, n, w)) || (rc = la.rotate(A, n, G, n, n))) {
+    cleanup();
+    return rc;
+  }
+  // pass 2: G2 = B^H B (small singular values now resolved relative to themselves)
+  if ((rc = la.gram(A, n, A, n, G, n))) {
+    cleanup();
+    return rc;
+  }
+  hipLaunchKernelGGL(hermitize_kernel, dim3(256), dim3(256), 0, c->stream, G, n, la.cplx ? 1 : 0);
+  if ((rc = la.eigh(G, n, w))) {
+    cleanup();
+    return rc;
+  }
+  FBSE_HIP(c, hipMemcpyAsync(hw.data(), w, n * 8, hipMemcpyDeviceToHost, c->stream));
+  FBSE_HIP(c, hipStreamSynchronize(c->stream));
+  // singular values, descending; rank cut as ortho.c:68-75
+  std::vector<double> sv(n);
+  for (long i = 0; i < n; ++i) sv[i] = sqrt(std::max(0.0, hw[n - 1 - i]));
+  if (sing)
+    for (long i = 0; i < n; ++i) sing[i] = sv[i];
+  k = n;
+  for (long i = 0; i < n; ++i)
+    if (sv[0] == 0.0 || (sv[i] / sv[0]) < svd_eps) {
+      k = i;
+      break;
+    }
+  *rank = k;
+  if (k == 0) {
+    cleanup();
+    return set_err(c, FBSE_ERR_SOLVER, "ortho: all singular values vanish");
+  }
+  src.resize(k);
+  inv.resize(k);
+  for (long i = 0; i < k; ++i) {
+    src[i] = (int)(n - 1 - i);
+    inv[i] = 1.0 / sv[i];
+  }
+  FBSE_HIP(c, hipMemcpyAsync(d_src, src.data(), k * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  FBSE_HIP(c, hipMemcpyAsync(d_inv, inv.data(), k * 8, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(select_scale_kernel, dim3(256), dim3(256), 0, c->stream, G, n, d_src, d_inv, k, Z,
+                     la.cplx ? 1 : 0);
+  if ((rc = la.rotate(A, n, Z, n, k))) {
+    cleanup();
+    return rc;
+  }
+  // pass 3: Cholesky re-orthogonalisation U <- U R^-1 with U^H U = R^H R
+  if ((rc = la.gram(A, k, A, k, G, k))) {
+    cleanup();
+    return rc;
+  }
+  {
+    int *d_info = nullptr;
+    FBSE_HIP(c, hipMalloc(&d_info, sizeof(int)));
+    rocblas_status st;
+    if (la.cplx)
+      st = rocsolver_zpotrf(c->blas, rocblas_fill_upper, (int)k, (rocblas_double_complex *)G, (int)k, d_info);
+    else
+      st = rocsolver_dpotrf(c->blas, rocblas_fill_upper, (int)k, G, (int)k, d_info);
+    int info = 0;
+    hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    hipStreamSynchronize(c->stream);
+    hipFree(d_info);
+    if (st != rocblas_status_success || info != 0) {
+      cleanup();
+      return set_err(c, FBSE_ERR_SOLVER, "ortho: Cholesky re-orthogonalisation failed (info %d)", info);
+    }
+    if (la.cplx) {
+      const rocblas_double_complex one = {1.0, 0.0};
+      st = rocblas_ztrsm(c->blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                         rocblas_diagonal_non_unit, (int)la.m, (int)k, &one, (const rocblas_double_complex *)G, (int)k,
+                         (rocblas_double_complex *)A, (int)la.m);
+    } else {
+      const double one = 1.0;
+      st = rocblas_dtrsm(c->blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                         rocblas_diagonal_non_unit, (int)la.m, (int)k, &one, G, (int)k, A, (int)la.m);
+    }
+    if (st != rocblas_status_success) {
+      cleanup();
+      return set_err(c, FBSE_ERR_SOLVER, "ortho: trsm status %d", (int)st);
+    }
+  }
+  cleanup();
+  return normalize_dev_impl(c, A, k);
+}
+
+extern "C" int fbse_ortho_dev(fbse_ctx *c, double *d_psitot, long n_cols, double svd_eps, long *rank,
+                              double *sing) {
+  if (!c || !d_psitot || n_cols < 1 || !rank) return FBSE_ERR_ARG;
+  if (c->nsg > 0x7fffffffL) return FBSE_ERR_UNSUPPORTED;
+  FBSE_HIP(c, hipSetDevice(c->device));
+  return ortho_dev_impl(c, d_psitot, n_cols, svd_eps, rank, sing);
+}
+extern "C" int fbse_ortho(fbse_ctx *c, double *psitot, long n_cols, double svd_eps, long *rank, double *sing) {
+  if (!c || !psitot || n_cols < 1 || !rank) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipSetDevice(c->device));
+  const long sl = stlen_of(c);
+  return with_device_copy(c, psitot, n_cols, [&](double *d) -> int {
+    int rc = ortho_dev_impl(c, d, n_cols, svd_eps, rank, sing);
+    if (rc) return rc;
+    FBSE_HIP(c, hipMemcpyAsync(psitot, d, (size_t)n_cols * sl * 8, hipMemcpyDeviceToHost, c->stream));
+    FBSE_HIP(c, hipStreamSynchronize(c->stream));
+    return FBSE_OK;
+  });
+}
+
+static int rr_dev_impl(fbse_ctx *c, double *U, long n, double *eval) {
+  LinAlg la{c, c->is_complex != 0, c->nsg};
+  const long sl = stlen_of(c);
+  const size_t es = la.es();
+  int rc = FBSE_OK;
+  double *Hs = nullptr, *w = nullptr, *HU = nullptr;
+  const long ch = std::max(1L, std::min(n, h_chunk(c)));
+  FBSE_HIP(c, hipMalloc(&Hs, (size_t)n * n * es));
+  FBSE_HIP(c, hipMalloc(&w, (size_t)n * 8));
+  FBSE_HIP(c, hipMalloc(&HU, (size_t)ch * sl * 8));
+  for (long c0 = 0; c0 < n && !rc; c0 += ch) {
+    const long nc = std::min(ch, n - c0);
+    rc = apply_h_dev_impl(c, U + c0 * sl, HU, nc);
+    if (!rc) rc = la.gram(U, n, HU, nc, (double *)((char *)Hs + (size_t)c0 * n * es), n);
+  }
+  if (!rc) {
+    hipLaunchKernelGGL(hermitize_kernel, dim3(256), dim3(256), 0, c->stream, Hs, n, la.cplx ? 1 : 0);
+    rc = la.eigh(Hs, n, w);
+  }
+  if (!rc) rc = la.rotate(U, n, Hs, n, n);
+  if (!rc) {
+    std::vector<double> hw(n);
+    hipMemcpyAsync(hw.data(), w, n * 8, hipMemcpyDeviceToHost, c->stream);
+    hipStreamSynchronize(c->stream);
+    for (long i = 0; i < n; ++i) eval[i] = hw[i] * c->dv;
+  }
+  hipStreamSynchronize(c->stream);
+  hipFree(Hs);
+  hipFree(w);
+  hipFree(HU);
+  if (rc) return rc;
+  return normalize_dev_impl(c, U, n);
+}
+extern "C" int fbse_rayleigh_ritz_dev(fbse_ctx *c, double *d_psitot, long n, double *eval) {
+  if (!c || !d_psitot || n < 1 || !eval) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipSetDevice(c->device));
+  return rr_dev_impl(c, d_psitot, n, eval);
+}
+extern "C" int fbse_rayleigh_ritz(fbse_ctx *c, double *psitot, long n, double *eval) {
+  if (!c || !psitot || n < 1 || !eval) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipSetDevice(c->device));
+  const long sl = stlen_of(c);
+  return with_device_copy(c, psitot, n, [&](double *d) -> int {
+    int rc = rr_dev_impl(c, d, n, eval);
+    if (rc) return rc;
+    FBSE_HIP(c, hipMemcpyAsync(psitot, d, (size_t)n * sl * 8, hipMemcpyDeviceToHost, c->stream));
+    FBSE_HIP(c, hipStreamSynchronize(c->stream));
+    return FBSE_OK;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Spectrum bounds (energy.c:222-300 and 343-369)
+// ---------------------------------------------------------------------------
+extern "C" int fbse_energy_range(fbse_ctx *c, const double *psi0_min, const double *psi0_max, long max_iter,
+                                 double nl_lambda_min, double vloc_min, double *out) {
+  if (!c || !psi0_min || !psi0_max || !out || max_iter < 0) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipSetDevice(c->device));
+  const long sl = stlen_of(c);
+  double *psi = nullptr, *phi = nullptr, *d_s = nullptr;
+  FBSE_HIP(c, hipMalloc(&psi, sl * 8));
+  FBSE_HIP(c, hipMalloc(&phi, sl * 8));
+  FBSE_HIP(c, hipMalloc(&d_s, 2 * 8));
+  int rc = FBSE_OK;
+  double s2[2];
+  auto dot = [&](const double *a, const double *b, double *res) -> int {
+    int r = b ? state_dots(c, a, b, 1, sl, d_s) : state_norms(c, a, 1, sl, d_s);
+    if (r) return r;
+    FBSE_HIP(c, hipMemcpyAsync(s2, d_s, 8, hipMemcpyDeviceToHost, c->stream));
+    FBSE_HIP(c, hipStreamSynchronize(c->stream));
+    *res = s2[0];
+    return FBSE_OK;
+  };
+  auto normalize = [&](double *a) -> int {
+    double nn;
+    int r = dot(a, nullptr, &nn);
+    if (r) return r;
+    const double N = sqrt(nn * c->dv);
+    return axpby_states(c, a, a, sl, 0.0, 1.0 / N);
+  };
+  double tau = 0.05, ene_old, Emin, Emax;
+  long i = 0, imin = 0;
+  FBSE_HIP(c, hipMemcpyAsync(phi, psi0_min, sl * 8, hipMemcpyHostToDevice, c->stream));
+  Emin = (ene_old = 0.0) + 10.0;
+  for (i = 0; !rc && (fabs((Emin - ene_old) / Emin) > 1.0e-6) && (i < max_iter); i++) {
+    // psi <- phi ; phi <- H psi ; phi <- psi - tau phi ; normalise ; energy
+    rc = apply_h_dev_impl(c, phi, psi, 1);  // psi = H phi_old
+    if (!rc) rc = axpby_states(c, phi, psi, sl, 1.0, -tau);  // psi = phi_old - tau H phi_old
+    if (!rc) {
+      double *t = phi;
+      phi = psi;
+      psi = t;
+      rc = normalize(phi);
+    }
+    if (!rc) rc = apply_h_dev_impl(c, phi, psi, 1);
+    double e = 0.0;
+    if (!rc) rc = dot(phi, psi, &e);
+    ene_old = Emin;
+    Emin = e * c->dv;
+    if ((i > 5) && (Emin - ene_old) > 0) tau -= 0.025;
+  }
+  imin = i;
+  if (!rc) {
+    FBSE_HIP(c, hipMemcpyAsync(psi, psi0_max, sl * 8, hipMemcpyHostToDevice, c->stream));
+    Emax = (ene_old = 0.0) + 0.1;
+    for (i = 0; !rc && (fabs((Emax - ene_old) / Emax) > 1.0e-6) && (i < max_iter); i++) {
+      rc = apply_h_dev_impl(c, psi, phi, 1);  // phi = H psi
+      if (!rc) {
+        double *t = psi;
+        psi = phi;
+        phi = t;
+        rc = normalize(psi);
+      }
+      if (!rc) rc = apply_h_dev_impl(c, psi, phi, 1);
+      double e = 0.0;
+      if (!rc) rc = dot(psi, phi, &e);
+      ene_old = Emax;
+      Emax = e * c->dv;
+    }
+  }
+  if (!rc) {
+    out[0] = Emin;
+    out[1] = Emax;
+    if (c->has_proj && c->pj.nl && !c->pj.so && nl_lambda_min < 0.0) {
+      Emax *= 1.05;
+      Emin -= 0.1 * fabs(Emin);
+      const double weyl_min = 1.05 * (vloc_min + nl_lambda_min);
+      if (Emin > weyl_min) Emin = weyl_min;
+    } else {
+      Emax *= 1.2;
+      Emin -= 0.2 * fabs(Emin);
+    }
+    out[2] = Emin;
+    out[3] = Emax - Emin;
+    out[4] = (double)imin;
+    out[5] = (double)i;
+  }
+  hipStreamSynchronize(c->stream);
+  hipFree(psi);
+  hipFree(phi);
+  hipFree(d_s);
+  return rc;
+}

@@ -1,0 +1,208 @@
+// fbse_internal.h -- shared definitions of the gfx950 filter-diagonalization
+// library: context, device-side tables and kernel argument blocks.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fbse_gpu.h"
+
+namespace fbse {
+
+template <typename T> struct cplx_of;
+template <> struct cplx_of<double> { using type = double2; };
+template <> struct cplx_of<float> { using type = float2; };
+template <typename T> using cplx_t = typename cplx_of<T>::type;
+
+constexpr int kMaxFactors = 16;
+constexpr int kFftThreads = 256;
+constexpr int kProjRows = 30;     // 15 SO rows (5 radial x 3 m) + 15 NL rows
+constexpr int kProjCols = 16;     // columns (state x spin) per projector work-group
+constexpr int kProjPts = 64;      // grid points staged in LDS per projector tile
+
+// One 1D transform length with its radix schedule and tile shape.
+struct AxisPlan {
+  int n = 0;
+  int nf = 0;
+  int f[kMaxFactors] = {};
+  int lines = 0;       // lines per work-group tile
+};
+
+// Argument block of the batched axis FFT kernel (fft_kernels.hip).
+template <typename T> struct AxisArgs {
+  const cplx_t<T> *src;
+  cplx_t<T> *dst;
+  const cplx_t<T> *tw;     // exp(-2 pi i t / n), t < n
+  int n, nf;
+  int f[kMaxFactors];
+  int lines;               // lines per tile
+  int xaxis;               // 1: lines are contiguous rows
+  long inner;              // element stride between axis points (y: nx, z: nx*ny)
+  long inner_count;        // distinct inner indices per outer block
+  long outer_stride;       // elements between outer blocks (n*inner)
+  long tiles_per_outer;
+  long total_lines;        // x axis: rows in the batch
+  // kinetic scaling (z pass): k2 = min(kx[jx] + ky[jy] + kz[jz], kcap) when the
+  // table is separable, else ksqr[jz*nx*ny + inner]
+  const T *kx, *ky, *kz, *ksqr;
+  T kcap;
+  int nx;
+  // combine (x backward pass): dst = a1*(T psi + V psi_in) + a0*psi_in
+  const cplx_t<T> *psi_in;
+  const T *vloc;
+  long vloc_rows;          // rows per V period (nspin*nz*ny)
+  T a1, a0;
+};
+
+enum PassKind { kPassFwd = 0, kPassBwd = 1, kPassKin = 2, kPassBwdCombine = 3 };
+
+// Device projector tables (compact per-atom storage).
+template <typename T> struct ProjTables {
+  int n_atoms = 0, npts = 0;
+  int so = 0, nl = 0, n_so = 0, n_nl = 0;
+  int *a_off = nullptr, *a_cnt = nullptr;
+  int *jxyz = nullptr;
+  T *proj = nullptr;        // [5][npts]
+  T *nlproj = nullptr;      // [5][npts]
+  cplx_t<T> *y1 = nullptr;  // [3][npts]
+  int *sign = nullptr;      // [n_atoms*5]
+  cplx_t<T> *ls = nullptr;  // 36
+};
+
+template <typename T> struct ProjArgs {
+  int n_atoms, npts, so, nl, n_so, n_nl;
+  const int *a_off, *a_cnt, *jxyz;
+  const T *proj, *nlproj;
+  const cplx_t<T> *y1;
+  const int *sign;
+  const cplx_t<T> *ls;
+  const cplx_t<T> *psi_in;  // columns: state*nsg + spin*ngrid
+  cplx_t<T> *psi_out;
+  long ngrid, nsg;
+  int nspin, ncols;
+  T scale;                  // dv * a1
+};
+
+struct Timer {
+  long launches = 0;
+  double ms = 0.0;
+};
+
+}  // namespace fbse
+
+struct fbse_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  rocblas_handle blas = nullptr;
+  std::string err;
+
+  long nx = 0, ny = 0, nz = 0, ngrid = 0, nspin = 1, nsg = 0;
+  int is_complex = 0;
+  double dv = 0.0;
+  int ksqr_sep = 0;
+  double kcap = 0.0;
+
+  // local part of H (FP64 and FP32 copies)
+  double *d_vloc = nullptr;
+  float *d_vloc_f = nullptr;
+  double *d_kx = nullptr, *d_ky = nullptr, *d_kz = nullptr, *d_ksqr = nullptr;
+  float *d_kx_f = nullptr, *d_ky_f = nullptr, *d_kz_f = nullptr, *d_ksqr_f = nullptr;
+
+  // twiddles per axis
+  fbse::AxisPlan plan[3];
+  double2 *d_tw[3] = {nullptr, nullptr, nullptr};
+  float2 *d_tw_f[3] = {nullptr, nullptr, nullptr};
+
+  // projectors
+  fbse::ProjTables<double> pj;
+  fbse::ProjTables<float> pj_f;
+  int has_proj = 0;
+
+  // scratch (grown on demand)
+  void *d_scratch = nullptr;
+  size_t scratch_bytes = 0;
+  void *d_scratch2 = nullptr;
+  size_t scratch2_bytes = 0;
+  double *d_red = nullptr;      // reductions
+  size_t red_bytes = 0;
+  double *h_red = nullptr;      // pinned host mirror
+  size_t h_red_bytes = 0;
+
+  int window = 0;               // forced window depth (0 = auto)
+
+  // profiling
+  int profile = 0;
+  fbse::Timer timers[5];
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+};
+
+namespace fbse {
+
+// error helpers
+int set_err(fbse_ctx *c, int code, const char *fmt, ...);
+#define FBSE_HIP(c, call)                                                                  \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return ::fbse::set_err((c), FBSE_ERR_HIP, "%s failed: %s (%s:%d)", #call,            \
+                             hipGetErrorString(e_), __FILE__, __LINE__);                   \
+  } while (0)
+#define FBSE_BLAS(c, call)                                                                 \
+  do {                                                                                     \
+    rocblas_status s_ = (call);                                                            \
+    if (s_ != rocblas_status_success)                                                      \
+      return ::fbse::set_err((c), FBSE_ERR_SOLVER, "%s failed: status %d (%s:%d)", #call,  \
+                             (int)s_, __FILE__, __LINE__);                                 \
+  } while (0)
+
+int ensure_scratch(fbse_ctx *c, size_t bytes);
+int ensure_scratch2(fbse_ctx *c, size_t bytes);
+int ensure_red(fbse_ctx *c, size_t bytes);
+
+// profiling brackets (no-ops unless profiling is enabled)
+void prof_begin(fbse_ctx *c, int kind, hipEvent_t *ev);
+void prof_end(fbse_ctx *c, int kind, hipEvent_t ev);
+void prof_collect(fbse_ctx *c);
+
+// FFT launchers (fft_kernels.hip).  buf holds nbatch = (states*nspin) grids.
+template <typename T>
+int fft_pass(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src, cplx_t<T> *dst, long nbatch,
+             const cplx_t<T> *psi_in, T a1, T a0);
+// Full H (or Newton step) for nstates complex states: out = a1*H psi + a0*psi.
+// work must hold nstates*nsg complex values.
+template <typename T>
+int apply_h_cplx(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *work, long nstates, T a1,
+                 T a0);
+
+// projector launcher (ham_kernels.hip): psi_out += scale * Vproj psi_in
+template <typename T>
+int proj_apply(fbse_ctx *c, const cplx_t<T> *psi_in, cplx_t<T> *psi_out, long nstates, double a1);
+
+// element-wise helpers (ham_kernels.hip)
+int pack_real(fbse_ctx *c, const double *src, long src_stride, long nstates, double2 *dst);
+int unpack_real(fbse_ctx *c, const double2 *src, long nstates, double *dst, long dst_stride);
+int to_f32(fbse_ctx *c, const double2 *src, float2 *dst, long n);
+int fill_zero(fbse_ctx *c, void *dst, size_t bytes);
+// per-state sums over nsg complex elements (or real layout): out[s] = sum |a|^2,
+// sum Re(conj(a) b), sum |b - e a|^2 ...
+int state_norms(fbse_ctx *c, const double *psi, long nstates, long elems_per_state, double *d_out);
+int state_dots(fbse_ctx *c, const double *a, const double *b, long nstates, long elems_per_state, double *d_out);
+int state_resid(fbse_ctx *c, const double *a, const double *ha, const double *eps, long nstates,
+                long elems_per_state, double *d_out);
+int state_scale(fbse_ctx *c, double *psi, long nstates, long elems_per_state, const double *d_fac);
+int axpby_states(fbse_ctx *c, const double *x, double *y, long n, double a, double b);  // y = a*x + b*y
+
+// window accumulation (ham_kernels.hip)
+template <typename T>
+int accumulate_window(fbse_ctx *c, const cplx_t<T> *ring, int ring_slots, long slot_stride, long j_lo,
+                      long j_hi, const double *d_an, long ms, long ncheby, double *psi_rank, long n_starts,
+                      long n_items, int packed);
+
+}  // namespace fbse

@@ -1,0 +1,479 @@
+// ham_kernels.hip -- projector (SO / NL) application, windowed Newton
+// accumulation and per-state reductions for gfx950.
+#include "fbse_internal.h"
+
+namespace fbse {
+
+template <typename C> __device__ __forceinline__ C cm(C a, C b) {
+  C r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+// conj(a) * b
+template <typename C> __device__ __forceinline__ C cjm(C a, C b) {
+  C r;
+  r.x = a.x * b.x + a.y * b.y;
+  r.y = a.x * b.y - a.y * b.x;
+  return r;
+}
+
+__device__ __forceinline__ void atomic_add_real(double *p, double v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void atomic_add_real(float *p, float v) { unsafeAtomicAdd(p, v); }
+
+// ---------------------------------------------------------------------------
+// Projectors.  One work-group per (atom, chunk of kProjCols columns); a column
+// is one (state, spin) grid.  For every column it forms the 30 projections
+//   SO rows r = p*3+m : c = sum_g proj_p(g) conj(Y_m(g)) psi(g)
+//   NL rows r = 15+p*3+m : c = sum_g NL_p(g) conj(Y_m(g)) psi(g)
+// (phamiltonian.c:199-226, 372-404), mixes them (L.S for SO, +-1 for NL,
+// times dv), and scatters  out(g) += sum_rows beta_row(g) d_row  with
+// hardware float atomics (phamiltonian.c:230-306, 406-438: the reference
+// scatters with omp atomics as well).  Grid points are staged in LDS in
+// tiles of kProjPts; the 30 x 16 projections are a small GEMM over the tile.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void proj_kernel(ProjArgs<T> P) {
+  using C = cplx_t<T>;
+  __shared__ C s_psi[kProjPts][kProjCols + 1];
+  __shared__ T s_pj[5][kProjPts];
+  __shared__ T s_nl[5][kProjPts];
+  __shared__ C s_y[3][kProjPts];
+  __shared__ int s_j[kProjPts];
+  __shared__ C s_c[kProjRows][kProjCols];
+  __shared__ C s_d[kProjRows][kProjCols];
+
+  const int atom = blockIdx.x;
+  const int col0 = blockIdx.y * kProjCols;
+  const int ncol = min(kProjCols, P.ncols - col0);
+  const int off = P.a_off[atom], cnt = P.a_cnt[atom];
+  const int tid = threadIdx.x;
+  const int nout = kProjRows * kProjCols;  // 480 outputs, two per thread
+
+  C acc0, acc1;
+  acc0.x = acc0.y = 0;
+  acc1.x = acc1.y = 0;
+  const int o0 = tid, o1 = tid + 256;
+  const int r0 = o0 / kProjCols, c0 = o0 % kProjCols;
+  const int r1 = o1 / kProjCols, c1 = o1 % kProjCols;
+
+  for (int t0 = 0; t0 < cnt; t0 += kProjPts) {
+    const int npt = min(kProjPts, cnt - t0);
+    for (int e = tid; e < kProjPts; e += 256) {
+      if (e < npt) {
+        const int idx = off + t0 + e;
+        s_j[e] = P.jxyz[idx];
+        for (int p = 0; p < 5; ++p) {
+          s_pj[p][e] = P.proj[(long)p * P.npts + idx];
+          s_nl[p][e] = P.nlproj[(long)p * P.npts + idx];
+        }
+        for (int m = 0; m < 3; ++m) s_y[m][e] = P.y1[(long)m * P.npts + idx];
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < kProjPts * kProjCols; e += 256) {
+      const int pt = e % kProjPts, cc = e / kProjPts;
+      C v;
+      v.x = v.y = 0;
+      if (pt < npt && cc < ncol) {
+        const int col = col0 + cc;
+        const int st = col / P.nspin, sp = col - st * P.nspin;
+        v = P.psi_in[(long)st * P.nsg + (long)sp * P.ngrid + s_j[pt]];
+      }
+      s_psi[pt][cc] = v;
+    }
+    __syncthreads();
+    // partial projections over this tile
+    {
+      const int rr[2] = {r0, r1};
+      const int cc[2] = {c0, c1};
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int o = q ? o1 : o0;
+        if (o >= nout) continue;
+        const int r = rr[q];
+        const int so = r < 15;
+        const int p = (so ? r : r - 15) / 3, m = (so ? r : r - 15) % 3;
+        C a;
+        a.x = a.y = 0;
+        for (int pt = 0; pt < npt; ++pt) {
+          const T w = so ? s_pj[p][pt] : s_nl[p][pt];
+          const C t = cjm(s_y[m][pt], s_psi[pt][cc[q]]);  // conj(Y) psi
+          a.x += w * t.x;
+          a.y += w * t.y;
+        }
+        if (q) {
+          acc1.x += a.x;
+          acc1.y += a.y;
+        } else {
+          acc0.x += a.x;
+          acc0.y += a.y;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (o0 < nout) s_c[r0][c0] = acc0;
+  if (o1 < nout) s_c[r1][c1] = acc1;
+  __syncthreads();
+
+  // mix: SO via L.S (coupling the two spins of one state), NL via the sign
+  for (int o = tid; o < nout; o += 256) {
+    const int r = o / kProjCols, cc = o % kProjCols;
+    C d;
+    d.x = d.y = 0;
+    if (r < 15) {
+      const int p = r / 3, m = r % 3;
+      if (P.so && p < P.n_so && P.nspin == 2) {
+        const int sp = (col0 + cc) % 2;  // output spin s of this column
+        const int cbase = cc - sp;       // column of spin 0 of the same state
+        const int j = sp * 3 + m;
+        for (int jp = 0; jp < 6; ++jp) {
+          const int spp = jp / 3, mpp = jp % 3;
+          const C ls = P.ls[jp * 6 + j];
+          if (ls.x == (T)0 && ls.y == (T)0) continue;
+          C t = cm(ls, s_c[p * 3 + mpp][cbase + spp]);
+          d.x += t.x;
+          d.y += t.y;
+        }
+      }
+    } else {
+      const int p = (r - 15) / 3;
+      if (P.nl && p < P.n_nl) {
+        const T sg = (T)P.sign[atom * 5 + p];
+        d = s_c[r][cc];
+        d.x *= sg;
+        d.y *= sg;
+      }
+    }
+    d.x *= P.scale;
+    d.y *= P.scale;
+    s_d[r][cc] = d;
+  }
+  __syncthreads();
+
+  // expand and scatter: out(g) += sum_{p,m} proj_p Y_m d_so + NL_p Y_m d_nl
+  for (int t0 = 0; t0 < cnt; t0 += kProjPts) {
+    const int npt = min(kProjPts, cnt - t0);
+    for (int e = tid; e < npt * kProjCols; e += 256) {
+      const int pt = e % npt, cc = e / npt;
+      if (cc >= ncol) continue;
+      const int idx = off + t0 + pt;
+      const int col = col0 + cc;
+      const int st = col / P.nspin, sp = col - st * P.nspin;
+      C acc;
+      acc.x = acc.y = 0;
+      for (int m = 0; m < 3; ++m) {
+        C s;
+        s.x = s.y = 0;
+        for (int p = 0; p < 5; ++p) {
+          const T wso = P.proj[(long)p * P.npts + idx];
+          const T wnl = P.nlproj[(long)p * P.npts + idx];
+          const C dso = s_d[p * 3 + m][cc], dnl = s_d[15 + p * 3 + m][cc];
+          s.x += wso * dso.x + wnl * dnl.x;
+          s.y += wso * dso.y + wnl * dnl.y;
+        }
+        const C t = cm(P.y1[(long)m * P.npts + idx], s);
+        acc.x += t.x;
+        acc.y += t.y;
+      }
+      T *dst = reinterpret_cast<T *>(&P.psi_out[(long)st * P.nsg + (long)sp * P.ngrid + P.jxyz[idx]]);
+      atomic_add_real(dst, acc.x);
+      atomic_add_real(dst + 1, acc.y);
+    }
+  }
+}
+
+template <typename T>
+int proj_apply(fbse_ctx *c, const cplx_t<T> *psi_in, cplx_t<T> *psi_out, long nstates, double a1) {
+  ProjArgs<T> P{};
+  const ProjTables<T> *pt;
+  if constexpr (sizeof(T) == 8) pt = &c->pj;
+  else pt = &c->pj_f;
+  P.n_atoms = pt->n_atoms;
+  P.npts = pt->npts;
+  P.so = pt->so;
+  P.nl = pt->nl;
+  P.n_so = pt->n_so;
+  P.n_nl = pt->n_nl;
+  P.a_off = pt->a_off;
+  P.a_cnt = pt->a_cnt;
+  P.jxyz = pt->jxyz;
+  P.proj = pt->proj;
+  P.nlproj = pt->nlproj;
+  P.y1 = pt->y1;
+  P.sign = pt->sign;
+  P.ls = pt->ls;
+  P.psi_in = psi_in;
+  P.psi_out = psi_out;
+  P.ngrid = c->ngrid;
+  P.nsg = c->nsg;
+  P.nspin = (int)c->nspin;
+  P.ncols = (int)(nstates * c->nspin);
+  P.scale = (T)(c->dv * a1);
+  if (P.n_atoms == 0 || P.ncols == 0) return FBSE_OK;
+  dim3 grid(P.n_atoms, (P.ncols + kProjCols - 1) / kProjCols);
+  hipEvent_t ev = nullptr;
+  prof_begin(c, 2, &ev);
+  hipLaunchKernelGGL(proj_kernel<T>, grid, dim3(256), 0, c->stream, P);
+  FBSE_HIP(c, hipGetLastError());
+  prof_end(c, 2, ev);
+  return FBSE_OK;
+}
+template int proj_apply<double>(fbse_ctx *, const double2 *, double2 *, long, double);
+template int proj_apply<float>(fbse_ctx *, const float2 *, float2 *, long, double);
+
+// ---------------------------------------------------------------------------
+// Layout conversions.  Real states are packed two per complex grid (state 2q
+// in .x, 2q+1 in .y): T, V_loc and V_NL are real operators, so H acting on the
+// packed grid acts on each real state separately (an odd count pads a zero).
+// ---------------------------------------------------------------------------
+__global__ void pack_real_kernel(const double *src, long src_stride, long nstates, long nsg, double2 *dst) {
+  const long q = blockIdx.y;
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < nsg; g += (long)gridDim.x * blockDim.x) {
+    double2 v;
+    v.x = src[(2 * q) * src_stride + g];
+    v.y = (2 * q + 1 < nstates) ? src[(2 * q + 1) * src_stride + g] : 0.0;
+    dst[q * nsg + g] = v;
+  }
+}
+__global__ void unpack_real_kernel(const double2 *src, long nstates, long nsg, double *dst, long dst_stride) {
+  const long q = blockIdx.y;
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < nsg; g += (long)gridDim.x * blockDim.x) {
+    const double2 v = src[q * nsg + g];
+    dst[(2 * q) * dst_stride + g] = v.x;
+    if (2 * q + 1 < nstates) dst[(2 * q + 1) * dst_stride + g] = v.y;
+  }
+}
+static dim3 grid1d(long n, long y) {
+  long b = (n + 255) / 256;
+  if (b > 2048) b = 2048;
+  return dim3((unsigned)b, (unsigned)y);
+}
+int pack_real(fbse_ctx *c, const double *src, long src_stride, long nstates, double2 *dst) {
+  const long np = (nstates + 1) / 2;
+  if (np == 0) return FBSE_OK;
+  hipLaunchKernelGGL(pack_real_kernel, grid1d(c->nsg, np), dim3(256), 0, c->stream, src, src_stride, nstates, c->nsg,
+                     dst);
+  FBSE_HIP(c, hipGetLastError());
+  return FBSE_OK;
+}
+int unpack_real(fbse_ctx *c, const double2 *src, long nstates, double *dst, long dst_stride) {
+  const long np = (nstates + 1) / 2;
+  if (np == 0) return FBSE_OK;
+  hipLaunchKernelGGL(unpack_real_kernel, grid1d(c->nsg, np), dim3(256), 0, c->stream, src, nstates, c->nsg, dst,
+                     dst_stride);
+  FBSE_HIP(c, hipGetLastError());
+  return FBSE_OK;
+}
+__global__ void to_f32_kernel(const double2 *src, float2 *dst, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double2 v = src[i];
+    dst[i] = make_float2((float)v.x, (float)v.y);
+  }
+}
+int to_f32(fbse_ctx *c, const double2 *src, float2 *dst, long n) {
+  hipLaunchKernelGGL(to_f32_kernel, grid1d(n, 1), dim3(256), 0, c->stream, src, dst, n);
+  FBSE_HIP(c, hipGetLastError());
+  return FBSE_OK;
+}
+// zero fill by kernel (graph-capture safe, unlike hipMemsetAsync)
+__global__ void fill_zero_kernel(uint4 *dst, size_t n16, unsigned char *tail, size_t ntail) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = make_uint4(0, 0, 0, 0);
+  if (blockIdx.x == 0 && threadIdx.x < ntail) tail[threadIdx.x] = 0;
+}
+int fill_zero(fbse_ctx *c, void *dst, size_t bytes) {
+  const size_t n16 = bytes / 16, ntail = bytes % 16;
+  hipLaunchKernelGGL(fill_zero_kernel, grid1d((long)(n16 ? n16 : 1), 1), dim3(256), 0, c->stream,
+                     reinterpret_cast<uint4 *>(dst), n16, reinterpret_cast<unsigned char *>(dst) + n16 * 16, ntail);
+  FBSE_HIP(c, hipGetLastError());
+  return FBSE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Per-state reductions (deterministic: fixed block partition, then an ordered
+// second stage).  elems = doubles per state.
+// ---------------------------------------------------------------------------
+constexpr int kRedBlocks = 64;
+
+template <int OP>
+__global__ void state_red_kernel(const double *a, const double *b, const double *eps, long elems,
+                                 double *partial) {
+  const long s = blockIdx.y;
+  const double *pa = a + s * elems;
+  const double *pb = b ? b + s * elems : nullptr;
+  double acc = 0.0;
+  const double e = eps ? eps[s] : 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < elems; i += (long)gridDim.x * blockDim.x) {
+    if (OP == 0) acc += pa[i] * pa[i];
+    else if (OP == 1) acc += pa[i] * pb[i];
+    else {
+      const double r = pb[i] - e * pa[i];
+      acc += r * r;
+    }
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[s * gridDim.x + blockIdx.x] = sh[0];
+}
+__global__ void red_final_kernel(const double *partial, int nb, long nstates, double *out) {
+  for (long s = blockIdx.x * (long)blockDim.x + threadIdx.x; s < nstates; s += (long)gridDim.x * blockDim.x) {
+    double t = 0.0;
+    for (int b = 0; b < nb; ++b) t += partial[s * nb + b];
+    out[s] = t;
+  }
+}
+template <int OP>
+static int state_red(fbse_ctx *c, const double *a, const double *b, const double *eps, long nstates, long elems,
+                     double *d_out) {
+  if (nstates == 0) return FBSE_OK;
+  int rc = ensure_red(c, (size_t)nstates * kRedBlocks * sizeof(double));
+  if (rc) return rc;
+  for (long s0 = 0; s0 < nstates; s0 += 65535) {
+    const long ns = nstates - s0 < 65535 ? nstates - s0 : 65535;
+    hipLaunchKernelGGL(state_red_kernel<OP>, dim3(kRedBlocks, ns), dim3(256), 0, c->stream, a + s0 * elems,
+                       b ? b + s0 * elems : nullptr, eps ? eps + s0 : nullptr, elems, c->d_red);
+    FBSE_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(red_final_kernel, dim3((ns + 255) / 256), dim3(256), 0, c->stream, c->d_red, kRedBlocks, ns,
+                       d_out + s0);
+    FBSE_HIP(c, hipGetLastError());
+  }
+  return FBSE_OK;
+}
+int state_norms(fbse_ctx *c, const double *psi, long nstates, long elems, double *d_out) {
+  return state_red<0>(c, psi, nullptr, nullptr, nstates, elems, d_out);
+}
+int state_dots(fbse_ctx *c, const double *a, const double *b, long nstates, long elems, double *d_out) {
+  return state_red<1>(c, a, b, nullptr, nstates, elems, d_out);
+}
+int state_resid(fbse_ctx *c, const double *a, const double *ha, const double *eps, long nstates, long elems,
+                double *d_out) {
+  return state_red<2>(c, a, ha, eps, nstates, elems, d_out);
+}
+__global__ void state_scale_kernel(double *psi, long elems, const double *fac) {
+  const long s = blockIdx.y;
+  const double f = fac[s];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < elems; i += (long)gridDim.x * blockDim.x)
+    psi[s * elems + i] *= f;
+}
+int state_scale(fbse_ctx *c, double *psi, long nstates, long elems, const double *d_fac) {
+  for (long s0 = 0; s0 < nstates; s0 += 65535) {
+    const long ns = nstates - s0 < 65535 ? nstates - s0 : 65535;
+    hipLaunchKernelGGL(state_scale_kernel, grid1d(elems, ns), dim3(256), 0, c->stream, psi + s0 * elems, elems,
+                       d_fac + s0);
+    FBSE_HIP(c, hipGetLastError());
+  }
+  return FBSE_OK;
+}
+__global__ void axpby_kernel(const double *x, double *y, long n, double a, double b) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] = a * x[i] + b * y[i];
+}
+int axpby_states(fbse_ctx *c, const double *x, double *y, long n, double a, double b) {
+  hipLaunchKernelGGL(axpby_kernel, grid1d(n, 1), dim3(256), 0, c->stream, x, y, n, a, b);
+  FBSE_HIP(c, hipGetLastError());
+  return FBSE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Windowed Newton accumulation: for every start b, target t and grid element,
+//   acc[b,t] += sum_{j_lo <= j < j_hi} an[t,j] * psi_j
+// (filter.c:284-312 term 0 and :329-359, blocked over a window of steps so the
+// ms accumulators are read and written once per window instead of per step).
+// ring slot of step j is j % ring_slots; slot_stride = elements per slot.
+// packed = 1: real states packed in pairs, only an.re is used (an.im is
+// identically 0 for the Gaussian filter, coeff.c:89-105).
+// ---------------------------------------------------------------------------
+constexpr int kMaxWin = 64;
+constexpr int kMaxMs = 16;
+
+template <typename T>
+__global__ __launch_bounds__(256) void accumulate_kernel(const cplx_t<T> *ring, int ring_slots, long slot_stride,
+                                                         long j_lo, int nj, const double *an, int ms, long ncheby,
+                                                         double *psi_rank, long n_starts, long n_items,
+                                                         int packed, long nsg) {
+  __shared__ double2 s_a[kMaxMs * kMaxWin];
+  for (int e = threadIdx.x; e < ms * nj; e += blockDim.x) {
+    const int t = e / nj, j = e % nj;
+    s_a[t * kMaxWin + j] = reinterpret_cast<const double2 *>(an)[(long)t * ncheby + j_lo + j];
+  }
+  __syncthreads();
+  const long item = blockIdx.y;  // complex: start state; packed: start pair
+  for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < nsg; g += (long)gridDim.x * blockDim.x) {
+    double2 acc[kMaxMs];
+#pragma unroll
+    for (int t = 0; t < kMaxMs; ++t) acc[t] = make_double2(0.0, 0.0);
+    for (int j = 0; j < nj; ++j) {
+      const long slot = (j_lo + j) % ring_slots;
+      const cplx_t<T> x = ring[slot * slot_stride + item * nsg + g];
+      const double vx = (double)x.x, vy = (double)x.y;
+#pragma unroll
+      for (int t = 0; t < kMaxMs; ++t) {
+        if (t < ms) {
+          const double2 a = s_a[t * kMaxWin + j];
+          if (!packed) {
+            acc[t].x += a.x * vx - a.y * vy;
+            acc[t].y += a.x * vy + a.y * vx;
+          } else {
+            acc[t].x += a.x * vx;
+            acc[t].y += a.x * vy;
+          }
+        }
+      }
+    }
+    if (!packed) {
+      const long stlen = 2 * nsg;
+#pragma unroll
+      for (int t = 0; t < kMaxMs; ++t) {
+        if (t < ms) {
+          double2 *dst = reinterpret_cast<double2 *>(psi_rank + (item * ms + t) * stlen + 2 * g);
+          double2 o = *dst;
+          o.x += acc[t].x;
+          o.y += acc[t].y;
+          *dst = o;
+        }
+      }
+    } else {
+      const long stlen = nsg;
+#pragma unroll
+      for (int t = 0; t < kMaxMs; ++t) {
+        if (t < ms) {
+          psi_rank[((2 * item) * ms + t) * stlen + g] += acc[t].x;
+          if (2 * item + 1 < n_starts) psi_rank[((2 * item + 1) * ms + t) * stlen + g] += acc[t].y;
+        }
+      }
+    }
+  }
+}
+
+template <typename T>
+int accumulate_window(fbse_ctx *c, const cplx_t<T> *ring, int ring_slots, long slot_stride, long j_lo, long j_hi,
+                      const double *d_an, long ms, long ncheby, double *psi_rank, long n_starts, long n_items,
+                      int packed) {
+  const int nj = (int)(j_hi - j_lo);
+  if (nj <= 0) return FBSE_OK;
+  if (nj > kMaxWin || ms > kMaxMs)
+    return set_err(c, FBSE_ERR_UNSUPPORTED, "window %d > %d or ms %ld > %d", nj, kMaxWin, ms, kMaxMs);
+  hipEvent_t ev = nullptr;
+  prof_begin(c, 3, &ev);
+  long blocks = (c->nsg + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(accumulate_kernel<T>, dim3((unsigned)blocks, (unsigned)n_items), dim3(256), 0, c->stream, ring,
+                     ring_slots, slot_stride, j_lo, nj, d_an, (int)ms, ncheby, psi_rank, n_starts, n_items, packed,
+                     c->nsg);
+  FBSE_HIP(c, hipGetLastError());
+  prof_end(c, 3, ev);
+  return FBSE_OK;
+}
+template int accumulate_window<double>(fbse_ctx *, const double2 *, int, long, long, long, const double *, long,
+                                       long, double *, long, long, int);
+template int accumulate_window<float>(fbse_ctx *, const float2 *, int, long, long, long, const double *, long, long,
+                                      double *, long, long, int);
+
+}  // namespace fbse

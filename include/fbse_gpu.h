@@ -1,0 +1,165 @@
+/*
+ * fbse_gpu.h -- C ABI of the MI355X (gfx950) filter-diagonalization library.
+ *
+ * Plain C, POD only: no FFTW, MKL, MPI or HIP types cross this boundary, so
+ * FilterBSE's host C (which includes fd.h -> mpi.h/mkl.h/fftw3.h) can call it
+ * without either side seeing the other's headers.  Every host array stays
+ * caller-owned; device buffers are owned by the context.  All functions return
+ * FBSE_OK (0) or a negative FBSE_ERR_* code; fbse_last_error() gives the text.
+ * A context owns one device and one HIP stream and is not re-entrant across
+ * threads (the reference calls these from each MPI rank's master thread).
+ *
+ * Layouts are the reference's (filter_mpi/):
+ *   grid index   jxyz = jx + nx*(jy + ny*jz)                (init.c:592-595)
+ *   spinor       [up: ngrid | down: ngrid]                    (phamiltonian.c:236)
+ *   one state    stlen = nspin*ngrid*(is_complex+1) doubles, complex values
+ *                interleaved re,im                            (filter.c:259-278)
+ *   state block  psi_rank[(jns*ms + jms)*stlen + ...]         (filter.c:253,286)
+ *   coefficients an[2*(ncheby*jms + jc)] re,im                (filter.c:287,331)
+ *   Leja nodes   zn[jc-1] used at Newton step jc             (filter.c:320)
+ *   projectors   per-atom tables of stride n_gridpts          (init.c:1016)
+ * "*_dev" entry points take device pointers (buffers already resident in HBM)
+ * in the same layouts; the others take host pointers.
+ */
+#ifndef FBSE_GPU_H
+#define FBSE_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FBSE_OK 0
+#define FBSE_ERR_ARG (-1)
+#define FBSE_ERR_HIP (-2)
+#define FBSE_ERR_NORM_ZERO (-3)   /* normalize_all: "Norm is 0! exiting!" (norm.c:150-154) */
+#define FBSE_ERR_NORM_NAN (-4)    /* normalize_all: "Norm is NAN! exiting!" (norm.c:155-159) */
+#define FBSE_ERR_SOLVER (-5)      /* LAPACK info != 0 analogue (ortho.c:56-62, Hmat.c:202-216) */
+#define FBSE_ERR_NOMEM (-6)
+#define FBSE_ERR_UNSUPPORTED (-7)
+
+#define FBSE_PREC_FP64 0          /* FFT, recurrence and accumulation in FP64 */
+#define FBSE_PREC_MIXED 1         /* FP32 complex FFT/recurrence, FP64 coefficients,
+                                     FP64 accumulators, FP64 ortho/RR */
+
+typedef struct fbse_ctx fbse_ctx;
+
+/* Grid and local part of H.  Flat copies of ist->nx/ny/nz/nspin, flag->isComplex,
+ * par->dv, ksqr (build_grid_ksqr, init.c:318-349: 0.5|k|^2/N, capped at
+ * KE_max/N) and pot_local (nspin*ngrid, build_local_pot, init.c:666). */
+typedef struct {
+  long nx, ny, nz;
+  long nspin;
+  int is_complex;
+  double dv;
+  const double *ksqr;
+  const double *pot_local;
+} fbse_grid_desc;
+
+/* Spin-orbit / nonlocal projector tables: the nlc_st array (fd.h:382-398)
+ * split into structure-of-arrays by the caller.  Index idx = atom*n_gridpts + g
+ * for g < n_gpt[atom] (init_NL_projectors, init.c:988-1068). */
+typedef struct {
+  int so, nl;               /* flag->SO, flag->NL */
+  long n_atoms;             /* ist->n_NL_atoms */
+  long n_gridpts;           /* ist->n_NL_gridpts (table stride) */
+  const long *n_gpt;        /* nl[atom] */
+  const long *jxyz;         /* nlc[idx].jxyz */
+  const double *y1;         /* nlc[idx].y1[m] re,im: 6 doubles per idx */
+  const double *proj;       /* nlc[idx].proj[p]: 5 doubles per idx */
+  const double *nl_proj;    /* nlc[idx].NL_proj[p]: 5 doubles per idx */
+  const int *nl_sign;       /* nlc[atom*n_gridpts].NL_proj_sign[p]: 5 ints per atom */
+  int n_so_proj;            /* radial SO projectors (5 in phamiltonian.c:225) */
+  int n_nl_proj;            /* ist->nproj */
+  const double *LS;         /* L.S, 36 complex (def_LS, hamiltonian.c:312-442) */
+} fbse_proj_desc;
+
+/* Replaces the FFTW plan set-up and table copies of run_filter_cycles
+ * (filter.c:5-91) and of every H consumer.  proj may be NULL (no SO/NL).
+ * device < 0 selects the current device. */
+int fbse_create(const fbse_grid_desc *grid, const fbse_proj_desc *proj, int device, fbse_ctx **out);
+void fbse_destroy(fbse_ctx *ctx);
+const char *fbse_last_error(const fbse_ctx *ctx);
+
+/* Use the caller's HIP stream (a hipStream_t passed as void*); NULL restores the
+ * context's own stream. */
+int fbse_set_stream(fbse_ctx *ctx, void *stream);
+int fbse_synchronize(fbse_ctx *ctx);
+
+/* H applied to nstates states: replaces p_hamiltonian (phamiltonian.c:10-60) /
+ * hamiltonian (hamiltonian.c:9-57) as used by energy (energy.c:6-57),
+ * energy_all (energy.c:61-165), diag_H (Hmat.c:141) and calc_sigma_E
+ * (energy.c:474,500).  Real layouts return the real part, as those callers read it. */
+int fbse_apply_h(fbse_ctx *ctx, const double *psi, double *hpsi, long nstates);
+int fbse_apply_h_dev(fbse_ctx *ctx, const double *d_psi, double *d_hpsi, long nstates);
+
+/* The Newton filter over all start states of psi_rank: replaces the loop
+ * filter.c:102-116 (filter_cycle, filter.c:187-371, with p_hnorm filter.c:375-410).
+ * On entry start state jns sits at psi_rank[jns*ms*stlen]; on exit
+ * psi_rank[(jns*ms+jms)*stlen] holds sum_jc an[jms,jc] psi_jc.  Not normalised
+ * (run_filter_cycles calls normalize_all afterwards). */
+int fbse_filter(fbse_ctx *ctx, double *psi_rank, long n_starts, long ms, long ncheby, const double *an,
+                const double *zn, double Vmin, double dE_1, int precision);
+int fbse_filter_dev(fbse_ctx *ctx, double *d_psi_rank, long n_starts, long ms, long ncheby, const double *an,
+                    const double *zn, double Vmin, double dE_1, int precision);
+
+/* normalize_all (norm.c:82-180): each state scaled to sum |psi|^2 dv = 1.
+ * Returns FBSE_ERR_NORM_ZERO / FBSE_ERR_NORM_NAN where the reference exits. */
+int fbse_normalize_all(fbse_ctx *ctx, double *psitot, long n_states);
+int fbse_normalize_all_dev(fbse_ctx *ctx, double *d_psitot, long n_states);
+
+/* energy_all (energy.c:61-165): ene[i] = <psi_i|H|psi_i> dv (host array out). */
+int fbse_energy_all(fbse_ctx *ctx, const double *psitot, long n_states, double *ene);
+int fbse_energy_all_dev(fbse_ctx *ctx, const double *d_psitot, long n_states, double *ene);
+
+/* ortho_real / ortho_cplx (ortho.c:5-185): the first *rank columns of psitot
+ * are overwritten with an orthonormal basis of the left singular space whose
+ * singular values satisfy s_i/s_0 >= svd_eps (SVDEPS = 1e-10, fd.h:535), then
+ * normalised as mod_ortho.c:84 does.  sing_vals (n_cols, may be NULL)
+ * receives the singular values (descending) in units of the unweighted
+ * column norms. */
+int fbse_ortho(fbse_ctx *ctx, double *psitot, long n_cols, double svd_eps, long *rank, double *sing_vals);
+int fbse_ortho_dev(fbse_ctx *ctx, double *d_psitot, long n_cols, double svd_eps, long *rank, double *sing_vals);
+
+/* diag_H (Hmat.c:5-297) followed by normalize_all (mod_diag.c:433): builds
+ * H_ij = <psi_i|H|psi_j> dv, diagonalises it (ascending eigenvalues into
+ * eval) and rotates psitot in place into the eigenvectors. */
+int fbse_rayleigh_ritz(fbse_ctx *ctx, double *psitot, long n, double *eval);
+int fbse_rayleigh_ritz_dev(fbse_ctx *ctx, double *d_psitot, long n, double *eval);
+
+/* calc_sigma_E (energy.c:383-539): sigma_E[i] = sqrt|<H^2> - <H>^2| and
+ * eval_aux[i] = <H> (may be NULL), evaluated as ||H psi - <H> psi|| dv^(1/2)
+ * (the same quantity for Hermitian H, without the cancellation). */
+int fbse_sigma(fbse_ctx *ctx, const double *psitot, long n, double *sigma_E, double *eval_aux);
+int fbse_sigma_dev(fbse_ctx *ctx, const double *d_psitot, long n, double *sigma_E, double *eval_aux);
+
+/* get_energy_range iterative branch (energy.c:222-300) and range inflation
+ * (energy.c:343-369).  psi0_min / psi0_max are the two host-drawn start vectors
+ * (init_psi with the fixed seed, energy.c:207,228-231,270-273), one state each
+ * in the reference layout.  out[0..5] = Emin, Emax, par->Vmin, par->dE,
+ * iterations(Emin), iterations(Emax).  nl_lambda_min / vloc_min drive the
+ * NL-only trimmed inflation (pass 0.0 for the blanket one). */
+int fbse_energy_range(fbse_ctx *ctx, const double *psi0_min, const double *psi0_max, long max_iter,
+                      double nl_lambda_min, double vloc_min, double *out);
+
+/* Per-kernel GPU time recorded with HIP events on the context's stream while
+ * profiling is on (for roofline reporting).  kind: 0 = FFT passes (all axes),
+ * 1 = x-backward+Newton combine, 2 = projectors, 3 = window accumulation,
+ * 4 = all kernels of the filter.  Returns launches and total milliseconds. */
+int fbse_profile(fbse_ctx *ctx, int enable);
+int fbse_profile_read(fbse_ctx *ctx, int kind, long *launches, double *total_ms);
+
+/* Window depth K of the blocked Newton accumulation, FFT tile widths etc. are
+ * chosen by the built-in autotuner at the first call; this forces K (0 = auto). */
+int fbse_set_window(fbse_ctx *ctx, int window);
+
+/* Device memory helpers for callers without a GPU runtime of their own. */
+int fbse_dev_alloc(fbse_ctx *ctx, size_t bytes, void **d_ptr);
+int fbse_dev_free(fbse_ctx *ctx, void *d_ptr);
+int fbse_memcpy_h2d(fbse_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
+int fbse_memcpy_d2h(fbse_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FBSE_GPU_H */

@@ -1,0 +1,138 @@
+"""GPU parity: every C-ABI entry point against the CPU oracle on the same
+seeded inputs (small grids so the oracle runs in seconds).  Tolerances:
+H, filter, normalisation, energies and sigma in FP64 are limited only by FFT
+rounding (the oracle's FFT is an independent DFT code), so 1e-11 relative;
+the mixed-precision filter is held to 3e-5 relative to the FP64 result."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from systems import VARIANTS, gpu_context, oracle_ham, random_states, small_system
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.fixture(scope="module", params=sorted(VARIANTS))
+def system(request):
+    s = small_system(**VARIANTS[request.param])
+    print("case", request.param, flush=True)
+    return request.param, s, oracle_ham(s), gpu_context(s)
+
+
+def test_apply_h(system):
+    name, s, h, g = system
+    psi = random_states(s, 3, seed=11)
+    ref = h.apply(psi)
+    out = g.apply_h(psi)
+    assert rel(out, ref) < 1e-11, name
+
+
+def spectrum_bounds(s):
+    vmin = s["pot_local"].min() - 3.0
+    vmax = s["pot_local"].max() + 20.0 + 5.0
+    return vmin, vmax - vmin
+
+
+def filter_inputs(s, n_starts, ms, M):
+    stlen = (s["is_complex"] + 1) * s["nspin"] * s["nx"] * s["ny"] * s["nz"]
+    psi_rank, _ = O.init_filter_states(n_starts, ms, -3, s["nx"], s["ny"], s["nz"], s["nspin"], s["is_complex"],
+                                       s["dv"])
+    Vmin, dE = spectrum_bounds(s)
+    zn = O.leja_points(M)
+    targets = np.linspace(s["pot_local"].min() + 0.5, s["pot_local"].min() + 2.0, ms)
+    dt = (M / (2.5 * dE)) ** 2
+    an = O.newton_coeff(zn, targets, Vmin, dE, dt)
+    return psi_rank, stlen, an, zn, Vmin, 4.0 / dE
+
+
+@pytest.mark.parametrize("window", [3, 0])
+def test_filter_fp64(system, window):
+    name, s, h, g = system
+    n_starts, ms, M = 3, 4, 40
+    psi_rank, stlen, an, zn, Vmin, dE_1 = filter_inputs(s, n_starts, ms, M)
+    ref = h.filter_cycles(psi_rank, n_starts, ms, an, zn, Vmin, dE_1)
+    g.set_window(window)
+    out = g.filter(psi_rank, n_starts, ms, an, zn, Vmin, dE_1)
+    g.set_window(0)
+    assert rel(out, ref) < 1e-10, name
+
+
+def test_filter_mixed(system):
+    name, s, h, g = system
+    n_starts, ms, M = 2, 3, 40
+    psi_rank, stlen, an, zn, Vmin, dE_1 = filter_inputs(s, n_starts, ms, M)
+    ref = g.filter(psi_rank, n_starts, ms, an, zn, Vmin, dE_1)
+    out = g.filter(psi_rank, n_starts, ms, an, zn, Vmin, dE_1, precision=1)
+    assert rel(out, ref) < 3e-5, name
+
+
+def test_normalize_energy_sigma(system):
+    name, s, h, g = system
+    psi = random_states(s, 3, seed=5)
+    ref, rc = h.normalize_all(psi)
+    assert rc == 0
+    out = g.normalize_all(psi)
+    assert rel(out, ref) < 1e-13
+    e_ref = h.energies(ref)
+    e = g.energies(ref)
+    assert np.abs(e - e_ref).max() < 1e-11 * np.abs(e_ref).max()
+    s_ref, ev_ref = h.sigma(ref)
+    sg, ev = g.sigma(ref, 3)
+    assert np.abs(ev - ev_ref).max() < 1e-11 * np.abs(ev_ref).max()
+    assert np.abs(sg - s_ref).max() < 1e-9 * np.abs(s_ref).max()
+
+
+def test_normalize_zero_norm_is_an_error(system):
+    name, s, h, g = system
+    from filterbse_amd import api
+    psi = random_states(s, 2, seed=5)
+    psi[psi.size // 2:] = 0.0
+    with pytest.raises(api.FbseError) as ei:
+        g.normalize_all(psi)
+    assert ei.value.code == -3
+
+
+def test_ortho_and_rayleigh_ritz(system):
+    name, s, h, g = system
+    stlen = (s["is_complex"] + 1) * s["nspin"] * s["nx"] * s["ny"] * s["nz"]
+    rng = np.random.default_rng(2)
+    # 12 states spanning an 8-dimensional space, one direction at 1e-12 relative
+    base = random_states(s, 8, seed=9).reshape(8, stlen)
+    mix = rng.standard_normal((12, 8))
+    mix[:, 7] *= 1e-12
+    psi = (mix @ base).ravel()
+    U_ref, sv_ref, k_ref = O.ortho(psi, 12, stlen, s["is_complex"], s["dv"])
+    out, k, sv = g.ortho(psi, 12)
+    assert k == k_ref == 8, (k, k_ref, sv_ref)
+    assert np.allclose(sv[:8], sv_ref[:8], rtol=1e-9, atol=0)
+    U = O.as_matrix(out, k, stlen, s["is_complex"])
+    # orthonormal under the dv weight and spanning the same space
+    gram = s["dv"] * (U.conj().T @ U)
+    assert np.abs(gram - np.eye(k)).max() < 1e-10
+    P_ref = U_ref @ U_ref.conj().T * s["dv"]
+    P = U @ U.conj().T * s["dv"]
+    assert np.abs(P - P_ref).max() < 1e-8
+    # Rayleigh-Ritz in that basis
+    HU = O.as_matrix(h.apply(out[: k * stlen]), k, stlen, s["is_complex"])
+    w_ref, _ = O.rayleigh_ritz(U, HU, s["dv"])
+    rot, w = g.rayleigh_ritz(out[: k * stlen], k)
+    assert np.abs(w - w_ref).max() < 1e-10 * np.abs(w_ref).max()
+    # the rotated vectors are Ritz vectors: <psi_i|H|psi_j> dv = diag(w)
+    V = O.as_matrix(rot, k, stlen, s["is_complex"])
+    HV = O.as_matrix(h.apply(rot), k, stlen, s["is_complex"])
+    Hs = s["dv"] * (V.conj().T @ HV)
+    assert np.abs(Hs - np.diag(w)).max() < 1e-9 * np.abs(w).max()
+
+
+def test_energy_range(system):
+    name, s, h, g = system
+    from filterbse_amd import api
+    ref = h.energy_range(max_iter=60)
+    p0, p1 = api.energy_range_start_states(s["nx"], s["ny"], s["nz"], s["nspin"], s["is_complex"], s["dv"])
+    out = g.energy_range(p0, p1, max_iter=60)
+    assert out[4] == ref[4] and out[5] == ref[5]
+    assert np.abs(out[:4] - ref[:4]).max() < 1e-9 * np.abs(ref[:4]).max()
