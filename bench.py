@@ -1,0 +1,247 @@
+"""Benchmark of the filter-diagonalization path on one MI355X.
+
+One step = one full pass of the path over the workload with the start states
+already in HBM: Newton filter of all start states at all energy targets,
+normalize_all, orthogonalisation (rank cut 1e-10), Rayleigh-Ritz, and sigma_E
+for every Ritz pair.  Metric: eigenpairs with sigma_E < 1e-4 Ha per second of
+that pass (headline: mixed precision), next to the plain FP64 pass, Hpsi/s of
+the filter and the HBM roofline of the dominant kernel.
+
+    python bench.py [--steps K] [--warmup W] [--config c1_so_96]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "eigenpairs/s at resid<1e-4 Ha, mixed+warm vs plain FP64; Hψ/s, HBM roofline %"
+HBM_PEAK_GBS = 8000.0
+RESID = 1e-4
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def setup(cfg_name, dev):
+    import torch
+    from filterbse_amd import api, dots
+
+    cfg = dict(dots.CONFIGS[cfg_name])
+    t0 = time.perf_counter()
+    job = dots.build_dot(cfg["radius_nm"], cfg["n_req"], cfg["dx"], so=cfg["so"], nl=cfg["nl"])
+    log("dot: %d core atoms, %d ligands, grid %dx%dx%d, %.1fs" % (
+        job["n_core"], len(job["syms"]) - job["n_core"], job["nx"], job["ny"], job["nz"], time.perf_counter() - t0))
+    ctx = api.Context(job["nx"], job["ny"], job["nz"], job["nspin"], job["is_complex"], job["dv"], job["ksqr"],
+                      job["pot_local"], job["proj"])
+    t0 = time.perf_counter()
+    p0, p1 = api.energy_range_start_states(job["nx"], job["ny"], job["nz"], job["nspin"], job["is_complex"],
+                                           job["dv"])
+    rng_out = ctx.energy_range(p0, p1, max_iter=1500)
+    t_range = time.perf_counter() - t0
+    Emin_raw, Emax_raw, Vmin, dE = rng_out[:4]
+    M, ms, n_starts = cfg["ncheby"], cfg["ms"], cfg["n_starts"]
+    dt = (M / (2.5 * dE)) ** 2  # mod_filter.c:114
+    sig_f = math.sqrt(1.0 / (2.0 * dt))
+    # targets from just above the lowest eigenvalue, 2.5 filter widths apart
+    targets = Emin_raw + sig_f * (2.0 + 2.5 * np.arange(ms))
+    zn = api.leja_points(M)
+    an = api.newton_coeff(zn, targets, Vmin, dE, dt)
+    stlen = (job["is_complex"] + 1) * job["nspin"] * job["nx"] * job["ny"] * job["nz"]
+    starts, _ = api.init_filter_states(n_starts, 1, -1, job["nx"], job["ny"], job["nz"], job["nspin"],
+                                       job["is_complex"], job["dv"])
+    starts_dev = torch.from_numpy(starts).to(dev).view(n_starts, stlen)
+    del starts
+    psi_rank = torch.empty(n_starts * ms * stlen, dtype=torch.float64, device=dev)
+    log("energy range [%.6g, %.6g] (%d + %d iterations, %.2fs); Vmin %.6g dE %.6g; sigma_filter %.4g" % (
+        Emin_raw, Emax_raw, rng_out[4], rng_out[5], t_range, Vmin, dE, sig_f))
+    return dict(cfg=cfg, job=job, ctx=ctx, M=M, ms=ms, n_starts=n_starts, stlen=stlen, an=an, zn=zn, Vmin=Vmin,
+                dE=dE, targets=targets, starts_dev=starts_dev, psi_rank=psi_rank, range=rng_out, t_range=t_range)
+
+
+def one_pass(S, precision, stage_times=None):
+    import torch
+    ctx = S["ctx"]
+    n = S["n_starts"] * S["ms"]
+    S["psi_rank"].view(S["n_starts"], S["ms"], S["stlen"])[:, 0, :].copy_(S["starts_dev"])
+    torch.cuda.synchronize()
+    ptr = S["psi_rank"].data_ptr()
+    t = [time.perf_counter()]
+    ctx.filter_dev(ptr, S["n_starts"], S["ms"], S["an"], S["zn"], S["Vmin"], 4.0 / S["dE"], precision)
+    t.append(time.perf_counter())
+    ctx.normalize_all_dev(ptr, n)
+    rank, sv = ctx.ortho_dev(ptr, n)
+    t.append(time.perf_counter())
+    ev = ctx.rayleigh_ritz_dev(ptr, rank)
+    t.append(time.perf_counter())
+    sig, ev2 = ctx.sigma_dev(ptr, rank)
+    t.append(time.perf_counter())
+    if stage_times is not None:
+        stage_times.append(np.diff(t))
+    return ev, sig, rank
+
+
+def converged(ev, sig):
+    ok = sig < RESID
+    return ev[ok], sig[ok]
+
+
+def match_error(ev_a, ev_b):
+    """Largest distance from each converged eigenvalue of a to the nearest of b."""
+    if len(ev_a) == 0 or len(ev_b) == 0:
+        return None
+    return float(max(np.min(np.abs(ev_b - x)) for x in ev_a))
+
+
+def cpu_baseline(S, budget_s=15.0):
+    """The oracle's H-psi (a direct restatement of p_hamiltonian, one core) on
+    the same Hamiltonian, over a bounded number of applications."""
+    from oracle import oracle as O
+    job = S["job"]
+    h = O.Ham(job["nx"], job["ny"], job["nz"], job["nspin"], job["is_complex"], job["dv"], job["ksqr"],
+              job["pot_local"], job["proj"])
+    psi = S["starts_dev"][0].cpu().numpy()
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        h.apply(psi)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or n >= 50:
+            break
+    return dict(value=n / el, unit="Hψ/s", cores=1, kind="port",
+                sample="%d oracle H-psi applications (T+V_loc+SO+NL, one state) on the %s Hamiltonian, %.1f s"
+                       % (n, S["cfg_name"], el))
+
+
+def profile_filter(S, precision):
+    """Per-kernel device times (HIP events on the library's stream) of one filter."""
+    import torch
+    ctx = S["ctx"]
+    S["psi_rank"].view(S["n_starts"], S["ms"], S["stlen"])[:, 0, :].copy_(S["starts_dev"])
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    ctx.filter_dev(S["psi_rank"].data_ptr(), S["n_starts"], S["ms"], S["an"], S["zn"], S["Vmin"], 4.0 / S["dE"],
+                   precision)
+    res = {k: ctx.profile_read(k) for k in range(5)}
+    ctx.profile(False)
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c1_so_96")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the filter on a single MI355X; "
+              "--gpus %d is not supported (start states would shard across GPUs with no exchange "
+              "until one all-to-all before ortho)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from filterbse_amd import api
+    dev = torch.device("cuda:0")
+    S = setup(args.config, dev)
+    S["cfg_name"] = args.config
+    mixed, fp64 = api.FBSE_PREC_MIXED, api.FBSE_PREC_FP64
+
+    for _ in range(args.warmup):
+        one_pass(S, mixed)
+    stages = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ev, sig, rank = one_pass(S, mixed, stages)
+    torch.cuda.synchronize()
+    t_mixed = (time.perf_counter() - t0) / args.steps
+    ev_c, sig_c = converged(ev, sig)
+    st = np.mean(stages, axis=0)
+    log("mixed: %.3fs/pass (filter %.3f ortho %.3f RR %.3f sigma %.3f), rank %d, converged %d"
+        % (t_mixed, *st, rank, len(ev_c)))
+
+    fp = None
+    if not args.no_fp64:
+        fst = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev64, sig64, rank64 = one_pass(S, fp64, fst)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t0
+        ev64_c, _ = converged(ev64, sig64)
+        fp = dict(seconds_per_pass=t64, eigenpairs_per_s=len(ev64_c) / t64, converged=int(len(ev64_c)),
+                  rank=int(rank64), stage_s=[round(float(x), 4) for x in fst[0]],
+                  hpsi_per_s=S["n_starts"] * (S["M"] - 1) / fst[0][0],
+                  max_eig_err_mixed_vs_fp64=match_error(ev_c, ev64_c))
+        log("fp64: %.3fs/pass, rank %d, converged %d, max |e_mixed - e_fp64| %s"
+            % (t64, rank64, len(ev64_c), fp["max_eig_err_mixed_vs_fp64"]))
+
+    prof = profile_filter(S, mixed)
+    job = S["job"]
+    nsg = job["nspin"] * job["nx"] * job["ny"] * job["nz"]
+    n_items = S["n_starts"] if job["is_complex"] else (S["n_starts"] + 1) // 2
+    elem = 8  # complex float
+    fft_launches, fft_ms = prof[0]
+    avg_ms = fft_ms / max(fft_launches, 1)
+    bytes_per_launch = 2.0 * n_items * nsg * elem  # read + write of the batch once
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel="fft_axis_kernel<float> (x/y forward, z kinetic, y backward passes)",
+                    avg_launch_ms=round(avg_ms, 4), bytes_per_launch=bytes_per_launch,
+                    launches=fft_launches)
+    kernels = {name: dict(launches=prof[k][0], total_ms=round(prof[k][1], 3))
+               for k, name in enumerate(["fft_passes", "x_backward_newton_combine", "projectors",
+                                         "window_accumulate", "all"])}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(S)
+    cfg = S["cfg"]
+    hpsi = S["n_starts"] * (S["M"] - 1) / st[0]
+    line = {
+        "metric": METRIC,
+        "value": round(len(ev_c) / t_mixed, 4),
+        "unit": "eigenpairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t_mixed, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-complex filter, f64 ortho/RR/sigma",
+        "data": "synthetic (seeded zincblende dot, ran2 start states)",
+        "config": {
+            "workload": "%s: %.1f nm dot, %d atoms + %d ligands, %s, grid %dx%dx%d (dx %.3g bohr), %d starts x %d "
+                        "targets, Newton length %d" % (
+                            args.config, cfg["radius_nm"], job["n_core"], len(job["syms"]) - job["n_core"],
+                            "SO+NL spinors" if cfg["so"] else "real states", job["nx"], job["ny"], job["nz"],
+                            cfg["dx"], S["n_starts"], S["ms"], S["M"]),
+            "parallelism": "single GPU; start states batched in one filter",
+        },
+        "converged": int(len(ev_c)),
+        "rank": int(rank),
+        "hpsi_per_s": round(hpsi, 1),
+        "stage_s": [round(float(x), 4) for x in st],
+        "fp64": fp,
+        "energy_range_s": round(S["t_range"], 3),
+        "roofline": roofline,
+        "kernels_ms": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -813,64 +813,59 @@ __global__ void hermitize_kernel(double *H, long n, int cplx) {
   }
 }
 
-// Z[:, k] = W[:, src_k] / sig_k for the selected columns (column-major, n rows)
-__global__ void select_scale_kernel(const double *W, long n, const int *src, const double *inv, long k, double *Z,
-                                    int cplx) {
-  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n * k; idx += (long)gridDim.x * blockDim.x) {
-    const long i = idx % n, j = idx / n;
-    if (cplx) {
-      double2 v = ((const double2 *)W)[i + (long)src[j] * n];
-      ((double2 *)Z)[i + j * n] = make_double2(v.x * inv[j], v.y * inv[j]);
-    } else {
-      Z[i + j * n] = W[i + (long)src[j] * n] * inv[j];
-    }
-  }
-}
-
 static int ortho_dev_impl(fbse_ctx *c, double *A, long n, double svd_eps, long *rank, double *sing) {
   LinAlg la{c, c->is_complex != 0, c->nsg};
   const size_t es = la.es();
   int rc;
   double *G = nullptr, *w = nullptr, *Z = nullptr;
-  int *d_src = nullptr;
-  double *d_inv = nullptr;
   FBSE_HIP(c, hipMalloc(&G, (size_t)n * n * es));
   FBSE_HIP(c, hipMalloc(&Z, (size_t)n * n * es));
   FBSE_HIP(c, hipMalloc(&w, (size_t)n * 8));
-  FBSE_HIP(c, hipMalloc(&d_src, (size_t)n * sizeof(int)));
-  FBSE_HIP(c, hipMalloc(&d_inv, (size_t)n * 8));
   std::vector<double> hw(n);
-  std::vector<int> src;
-  std::vector<double> inv;
   long k = 0;
   auto cleanup = [&]() {
     hipStreamSynchronize(c->stream);
     hipFree(G);
     hipFree(Z);
     hipFree(w);
-    hipFree(d_src);
-    hipFree(d_inv);
   };
   // pass 1: G = A^H A -> V ; B = A V
   if ((rc = la.gram(A, n, A, n, G, n)) || (rc = la.eigh(G, n, w)) || (rc = la.rotate(A, n, G, n, n))) {
     cleanup();
     return rc;
   }
-  // pass 2: G2 = B^H B (small singular values now resolved relative to themselves)
-  if ((rc = la.gram(A, n, A, n, G, n))) {
-    cleanup();
-    return rc;
-  }
-  hipLaunchKernelGGL(hermitize_kernel, dim3(256), dim3(256), 0, c->stream, G, n, la.cplx ? 1 : 0);
-  if ((rc = la.eigh(G, n, w))) {
-    cleanup();
-    return rc;
-  }
+  // The first Gram eigensolve resolves eigenvalues only to eps*||G||, i.e.
+  // singular values down to ~1e-8 s_0.  Columns of B = A V whose eigenvalue is
+  // below kSmall*lambda_max (the leading, ascending ones) are re-measured by a
+  // second Gram over that block alone, where the eigensolver's error scales
+  // with the block's own norm; this resolves s_i/s_0 to ~1e-12, enough for
+  // the 1e-10 cut of ortho.c:68-75.
   FBSE_HIP(c, hipMemcpyAsync(hw.data(), w, n * 8, hipMemcpyDeviceToHost, c->stream));
   FBSE_HIP(c, hipStreamSynchronize(c->stream));
-  // singular values, descending; rank cut as ortho.c:68-75
+  const double kSmall = 1e-8;
+  const double lmax = std::max(hw[n - 1], 0.0);
+  long ns = 0;
+  while (ns < n && hw[ns] < kSmall * lmax) ++ns;
+  std::vector<double> mu;
+  if (ns > 0) {
+    if ((rc = la.gram(A, ns, A, ns, Z, ns))) {
+      cleanup();
+      return rc;
+    }
+    hipLaunchKernelGGL(hermitize_kernel, dim3(256), dim3(256), 0, c->stream, Z, ns, la.cplx ? 1 : 0);
+    if ((rc = la.eigh(Z, ns, w))) {
+      cleanup();
+      return rc;
+    }
+    mu.resize(ns);
+    FBSE_HIP(c, hipMemcpyAsync(mu.data(), w, ns * 8, hipMemcpyDeviceToHost, c->stream));
+    FBSE_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  // singular values, descending: the large block from pass 1, then the small
+  // block from pass 2 (all of which are below every large one)
   std::vector<double> sv(n);
-  for (long i = 0; i < n; ++i) sv[i] = sqrt(std::max(0.0, hw[n - 1 - i]));
+  for (long i = 0; i < n - ns; ++i) sv[i] = sqrt(std::max(0.0, hw[n - 1 - i]));
+  for (long i = 0; i < ns; ++i) sv[n - ns + i] = sqrt(std::max(0.0, mu[ns - 1 - i]));
   if (sing)
     for (long i = 0; i < n; ++i) sing[i] = sv[i];
   k = n;
@@ -884,17 +879,31 @@ static int ortho_dev_impl(fbse_ctx *c, double *A, long n, double svd_eps, long *
     cleanup();
     return set_err(c, FBSE_ERR_SOLVER, "ortho: all singular values vanish");
   }
-  src.resize(k);
-  inv.resize(k);
-  for (long i = 0; i < k; ++i) {
-    src[i] = (int)(n - 1 - i);
-    inv[i] = 1.0 / sv[i];
+  // Zsel (n x k): large column i -> e_{col}/s ; small kept j -> W_s[:, j]/s
+  {
+    std::vector<double> zh((size_t)n * k * (la.cplx ? 2 : 1), 0.0);
+    std::vector<double> ws;
+    const long nsk = std::max(0L, k - (n - ns));
+    if (nsk > 0) {
+      ws.resize((size_t)ns * ns * (la.cplx ? 2 : 1));
+      FBSE_HIP(c, hipMemcpyAsync(ws.data(), Z, ws.size() * 8, hipMemcpyDeviceToHost, c->stream));
+      FBSE_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    const int cw = la.cplx ? 2 : 1;
+    for (long i = 0; i < k; ++i) {
+      const double inv = 1.0 / sv[i];
+      if (i < n - ns) {
+        const long col = n - 1 - i;  // column of B holding this singular value
+        zh[((size_t)i * n + col) * cw] = inv;
+      } else {
+        const long j = ns - 1 - (i - (n - ns));  // eigenvector index in the small block
+        for (long r = 0; r < ns; ++r)
+          for (int t = 0; t < cw; ++t) zh[((size_t)i * n + r) * cw + t] = ws[((size_t)j * ns + r) * cw + t] * inv;
+      }
+    }
+    FBSE_HIP(c, hipMemcpyAsync(G, zh.data(), zh.size() * 8, hipMemcpyHostToDevice, c->stream));
   }
-  FBSE_HIP(c, hipMemcpyAsync(d_src, src.data(), k * sizeof(int), hipMemcpyHostToDevice, c->stream));
-  FBSE_HIP(c, hipMemcpyAsync(d_inv, inv.data(), k * 8, hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(select_scale_kernel, dim3(256), dim3(256), 0, c->stream, G, n, d_src, d_inv, k, Z,
-                     la.cplx ? 1 : 0);
-  if ((rc = la.rotate(A, n, Z, n, k))) {
+  if ((rc = la.rotate(A, n, G, n, k))) {
     cleanup();
     return rc;
   }

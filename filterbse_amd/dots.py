@@ -1,0 +1,247 @@
+"""Synthetic nanocrystal inputs for the benchmark workloads.
+
+The reference has no generator: it reads conf.par, pot<Sym>.par and
+SO_<Sym>.par written by other tools.  This module produces the same arrays the
+reference's input stage would build from such files (grid, ksqr, pot_local,
+projector tables) for a seeded zincblende dot:
+
+* lattice 6.05 A, two species on the fcc / (1/4,1/4,1/4) sublattices, a
+  sphere cut around the origin, each coordinate jittered by +-0.01 A;
+* every missing tetrahedral bond of a surface atom gets a ligand pseudo-atom
+  at 0.55 of the bond length, with V(r) = 0.5 exp(-r^2 / 1.0^2) Ha
+  (the reference's a exp(-(r-c)^2/b) ligand form, read.c:1204-1206,1310-1320);
+* per species V(q) = a0 (q^2 - a1) / (a2 exp(a3 q^2) - 1), transformed to
+  V(r) = (Omega / (2 pi^2 r)) PV int q sin(qr) V(q) dq with Omega = a^3/8
+  (volume per atom), a Gaussian taper exp(-(q/q_c)^2), q_c = 6 1/bohr, and one
+  global scale making the species-A well 1 Ha deep (the form factors carry no
+  stated units);
+  the pole of species A at q = sqrt(ln(1/a2)/a3) is integrated as a principal
+  value (symmetric exclusion);
+* l = 1 Kleinman-Bylander and spin-orbit projectors, one radial function per
+  atom beta(r) = r exp(-(r/1.2)^2), on the grid points within
+  R_NLcut2 = 1.5 + 6 ln 10 + 3 dx (read.c:908), with Y_1m evaluated as in
+  init_NL_projectors (init.c:1025-1038).
+
+Grid sizes follow the reference's rule (init_grid_params, init.c:5-255) from a
+requested size and spacing; linear interpolation of the tables is
+interpolate() (interpolate.c:5-47).
+"""
+import math
+
+import numpy as np
+
+from . import api
+
+ANG = 1.889726125  # bohr per Angstrom (fd.h:543)
+SPECIES = {
+    "A": (-31.4, 1.99, 0.97, 0.31),
+    "B": (8.4, 4.1, 1.2, 0.5),
+}
+SO_PAR = {"A": 0.04, "B": 0.12}
+NL_LAMBDA = {"A": 0.25, "B": -0.15}
+TAB_DR = 0.02
+TAB_LEN = 640           # V(r) tabulated to 12.8 bohr (interpolate() returns 0 beyond)
+EPSDX = 1.0e-20
+
+
+def zincblende_dot(radius_nm, a_ang=6.05, jitter_ang=0.01, seed=1, ligands=True):
+    """Seeded zincblende sphere: (symbols, coordinates in bohr, number of core
+    atoms).  Core atoms come first, ligands after (the reference counts the
+    leading non-ligand atoms as n_NL_atoms, read.c:1100-1101)."""
+    a = a_ang * ANG
+    rad = radius_nm * 10.0 * ANG
+    q = a / 4.0  # integer lattice in units of a/4
+    m = int(math.ceil(rad / a)) + 2
+    fcc = np.array([[0, 0, 0], [0, 2, 2], [2, 0, 2], [2, 2, 0]])
+    cells = np.array([[i, j, k] for i in range(-m, m + 1) for j in range(-m, m + 1) for k in range(-m, m + 1)]) * 4
+    cat = (cells[:, None, :] + fcc[None]).reshape(-1, 3)
+    ani = cat + 1
+    # sphere centred on a cation-anion bond midpoint (lattice offset 1/2 in a/4 units)
+    def inside(p):
+        return np.linalg.norm((p - 0.5) * q, axis=1) <= rad
+    cat, ani = cat[inside(cat)], ani[inside(ani)]
+    tet = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]])
+    for _ in range(8):  # drop atoms left with a single bond
+        aset = {tuple(p) for p in ani}
+        cset = {tuple(p) for p in cat}
+        nb_c = np.array([sum(tuple(p + t) in aset for t in tet) for p in cat])
+        nb_a = np.array([sum(tuple(p - t) in cset for t in tet) for p in ani])
+        keep_c, keep_a = nb_c >= 2, nb_a >= 2
+        if keep_c.all() and keep_a.all():
+            break
+        cat, ani = cat[keep_c], ani[keep_a]
+    aset = {tuple(p) for p in ani}
+    cset = {tuple(p) for p in cat}
+    syms = ["A"] * len(cat) + ["B"] * len(ani)
+    R = (np.concatenate([cat, ani]) - 0.5) * q
+    lig_syms, lig_R = [], []
+    if ligands:
+        for p in cat:
+            for t in tet:
+                if tuple(p + t) not in aset:
+                    lig_syms.append("P1")
+                    lig_R.append((p - 0.5 + 0.55 * t) * q)
+        for p in ani:
+            for t in tet:
+                if tuple(p - t) not in cset:
+                    lig_syms.append("P2")
+                    lig_R.append((p - 0.5 - 0.55 * t) * q)
+    rng = np.random.default_rng(seed)
+    R = R + rng.uniform(-jitter_ang * ANG, jitter_ang * ANG, size=R.shape)
+    if lig_R:
+        lig_R = np.array(lig_R) + rng.uniform(-jitter_ang * ANG, jitter_ang * ANG, size=(len(lig_R), 3))
+        return syms + lig_syms, np.concatenate([R, lig_R]), len(syms)
+    return syms, R, len(syms)
+
+
+VDEPTH = 1.0  # Ha: depth of the species-A well after the global normalisation
+
+
+def vq_table(params, a_bohr, scale=None):
+    """V(r) on the table grid from V(q) (principal value through the pole).
+    The overall scale of the form factors is not fixed by the configuration;
+    one global factor (the same for both species) makes max |V_A(r)| = VDEPTH."""
+    a0, a1, a2, a3 = params
+    omega = a_bohr ** 3 / 8.0
+    qmax, nq = 14.0, 28001
+    q = np.linspace(1e-6, qmax, nq)
+    den = a2 * np.exp(a3 * q * q) - 1.0
+    vq = a0 * (q * q - a1) / den * np.exp(-(q / 6.0) ** 2)
+    w = np.full(nq, q[1] - q[0])
+    w[0] *= 0.5
+    w[-1] *= 0.5
+    if a2 < 1.0:
+        qp = math.sqrt(math.log(1.0 / a2) / a3)
+        h = 0.02
+        w[np.abs(q - qp) < h] = 0.0  # symmetric exclusion = principal value
+    r = np.arange(TAB_LEN) * TAB_DR
+    rr = np.maximum(r, 1e-8)
+    v = np.array([omega * (w * q * np.sin(q * x) * vq).sum() / (2 * math.pi ** 2 * x) for x in rr])
+    # smooth to zero over the last 2 bohr of the table
+    taper = np.clip((r[-1] - r) / 2.0, 0.0, 1.0)
+    v = v * taper
+    if scale is None:
+        ra, va = vq_table(SPECIES["A"], a_bohr, scale=1.0)
+        scale = VDEPTH / np.abs(va).max()
+    return r, v * scale
+
+
+def ligand_table():
+    r = np.arange(TAB_LEN) * TAB_DR
+    return r, 0.5 * np.exp(-r * r / 1.0)
+
+
+def _interp(r, dr, vr, pot):
+    """interpolate() of interpolate.c:5-47 (linear, 0 beyond the table)."""
+    i = (r / dr).astype(np.int64)
+    n = len(vr)
+    ok = i <= n - 2
+    ic = np.clip(i, 0, n - 2)
+    a = (pot[ic + 1] - pot[ic]) / (vr[ic + 1] - vr[ic])
+    b = pot[ic] - vr[ic] * a
+    return np.where(ok, a * r + b, 0.0)
+
+
+def _local_box(center, gaxes, rcut):
+    idx = []
+    for ax in range(3):
+        g = gaxes[ax]
+        lo = np.searchsorted(g, center[ax] - rcut)
+        hi = np.searchsorted(g, center[ax] + rcut, side="right")
+        idx.append(np.arange(lo, hi))
+    return idx
+
+
+def build_dot(radius_nm, n_req, dx, so=True, nl=True, seed=1, ke_max=20.0):
+    """Returns the Context arguments plus metadata for a seeded dot."""
+    syms, R, n_core = zincblende_dot(radius_nm, seed=seed)
+    n, mins, maxs, dk = api.grid_params(R, n_req, (dx, dx, dx))
+    ksqr, gx, gy, gz = api.grid_ksqr(n, (dx, dx, dx), mins, dk, ke_max)
+    nx, ny, nz = (int(v) for v in n)
+    a_bohr = 6.05 * ANG
+    tables = {s: vq_table(SPECIES[s], a_bohr) for s in SPECIES}
+    tables["P1"] = tables["P2"] = ligand_table()
+    vloc = np.zeros((nz, ny, nx))
+    rcut = TAB_DR * (TAB_LEN - 1)
+    gaxes = (gx, gy, gz)
+    for s, p in zip(syms, R):
+        vr, pot = tables[s]
+        ix, iy, iz = _local_box(p, gaxes, rcut)
+        if not len(ix) or not len(iy) or not len(iz):
+            continue
+        dz = (gz[iz] - p[2])[:, None, None]
+        dy = (gy[iy] - p[1])[None, :, None]
+        dxx = (gx[ix] - p[0])[None, None, :]
+        dist = np.sqrt(dxx * dxx + dy * dy + dz * dz)
+        vloc[np.ix_(iz, iy, ix)] += _interp(dist, TAB_DR, vr, pot)
+    vloc = vloc.ravel()
+    nspin = 2 if so else 1
+    pot_local = np.tile(vloc, nspin)
+    out = dict(nx=nx, ny=ny, nz=nz, nspin=nspin, is_complex=1 if so else 0, dv=dx ** 3, ksqr=ksqr,
+               pot_local=pot_local, proj=None, syms=syms, R=R, n_core=n_core, mins=mins, dk=dk)
+    if so or nl:
+        out["proj"] = projector_tables(R[:n_core], syms[:n_core], gaxes, (nx, ny, nz), dx, so, nl)
+    return out
+
+
+def projector_tables(R, syms, gaxes, n, dx, so, nl):
+    gx, gy, gz = gaxes
+    nx, ny, nz = n
+    R2 = 1.5 + 6.0 * math.log(10.0) + 3.0 * dx
+    rc = math.sqrt(R2)
+    tmp1 = 0.5 * math.sqrt(3.0 / math.pi)
+    tmp2 = 0.5 * math.sqrt(3.0 / (2.0 * math.pi))
+    per_atom = []
+    for p in R:
+        ix, iy, iz = _local_box(p, gaxes, rc)
+        Z, Y, X = np.meshgrid(iz, iy, ix, indexing="ij")
+        dz = gz[Z] - p[2]
+        dy = gy[Y] - p[1]
+        dxx = gx[X] - p[0]
+        d2 = dxx * dxx + dy * dy + dz * dz
+        sel = d2 < R2  # loop order jz, jy, jx as init.c:1000-1017
+        per_atom.append((X[sel] + nx * (Y[sel] + ny * Z[sel]), dxx[sel], dy[sel], dz[sel], d2[sel]))
+    stride = max(len(t[0]) for t in per_atom)
+    na = len(R)
+    n_gpt = np.zeros(na, dtype=np.int64)
+    jxyz = np.zeros(na * stride, dtype=np.int64)
+    y1 = np.zeros(na * stride * 6)
+    prj = np.zeros(na * stride * 5)
+    nlp = np.zeros(na * stride * 5)
+    sign = np.zeros(na * 5, dtype=np.int32)
+    for a, (j, ddx, ddy, ddz, d2) in enumerate(per_atom):
+        cnt = len(j)
+        n_gpt[a] = cnt
+        base = a * stride
+        jxyz[base:base + cnt] = j
+        dxe, dye, dze = ddx + EPSDX, ddy + EPSDX, ddz + EPSDX
+        y = y1[base * 6:(base + cnt) * 6].reshape(cnt, 6)
+        r1 = 1.0 / np.sqrt(ddx * ddx + ddy * ddy + dze * dze)
+        y[:, 2] = tmp1 * dze * r1
+        r1 = 1.0 / np.sqrt(dxe * dxe + ddy * ddy + ddz * ddz)
+        y[:, 4] = -tmp2 * dxe * r1
+        y[:, 0] = tmp2 * dxe * r1
+        r1 = 1.0 / np.sqrt(ddx * ddx + dye * dye + ddz * ddz)
+        y[:, 5] = -tmp2 * dye * r1
+        y[:, 1] = -tmp2 * dye * r1
+        r = np.sqrt(d2)
+        beta = r * np.exp(-(r / 1.2) ** 2)
+        s = syms[a]
+        if so:
+            prj[base * 5:(base + cnt) * 5:5] = beta * math.sqrt(SO_PAR[s])
+        if nl:
+            lam = NL_LAMBDA[s]
+            nlp[base * 5:(base + cnt) * 5:5] = beta * math.sqrt(abs(lam))
+            sign[a * 5] = -1 if lam < 0 else 1
+    return dict(so=int(so), nl=int(nl), n_atoms=na, n_gridpts=stride, n_gpt=n_gpt, jxyz=jxyz, y1=y1, proj=prj,
+                nl_proj=nlp, nl_sign=sign, n_so_proj=1 if so else 0, n_nl_proj=1 if nl else 0, LS=api.def_LS())
+
+
+# Benchmark workloads (sizes from the configuration list; spacing chosen so the
+# reference grid rule keeps the requested grid).
+CONFIGS = {
+    "c0_real_64": dict(radius_nm=1.2, n_req=(64, 64, 64), dx=0.875, so=False, nl=False, n_starts=16, ms=8,
+                       ncheby=512),
+    "c1_so_96": dict(radius_nm=2.0, n_req=(96, 96, 96), dx=0.9, so=True, nl=True, n_starts=64, ms=16,
+                     ncheby=1024),
+}

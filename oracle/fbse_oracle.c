@@ -455,21 +455,22 @@ void or_set_ene_targets(double *t, long nVB, long nCB, double VBmin, double VBma
 /* (FFTW_FORWARD = -1), x fastest.  Plain recursive mixed radix; used   */
 /* in place of FFTW (hamiltonian.c:83,93; aux.c:265-266).                */
 /* ------------------------------------------------------------------ */
-static void or_fft1d_rec(or_cplx *out, const or_cplx *in, long n, long stride, int sign)
+/* tw[t] = exp(sign 2 pi i t / n), t < n, precomputed once per line length */
+static void or_fft1d_rec(or_cplx *out, const or_cplx *in, long n, long stride, long N, const or_cplx *tw)
 {
+  /* length-n DFT of in[j*stride] using the length-N table (n divides N) */
   if (n == 1) { out[0] = in[0]; return; }
   long p = 2;
   while (p * p <= n && n % p) p++;
   if (n % p) p = n; /* prime: direct DFT */
-  long m = n / p;
+  long m = n / p, step = N / n;
   if (p == n) {
     for (long k = 0; k < n; k++) {
       double sre = 0.0, sim = 0.0;
       for (long j = 0; j < n; j++) {
-        double ang = sign * OR_TWOPI * (double)((j * k) % n) / (double)n;
-        double c = cos(ang), s = sin(ang);
-        sre += in[j * stride].re * c - in[j * stride].im * s;
-        sim += in[j * stride].re * s + in[j * stride].im * c;
+        const or_cplx w = tw[((j * k) % n) * step];
+        sre += in[j * stride].re * w.re - in[j * stride].im * w.im;
+        sim += in[j * stride].re * w.im + in[j * stride].im * w.re;
       }
       out[k].re = sre;
       out[k].im = sim;
@@ -478,16 +479,15 @@ static void or_fft1d_rec(or_cplx *out, const or_cplx *in, long n, long stride, i
   }
   /* decimation in time: p sub-sequences of length m */
   or_cplx *sub = (or_cplx *)malloc(n * sizeof(or_cplx));
-  for (long r = 0; r < p; r++) or_fft1d_rec(&sub[r * m], &in[r * stride], m, stride * p, sign);
+  for (long r = 0; r < p; r++) or_fft1d_rec(&sub[r * m], &in[r * stride], m, stride * p, N, tw);
   for (long k = 0; k < m; k++) {
     for (long q = 0; q < p; q++) {
       long kk = k + q * m;
       double sre = 0.0, sim = 0.0;
       for (long r = 0; r < p; r++) {
-        double ang = sign * OR_TWOPI * (double)((r * kk) % n) / (double)n;
-        double c = cos(ang), s = sin(ang);
-        sre += sub[r * m + k].re * c - sub[r * m + k].im * s;
-        sim += sub[r * m + k].re * s + sub[r * m + k].im * c;
+        const or_cplx w = tw[((r * kk) % n) * step];
+        sre += sub[r * m + k].re * w.re - sub[r * m + k].im * w.im;
+        sim += sub[r * m + k].re * w.im + sub[r * m + k].im * w.re;
       }
       out[kk].re = sre;
       out[kk].im = sim;
@@ -496,31 +496,46 @@ static void or_fft1d_rec(or_cplx *out, const or_cplx *in, long n, long stride, i
   free(sub);
 }
 
+static or_cplx *or_twiddles(long n, int sign)
+{
+  or_cplx *tw = malloc(n * sizeof(or_cplx));
+  for (long t = 0; t < n; t++) {
+    double ang = sign * OR_TWOPI * (double)t / (double)n;
+    tw[t].re = cos(ang);
+    tw[t].im = sin(ang);
+  }
+  return tw;
+}
+
 void or_fft3d(or_cplx *a, long nx, long ny, long nz, int sign)
 {
   long nmax = nx > ny ? nx : ny;
   if (nz > nmax) nmax = nz;
   or_cplx *line = malloc(nmax * sizeof(or_cplx)), *res = malloc(nmax * sizeof(or_cplx));
+  or_cplx *twx = or_twiddles(nx, sign), *twy = or_twiddles(ny, sign), *twz = or_twiddles(nz, sign);
   for (long jz = 0; jz < nz; jz++)
     for (long jy = 0; jy < ny; jy++) {
       or_cplx *p = &a[nx * (ny * jz + jy)];
-      or_fft1d_rec(res, p, nx, 1, sign);
+      or_fft1d_rec(res, p, nx, 1, nx, twx);
       memcpy(p, res, nx * sizeof(or_cplx));
     }
   for (long jz = 0; jz < nz; jz++)
     for (long jx = 0; jx < nx; jx++) {
       for (long jy = 0; jy < ny; jy++) line[jy] = a[nx * (ny * jz + jy) + jx];
-      or_fft1d_rec(res, line, ny, 1, sign);
+      or_fft1d_rec(res, line, ny, 1, ny, twy);
       for (long jy = 0; jy < ny; jy++) a[nx * (ny * jz + jy) + jx] = res[jy];
     }
   for (long jy = 0; jy < ny; jy++)
     for (long jx = 0; jx < nx; jx++) {
       for (long jz = 0; jz < nz; jz++) line[jz] = a[nx * (ny * jz + jy) + jx];
-      or_fft1d_rec(res, line, nz, 1, sign);
+      or_fft1d_rec(res, line, nz, 1, nz, twz);
       for (long jz = 0; jz < nz; jz++) a[nx * (ny * jz + jy) + jx] = res[jz];
     }
   free(line);
   free(res);
+  free(twx);
+  free(twy);
+  free(twz);
 }
 
 /* ------------------------------------------------------------------ */
