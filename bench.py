@@ -50,8 +50,7 @@ def setup(cfg_name, dev):
     M, ms, n_starts = cfg["ncheby"], cfg["ms"], cfg["n_starts"]
     dt = (M / (2.5 * dE)) ** 2  # mod_filter.c:114
     sig_f = math.sqrt(1.0 / (2.0 * dt))
-    # targets from just above the lowest eigenvalue, 2.5 filter widths apart
-    targets = Emin_raw + sig_f * (2.0 + 2.5 * np.arange(ms))
+    targets = api.ene_targets(*cfg["targets"])
     zn = api.leja_points(M)
     an = api.newton_coeff(zn, targets, Vmin, dE, dt)
     stlen = (job["is_complex"] + 1) * job["nspin"] * job["nx"] * job["ny"] * job["nz"]
@@ -63,10 +62,11 @@ def setup(cfg_name, dev):
     log("energy range [%.6g, %.6g] (%d + %d iterations, %.2fs); Vmin %.6g dE %.6g; sigma_filter %.4g" % (
         Emin_raw, Emax_raw, rng_out[4], rng_out[5], t_range, Vmin, dE, sig_f))
     return dict(cfg=cfg, job=job, ctx=ctx, M=M, ms=ms, n_starts=n_starts, stlen=stlen, an=an, zn=zn, Vmin=Vmin,
-                dE=dE, targets=targets, starts_dev=starts_dev, psi_rank=psi_rank, range=rng_out, t_range=t_range)
+                dE=dE, targets=targets, starts_dev=starts_dev, psi_rank=psi_rank, range=rng_out, t_range=t_range,
+                sig_f=sig_f)
 
 
-def one_pass(S, precision, stage_times=None):
+def one_pass(S, precision, stage_times=None, refine=False):
     import torch
     ctx = S["ctx"]
     n = S["n_starts"] * S["ms"]
@@ -82,6 +82,14 @@ def one_pass(S, precision, stage_times=None):
     ev = ctx.rayleigh_ritz_dev(ptr, rank)
     t.append(time.perf_counter())
     sig, ev2 = ctx.sigma_dev(ptr, rank)
+    if refine:
+        # one FP64 residual-refinement sweep over the Ritz pairs near the
+        # targets that are within two decades of the threshold
+        w = 4.0 * S["sig_f"]
+        ev_r, sig_r = ctx.refine_dev(ptr, rank, ev, S["targets"].min() - w, S["targets"].max() + w, n,
+                                     sigma=sig, sigma_max=100.0 * RESID)
+        if len(ev_r):
+            ev, sig, rank = ev_r, sig_r, len(ev_r)
     t.append(time.perf_counter())
     if stage_times is not None:
         stage_times.append(np.diff(t))
@@ -157,17 +165,17 @@ def main():
     mixed, fp64 = api.FBSE_PREC_MIXED, api.FBSE_PREC_FP64
 
     for _ in range(args.warmup):
-        one_pass(S, mixed)
+        one_pass(S, mixed, refine=True)
     stages = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        ev, sig, rank = one_pass(S, mixed, stages)
+        ev, sig, rank = one_pass(S, mixed, stages, refine=True)
     torch.cuda.synchronize()
     t_mixed = (time.perf_counter() - t0) / args.steps
     ev_c, sig_c = converged(ev, sig)
     st = np.mean(stages, axis=0)
-    log("mixed: %.3fs/pass (filter %.3f ortho %.3f RR %.3f sigma %.3f), rank %d, converged %d"
+    log("mixed: %.3fs/pass (filter %.3f ortho %.3f RR %.3f refine+sigma %.3f), n %d, converged %d"
         % (t_mixed, *st, rank, len(ev_c)))
 
     fp = None

@@ -68,6 +68,10 @@ GPU_SIGNATURES = {
     "fbse_rayleigh_ritz_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, DP]),
     "fbse_sigma": (C.c_int, [C.c_void_p, DP, C.c_long, DP, DP]),
     "fbse_sigma_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, DP, DP]),
+    "fbse_refine": (C.c_int, [C.c_void_p, DP, C.c_long, DP, DP, C.c_double, C.c_double, C.c_double, C.c_long,
+                              C.c_double, DP, DP, LP]),
+    "fbse_refine_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, DP, DP, C.c_double, C.c_double, C.c_double,
+                                  C.c_long, C.c_double, DP, DP, LP]),
     "fbse_energy_range": (C.c_int, [C.c_void_p, DP, DP, C.c_long, C.c_double, C.c_double, DP]),
     "fbse_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "fbse_profile_read": (C.c_int, [C.c_void_p, C.c_int, LP, DP]),
@@ -345,6 +349,24 @@ class Context:
         s, e = np.zeros(n), np.zeros(n)
         self._check(gpu().fbse_sigma_dev(self.ctx, C.c_void_p(d_psi), n, _dp(s), _dp(e)))
         return s, e
+
+    def refine(self, psi, n_ritz, eval_, e_lo, e_hi, capacity, e_p=1.0, sigma=None, sigma_max=0.0):
+        buf = np.zeros(capacity * self.stlen)
+        buf[: n_ritz * self.stlen] = _f64(psi)[: n_ritz * self.stlen]
+        ev, sg, n = np.zeros(capacity), np.zeros(capacity), C.c_long(0)
+        sp = _dp(_f64(sigma)) if sigma is not None else None
+        self._check(gpu().fbse_refine(self.ctx, _dp(buf), n_ritz, _dp(_f64(eval_)), sp, sigma_max, e_lo, e_hi,
+                                      capacity, e_p, _dp(ev), _dp(sg), C.byref(n)))
+        k = n.value
+        return buf[: k * self.stlen], ev[:k], sg[:k]
+
+    def refine_dev(self, d_psi, n_ritz, eval_, e_lo, e_hi, capacity, e_p=1.0, sigma=None, sigma_max=0.0):
+        ev, sg, n = np.zeros(capacity), np.zeros(capacity), C.c_long(0)
+        sp = _dp(_f64(sigma)) if sigma is not None else None
+        self._check(gpu().fbse_refine_dev(self.ctx, C.c_void_p(d_psi), n_ritz, _dp(_f64(eval_)), sp, sigma_max,
+                                          e_lo, e_hi, capacity, e_p, _dp(ev), _dp(sg), C.byref(n)))
+        k = n.value
+        return ev[:k], sg[:k]
 
     def set_window(self, k):
         self._check(gpu().fbse_set_window(self.ctx, k))

@@ -1019,6 +1019,101 @@ extern "C" int fbse_rayleigh_ritz(fbse_ctx *c, double *psitot, long n, double *e
 }
 
 // ---------------------------------------------------------------------------
+// FP64 residual refinement of the Ritz pairs in a window (mixed-precision path)
+// ---------------------------------------------------------------------------
+static int precondition_dev(fbse_ctx *c, const double *d_r, double *d_out, long n, double e_p) {
+  int rc;
+  const long ch = h_chunk(c);
+  const size_t per = (size_t)c->nsg * sizeof(double2);
+  if (c->is_complex) {
+    if ((rc = ensure_scratch(c, (size_t)std::min(ch, n) * per))) return rc;
+    for (long s0 = 0; s0 < n; s0 += ch) {
+      const long ns = std::min(ch, n - s0);
+      if ((rc = precondition_cplx<double>(c, reinterpret_cast<const double2 *>(d_r) + s0 * c->nsg,
+                                          reinterpret_cast<double2 *>(d_out) + s0 * c->nsg, (double2 *)c->d_scratch,
+                                          ns, e_p)))
+        return rc;
+    }
+  } else {
+    const long chp = std::max(2L, ch & ~1L);
+    const long np = (std::min(chp, n) + 1) / 2;
+    if ((rc = ensure_scratch(c, (size_t)np * per * 2))) return rc;
+    if ((rc = ensure_scratch2(c, (size_t)np * per))) return rc;
+    double2 *packed = (double2 *)c->d_scratch, *work = packed + np * c->nsg, *res = (double2 *)c->d_scratch2;
+    for (long s0 = 0; s0 < n; s0 += chp) {
+      const long ns = std::min(chp, n - s0);
+      if ((rc = pack_real(c, d_r + s0 * c->nsg, c->nsg, ns, packed))) return rc;
+      if ((rc = precondition_cplx<double>(c, packed, res, work, (ns + 1) / 2, e_p))) return rc;
+      if ((rc = unpack_real(c, res, ns, d_out + s0 * c->nsg, c->nsg))) return rc;
+    }
+  }
+  return FBSE_OK;
+}
+
+extern "C" int fbse_refine_dev(fbse_ctx *c, double *d_psi, long n_ritz, const double *eval, const double *sigma,
+                               double sigma_max, double e_lo, double e_hi, long capacity, double e_p,
+                               double *eval_out, double *sigma_out, long *n_out) {
+  if (!c || !d_psi || !eval || !eval_out || !sigma_out || !n_out || n_ritz < 0 || capacity < 2) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipSetDevice(c->device));
+  if (e_p <= 0.0) e_p = 1.0;
+  const long sl = stlen_of(c);
+  std::vector<long> sel;
+  for (long i = 0; i < n_ritz; ++i)
+    if (eval[i] >= e_lo && eval[i] <= e_hi && (!sigma || sigma[i] < sigma_max)) sel.push_back(i);
+  long k = std::min((long)sel.size(), capacity / 2);
+  *n_out = 0;
+  if (k == 0) return FBSE_OK;
+  // compact the selected Ritz vectors to the front (indices increase, so no overlap)
+  std::vector<double> theta(k);
+  for (long j = 0; j < k; ++j) {
+    if (sel[j] != j)
+      FBSE_HIP(c, hipMemcpyAsync(d_psi + j * sl, d_psi + sel[j] * sl, sl * 8, hipMemcpyDeviceToDevice, c->stream));
+  }
+  for (long j = 0; j < k; ++j) theta[j] = eval[sel[j]];
+  double *d_hu = nullptr, *d_th = nullptr;
+  FBSE_HIP(c, hipMalloc(&d_hu, (size_t)k * sl * 8));
+  FBSE_HIP(c, hipMalloc(&d_th, (size_t)k * 8));
+  int rc = FBSE_OK;
+  hipError_t e = hipMemcpyAsync(d_th, theta.data(), k * 8, hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) rc = set_err(c, FBSE_ERR_HIP, "copy: %s", hipGetErrorString(e));
+  if (!rc) rc = apply_h_dev_impl(c, d_psi, d_hu, k);
+  if (!rc) rc = states_sub_scaled(c, d_psi, d_hu, k, sl, d_th);
+  if (!rc) rc = precondition_dev(c, d_hu, d_psi + k * sl, k, e_p);
+  hipStreamSynchronize(c->stream);
+  hipFree(d_hu);
+  hipFree(d_th);
+  if (rc) return rc;
+  long rank = 0;
+  if ((rc = ortho_dev_impl(c, d_psi, 2 * k, 1.0e-10, &rank, nullptr))) return rc;
+  if ((rc = rr_dev_impl(c, d_psi, rank, eval_out))) return rc;
+  std::vector<double> ev(rank);
+  if ((rc = energies_dev_impl(c, d_psi, rank, ev.data(), sigma_out))) return rc;
+  *n_out = rank;
+  return FBSE_OK;
+}
+extern "C" int fbse_refine(fbse_ctx *c, double *psitot, long n_ritz, const double *eval, const double *sigma,
+                           double sigma_max, double e_lo, double e_hi, long capacity, double e_p, double *eval_out,
+                           double *sigma_out, long *n_out) {
+  if (!c || !psitot) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipSetDevice(c->device));
+  const long sl = stlen_of(c);
+  double *d = nullptr;
+  FBSE_HIP(c, hipMalloc(&d, (size_t)capacity * sl * 8));
+  int rc = FBSE_OK;
+  hipError_t e = hipMemcpyAsync(d, psitot, (size_t)n_ritz * sl * 8, hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) rc = set_err(c, FBSE_ERR_HIP, "copy: %s", hipGetErrorString(e));
+  if (!rc)
+    rc = fbse_refine_dev(c, d, n_ritz, eval, sigma, sigma_max, e_lo, e_hi, capacity, e_p, eval_out, sigma_out, n_out);
+  if (!rc) {
+    e = hipMemcpyAsync(psitot, d, (size_t)(*n_out) * sl * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e != hipSuccess) rc = set_err(c, FBSE_ERR_HIP, "copy: %s", hipGetErrorString(e));
+  }
+  hipStreamSynchronize(c->stream);
+  hipFree(d);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
 // Spectrum bounds (energy.c:222-300 and 343-369)
 // ---------------------------------------------------------------------------
 extern "C" int fbse_energy_range(fbse_ctx *c, const double *psi0_min, const double *psi0_max, long max_iter,

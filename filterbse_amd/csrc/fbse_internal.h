@@ -57,9 +57,11 @@ template <typename T> struct AxisArgs {
   const T *vloc;
   long vloc_rows;          // rows per V period (nspin*nz*ny)
   T a1, a0;
+  // preconditioner (z pass): g = pa / (1 + k2 * pb)
+  T pa, pb;
 };
 
-enum PassKind { kPassFwd = 0, kPassBwd = 1, kPassKin = 2, kPassBwdCombine = 3 };
+enum PassKind { kPassFwd = 0, kPassBwd = 1, kPassKin = 2, kPassBwdCombine = 3, kPassPrec = 4 };
 
 // Device projector tables (compact per-atom storage).
 template <typename T> struct ProjTables {
@@ -181,6 +183,10 @@ template <typename T>
 int apply_h_cplx(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *work, long nstates, T a1,
                  T a0);
 
+// kinetic-energy preconditioner out = F^-1 [ F[r] / (N (1 + 0.5|k|^2 / e_p)) ]
+template <typename T>
+int precondition_cplx(fbse_ctx *c, const cplx_t<T> *r, cplx_t<T> *out, cplx_t<T> *work, long nstates, double e_p);
+
 // projector launcher (ham_kernels.hip): psi_out += scale * Vproj psi_in
 template <typename T>
 int proj_apply(fbse_ctx *c, const cplx_t<T> *psi_in, cplx_t<T> *psi_out, long nstates, double a1);
@@ -198,6 +204,8 @@ int state_resid(fbse_ctx *c, const double *a, const double *ha, const double *ep
                 long elems_per_state, double *d_out);
 int state_scale(fbse_ctx *c, double *psi, long nstates, long elems_per_state, const double *d_fac);
 int axpby_states(fbse_ctx *c, const double *x, double *y, long n, double a, double b);  // y = a*x + b*y
+// y_s -= theta_s * x_s for nstates states of elems doubles (theta on the device)
+int states_sub_scaled(fbse_ctx *c, const double *x, double *y, long nstates, long elems, const double *d_theta);
 
 // window accumulation (ham_kernels.hip)
 template <typename T>

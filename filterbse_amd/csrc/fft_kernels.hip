@@ -277,7 +277,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_axis_kernel(AxisArgs<T> A) {
     res = run_passes<T>(buf0, buf1, tw, A, LS, nl, false);
   } else if (KIND == kPassBwd || KIND == kPassBwdCombine) {
     res = run_passes<T>(buf0, buf1, tw, A, LS, nl, true);
-  } else {  // kinetic: forward, x k^2, backward
+  } else {  // kinetic (x k^2) or preconditioner (x g(k^2)): forward, scale, backward
     res = run_passes<T>(buf0, buf1, tw, A, LS, nl, false);
     for (int e = threadIdx.x; e < nl * n; e += blockDim.x) {
       const int l = e / n, p = e - l * n;
@@ -290,6 +290,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_axis_kernel(AxisArgs<T> A) {
         k2 = A.kx[jx] + A.ky[jy] + A.kz[p];
         if (k2 > A.kcap) k2 = A.kcap;
       }
+      if (KIND == kPassPrec) k2 = A.pa / ((T)1 + k2 * A.pb);
       C v = res[l * LS + p];
       v.x *= k2;
       v.y *= k2;
@@ -364,6 +365,8 @@ int fft_pass(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src, cplx_t<T> *d
   A.psi_in = psi_in;
   A.a1 = a1;
   A.a0 = a0;
+  A.pa = a1;  // the preconditioner pass takes (pa, pb) in the (a1, a0) slots
+  A.pb = a0;
   A.vloc_rows = c->nspin * c->nz * c->ny;
   long ntiles;
   if (axis == 0) {
@@ -406,6 +409,9 @@ int fft_pass(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src, cplx_t<T> *d
     case kPassKin:
       hipLaunchKernelGGL((fft_axis_kernel<T, kPassKin>), dim3(ntiles), dim3(kFftThreads), lds, c->stream, A);
       break;
+    case kPassPrec:
+      hipLaunchKernelGGL((fft_axis_kernel<T, kPassPrec>), dim3(ntiles), dim3(kFftThreads), lds, c->stream, A);
+      break;
     default:
       hipLaunchKernelGGL((fft_axis_kernel<T, kPassBwdCombine>), dim3(ntiles), dim3(kFftThreads), lds, c->stream,
                          A);
@@ -430,6 +436,19 @@ int apply_h_cplx(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *w
   }
   return FBSE_OK;
 }
+
+template <typename T>
+int precondition_cplx(fbse_ctx *c, const cplx_t<T> *r, cplx_t<T> *out, cplx_t<T> *work, long nstates, double e_p) {
+  const long nb = nstates * c->nspin;
+  const double pa = 1.0 / (double)c->ngrid, pb = (double)c->ngrid / e_p;
+  int rc;
+  if ((rc = fft_pass<T>(c, 0, kPassFwd, r, work, nb, nullptr, 0, 0))) return rc;
+  if ((rc = fft_pass<T>(c, 1, kPassFwd, work, work, nb, nullptr, 0, 0))) return rc;
+  if ((rc = fft_pass<T>(c, 2, kPassPrec, work, work, nb, nullptr, (T)pa, (T)pb))) return rc;
+  if ((rc = fft_pass<T>(c, 1, kPassBwd, work, work, nb, nullptr, 0, 0))) return rc;
+  return fft_pass<T>(c, 0, kPassBwd, work, out, nb, nullptr, 0, 0);
+}
+template int precondition_cplx<double>(fbse_ctx *, const double2 *, double2 *, double2 *, long, double);
 
 template int fft_pass<double>(fbse_ctx *, int, int, const double2 *, double2 *, long, const double2 *, double,
                               double);

@@ -375,6 +375,21 @@ __global__ void axpby_kernel(const double *x, double *y, long n, double a, doubl
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     y[i] = a * x[i] + b * y[i];
 }
+__global__ void sub_scaled_kernel(const double *x, double *y, long elems, const double *theta) {
+  const long s = blockIdx.y;
+  const double t = theta[s];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < elems; i += (long)gridDim.x * blockDim.x)
+    y[s * elems + i] -= t * x[s * elems + i];
+}
+int states_sub_scaled(fbse_ctx *c, const double *x, double *y, long nstates, long elems, const double *d_theta) {
+  for (long s0 = 0; s0 < nstates; s0 += 65535) {
+    const long ns = nstates - s0 < 65535 ? nstates - s0 : 65535;
+    hipLaunchKernelGGL(sub_scaled_kernel, grid1d(elems, ns), dim3(256), 0, c->stream, x + s0 * elems,
+                       y + s0 * elems, elems, d_theta + s0);
+    FBSE_HIP(c, hipGetLastError());
+  }
+  return FBSE_OK;
+}
 int axpby_states(fbse_ctx *c, const double *x, double *y, long n, double a, double b) {
   hipLaunchKernelGGL(axpby_kernel, grid1d(n, 1), dim3(256), 0, c->stream, x, y, n, a, b);
   FBSE_HIP(c, hipGetLastError());

@@ -8,9 +8,13 @@ projector tables) for a seeded zincblende dot:
 * lattice 6.05 A, two species on the fcc / (1/4,1/4,1/4) sublattices, a
   sphere cut around the origin, each coordinate jittered by +-0.01 A;
 * every missing tetrahedral bond of a surface atom gets a ligand pseudo-atom
-  at 0.55 of the bond length, with V(r) = 0.5 exp(-r^2 / 1.0^2) Ha
-  (the reference's a exp(-(r-c)^2/b) ligand form, read.c:1204-1206,1310-1320);
-* per species V(q) = a0 (q^2 - a1) / (a2 exp(a3 q^2) - 1), transformed to
+  of the reference's a exp(-(r-c)^2/b) form (read.c:1204-1206,1310-1320),
+  by default with the reference's own P1/P2 parameters and placement (see
+  LIGANDS);
+* local pseudopotentials: by default the reference's tabulated CdSe tables
+  (zincblende a = 6.05 A is CdSe), which give the dot a band gap for the
+  energy targets to straddle; alternatively (potential="vq") per species
+  V(q) = a0 (q^2 - a1) / (a2 exp(a3 q^2) - 1), transformed to
   V(r) = (Omega / (2 pi^2 r)) PV int q sin(qr) V(q) dq with Omega = a^3/8
   (volume per atom), a Gaussian taper exp(-(q/q_c)^2), q_c = 6 1/bohr, and one
   global scale making the species-A well 1 Ha deep (the form factors carry no
@@ -27,6 +31,7 @@ requested size and spacing; linear interpolation of the tables is
 interpolate() (interpolate.c:5-47).
 """
 import math
+import os
 
 import numpy as np
 
@@ -44,7 +49,18 @@ TAB_LEN = 640           # V(r) tabulated to 12.8 bohr (interpolate() returns 0 b
 EPSDX = 1.0e-20
 
 
-def zincblende_dot(radius_nm, a_ang=6.05, jitter_ang=0.01, seed=1, ligands=True):
+# Ligand passivation.  "reference": the reference's default ligand Gaussians
+# a exp(-r^2/b) (read.c:1199-1202: P1 a = 0.64, P2 a = -0.384, b = 2.2287033)
+# with P1 on cation dangling bonds at 0.55 and P2 on anion bonds at 0.275 of
+# the bond length, as in its 2.2 nm CdSe example (EXAMPLES/2.2nm_CdSe conf.dat).
+# "config": 0.5 exp(-r^2/1.0) Ha on every dangling bond at 0.55 of the bond.
+LIGANDS = {
+    "reference": dict(P1=(0.64, 2.2287033, 0.55), P2=(-0.384, 2.2287033, 0.275)),
+    "config": dict(P1=(0.5, 1.0, 0.55), P2=(0.5, 1.0, 0.55)),
+}
+
+
+def zincblende_dot(radius_nm, a_ang=6.05, jitter_ang=0.01, seed=1, ligands=True, scheme="reference"):
     """Seeded zincblende sphere: (symbols, coordinates in bohr, number of core
     atoms).  Core atoms come first, ligands after (the reference counts the
     leading non-ligand atoms as n_NL_atoms, read.c:1100-1101)."""
@@ -80,12 +96,12 @@ def zincblende_dot(radius_nm, a_ang=6.05, jitter_ang=0.01, seed=1, ligands=True)
             for t in tet:
                 if tuple(p + t) not in aset:
                     lig_syms.append("P1")
-                    lig_R.append((p - 0.5 + 0.55 * t) * q)
+                    lig_R.append((p - 0.5 + LIGANDS[scheme]["P1"][2] * t) * q)
         for p in ani:
             for t in tet:
                 if tuple(p - t) not in cset:
                     lig_syms.append("P2")
-                    lig_R.append((p - 0.5 - 0.55 * t) * q)
+                    lig_R.append((p - 0.5 - LIGANDS[scheme]["P2"][2] * t) * q)
     rng = np.random.default_rng(seed)
     R = R + rng.uniform(-jitter_ang * ANG, jitter_ang * ANG, size=R.shape)
     if lig_R:
@@ -126,9 +142,12 @@ def vq_table(params, a_bohr, scale=None):
     return r, v * scale
 
 
-def ligand_table():
-    r = np.arange(TAB_LEN) * TAB_DR
-    return r, 0.5 * np.exp(-r * r / 1.0)
+def ligand_table(a, b):
+    """a exp(-r^2 / b) on the pot-file r grid (dr = 0.0251327, read.c:1310-1320)."""
+    r = np.arange(1024) * 0.0251327
+    v = a * np.exp(-r * r / b)
+    last = int(np.nonzero(np.abs(v) > 1e-10)[0][-1]) + 2
+    return r[:last], v[:last]
 
 
 def _interp(r, dr, vr, pot):
@@ -152,28 +171,51 @@ def _local_box(center, gaxes, rcut):
     return idx
 
 
-def build_dot(radius_nm, n_req, dx, so=True, nl=True, seed=1, ke_max=20.0):
-    """Returns the Context arguments plus metadata for a seeded dot."""
-    syms, R, n_core = zincblende_dot(radius_nm, seed=seed)
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+
+
+def cdse_tables():
+    """The reference's tabulated CdSe local pseudopotentials (pots/pots_II_VI/
+    potCd.par, potSe.par: r V(r), 1024 rows, read as read_pot_file does,
+    read.c:1801-1833) for species A = Cd and B = Se, cut after the last row
+    with |V| > 1e-10 (interpolate() returns 0 beyond a table)."""
+    out = {}
+    for sym, f in (("A", "potCd.par"), ("B", "potSe.par")):
+        a = np.loadtxt(os.path.join(DATA, f))
+        last = int(np.nonzero(np.abs(a[:, 1]) > 1e-10)[0][-1]) + 2
+        out[sym] = (a[:last, 0].copy(), a[:last, 1].copy())
+    return out
+
+
+def build_dot(radius_nm, n_req, dx, so=True, nl=True, seed=1, ke_max=20.0, potential="cdse", ligands="reference"):
+    """Returns the Context arguments plus metadata for a seeded dot.
+    potential = "cdse": the reference's CdSe tables; "vq": the V(q) form.
+    ligands = "reference" or "config" (see LIGANDS)."""
+    syms, R, n_core = zincblende_dot(radius_nm, seed=seed, scheme=ligands)
     n, mins, maxs, dk = api.grid_params(R, n_req, (dx, dx, dx))
     ksqr, gx, gy, gz = api.grid_ksqr(n, (dx, dx, dx), mins, dk, ke_max)
     nx, ny, nz = (int(v) for v in n)
     a_bohr = 6.05 * ANG
-    tables = {s: vq_table(SPECIES[s], a_bohr) for s in SPECIES}
-    tables["P1"] = tables["P2"] = ligand_table()
+    if potential == "cdse":
+        tables = cdse_tables()
+    else:
+        tables = {s: vq_table(SPECIES[s], a_bohr) for s in SPECIES}
+    for lig in ("P1", "P2"):
+        a_, b_, _ = LIGANDS[ligands][lig]
+        tables[lig] = ligand_table(a_, b_)
     vloc = np.zeros((nz, ny, nx))
-    rcut = TAB_DR * (TAB_LEN - 1)
     gaxes = (gx, gy, gz)
     for s, p in zip(syms, R):
         vr, pot = tables[s]
-        ix, iy, iz = _local_box(p, gaxes, rcut)
+        dr = vr[1] - vr[0]
+        ix, iy, iz = _local_box(p, gaxes, vr[-1])
         if not len(ix) or not len(iy) or not len(iz):
             continue
         dz = (gz[iz] - p[2])[:, None, None]
         dy = (gy[iy] - p[1])[None, :, None]
         dxx = (gx[ix] - p[0])[None, None, :]
         dist = np.sqrt(dxx * dxx + dy * dy + dz * dz)
-        vloc[np.ix_(iz, iy, ix)] += _interp(dist, TAB_DR, vr, pot)
+        vloc[np.ix_(iz, iy, ix)] += _interp(dist, dr, vr, pot)
     vloc = vloc.ravel()
     nspin = 2 if so else 1
     pot_local = np.tile(vloc, nspin)
@@ -239,9 +281,11 @@ def projector_tables(R, syms, gaxes, n, dx, so, nl):
 
 # Benchmark workloads (sizes from the configuration list; spacing chosen so the
 # reference grid rule keeps the requested grid).
+# Energy targets straddle the CdSe gap as in the reference's 2.2 nm CdSe job
+# (EXAMPLES/2.2nm_CdSe/filter/input.par: VB -0.31..-0.24, CB -0.15..-0.06 Ha).
 CONFIGS = {
     "c0_real_64": dict(radius_nm=1.2, n_req=(64, 64, 64), dx=0.875, so=False, nl=False, n_starts=16, ms=8,
-                       ncheby=512),
+                       ncheby=512, targets=(4, 4, -0.31, -0.24, -0.15, -0.06)),
     "c1_so_96": dict(radius_nm=2.0, n_req=(96, 96, 96), dx=0.9, so=True, nl=True, n_starts=64, ms=16,
-                     ncheby=1024),
+                     ncheby=1024, targets=(10, 6, -0.31, -0.24, -0.15, -0.06)),
 }

@@ -132,6 +132,22 @@ int fbse_rayleigh_ritz_dev(fbse_ctx *ctx, double *d_psitot, long n, double *eval
 int fbse_sigma(fbse_ctx *ctx, const double *psitot, long n, double *sigma_E, double *eval_aux);
 int fbse_sigma_dev(fbse_ctx *ctx, const double *d_psitot, long n, double *sigma_E, double *eval_aux);
 
+/* One FP64 residual-refinement sweep after a mixed-precision filter (no
+ * counterpart in the reference, whose filter is FP64 throughout): the Ritz
+ * pairs of a preceding fbse_rayleigh_ritz with eval in [e_lo, e_hi] (and, when
+ * sigma is not NULL, sigma[i] < sigma_max) are compacted to the front of psitot, their residuals r = H u - e u are
+ * preconditioned in k-space (r / (1 + 0.5|k|^2 / e_p)), and the 2k vectors
+ * [u, P r] are orthogonalised and diagonalised again in FP64.  psitot must hold
+ * capacity >= 2k states.  On return psitot holds *n_out Ritz vectors with
+ * eval_out / sigma_out (capacity-long host arrays) as from fbse_rayleigh_ritz
+ * and fbse_sigma.  e_p <= 0 selects 1 Ha. */
+int fbse_refine(fbse_ctx *ctx, double *psitot, long n_ritz, const double *eval, const double *sigma,
+                double sigma_max, double e_lo, double e_hi, long capacity, double e_p, double *eval_out,
+                double *sigma_out, long *n_out);
+int fbse_refine_dev(fbse_ctx *ctx, double *d_psitot, long n_ritz, const double *eval, const double *sigma,
+                    double sigma_max, double e_lo, double e_hi, long capacity, double e_p, double *eval_out,
+                    double *sigma_out, long *n_out);
+
 /* get_energy_range iterative branch (energy.c:222-300) and range inflation
  * (energy.c:343-369).  psi0_min / psi0_max are the two host-drawn start vectors
  * (init_psi with the fixed seed, energy.c:207,228-231,270-273), one state each
