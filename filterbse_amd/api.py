@@ -76,6 +76,7 @@ GPU_SIGNATURES = {
     "fbse_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "fbse_profile_read": (C.c_int, [C.c_void_p, C.c_int, LP, DP]),
     "fbse_set_window": (C.c_int, [C.c_void_p, C.c_int]),
+    "fbse_set_fft_kernel": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "fbse_dev_alloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     "fbse_dev_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fbse_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
@@ -367,6 +368,9 @@ class Context:
                                           e_lo, e_hi, capacity, e_p, _dp(ev), _dp(sg), C.byref(n)))
         k = n.value
         return ev[:k], sg[:k]
+
+    def set_fft_kernel(self, variant, lines=0):
+        self._check(gpu().fbse_set_fft_kernel(self.ctx, variant, lines))
 
     def set_window(self, k):
         self._check(gpu().fbse_set_window(self.ctx, k))

@@ -196,6 +196,49 @@ static int build_projectors(fbse_ctx *c, const fbse_proj_desc *pd) {
   if ((rc = upload(c, (void **)&F.nlproj, nf.data(), nf.size() * 4))) return rc;
   if ((rc = upload(c, (void **)&F.y1, yf.data(), yf.size() * 4))) return rc;
   if ((rc = upload(c, (void **)&F.ls, lf.data(), lf.size() * 4))) return rc;
+  // Colour the atoms so that atoms sharing a grid point differ in colour
+  // (greedy, in atom order, over the conflict graph of shared points).
+  {
+    std::vector<std::pair<int, int>> pa(npts);
+    for (long a = 0; a < na; ++a)
+      for (int g = 0; g < cnt[a]; ++g) pa[off[a] + g] = {jx[off[a] + g], (int)a};
+    std::sort(pa.begin(), pa.end());
+    std::vector<std::vector<int>> adj(na);
+    for (size_t i = 0; i < pa.size();) {
+      size_t j = i;
+      while (j < pa.size() && pa[j].first == pa[i].first) ++j;
+      for (size_t u = i; u < j; ++u)
+        for (size_t v = u + 1; v < j; ++v) {
+          adj[pa[u].second].push_back(pa[v].second);
+          adj[pa[v].second].push_back(pa[u].second);
+        }
+      i = j;
+    }
+    std::vector<int> color(na, -1);
+    int ncolors = 0;
+    for (long a = 0; a < na; ++a) {
+      std::sort(adj[a].begin(), adj[a].end());
+      adj[a].erase(std::unique(adj[a].begin(), adj[a].end()), adj[a].end());
+      std::vector<char> used(ncolors + 1, 0);
+      for (int b : adj[a])
+        if (color[b] >= 0) used[color[b]] = 1;
+      int k = 0;
+      while (used[k]) ++k;
+      color[a] = k;
+      ncolors = std::max(ncolors, k + 1);
+    }
+    std::vector<int> order;
+    P.color_off.assign(ncolors + 1, 0);
+    for (int k = 0; k < ncolors; ++k) {
+      P.color_off[k] = (int)order.size();
+      for (long a = 0; a < na; ++a)
+        if (color[a] == k) order.push_back((int)a);
+    }
+    P.color_off[ncolors] = (int)order.size();
+    if ((rc = upload(c, (void **)&P.color_atoms, order.data(), order.size() * sizeof(int)))) return rc;
+    F.color_atoms = P.color_atoms;
+    F.color_off = P.color_off;
+  }
   c->has_proj = (pd->so || pd->nl) && npts > 0;
   return FBSE_OK;
 }
@@ -325,7 +368,7 @@ extern "C" void fbse_destroy(fbse_ctx *c) {
   void *ptrs[] = {c->d_vloc, c->d_vloc_f, c->d_kx, c->d_ky, c->d_kz, c->d_ksqr, c->d_kx_f, c->d_ky_f, c->d_kz_f,
                   c->d_ksqr_f, c->d_scratch, c->d_scratch2, c->d_red, c->pj.a_off, c->pj.a_cnt, c->pj.jxyz,
                   c->pj.sign, c->pj.proj, c->pj.nlproj, c->pj.y1, c->pj.ls, c->pj_f.proj, c->pj_f.nlproj,
-                  c->pj_f.y1, c->pj_f.ls};
+                  c->pj_f.y1, c->pj_f.ls, c->pj.color_atoms};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (int a = 0; a < 3; ++a) {
@@ -373,6 +416,12 @@ extern "C" int fbse_profile_read(fbse_ctx *c, int kind, long *launches, double *
 extern "C" int fbse_set_window(fbse_ctx *c, int w) {
   if (!c || w < 0 || w > 63) return FBSE_ERR_ARG;
   c->window = w;
+  return FBSE_OK;
+}
+extern "C" int fbse_set_fft_kernel(fbse_ctx *c, int variant, int lines) {
+  if (!c || variant < -1 || variant > 1 || lines < 0 || lines > 256) return FBSE_ERR_ARG;
+  c->force_variant = variant;
+  c->force_lines = lines;
   return FBSE_OK;
 }
 extern "C" int fbse_dev_alloc(fbse_ctx *c, size_t bytes, void **p) {
@@ -629,13 +678,13 @@ static int filter_impl(fbse_ctx *c, double *d_psi_rank, long n_starts, long ms, 
   const long slot_elems = n_items * nsg;
   const size_t slot_bytes = (size_t)slot_elems * sizeof(C);
   int rc;
-  // window depth: forced, or as deep as ~40% of free memory allows (<= 32)
+  // window depth: forced, or as deep as ~40% of free memory allows (<= 63)
   int K = c->window;
   if (K <= 0) {
     size_t fr = 0, tot = 0;
     hipMemGetInfo(&fr, &tot);
     long kk = (long)((0.4 * (double)fr) / (double)slot_bytes) - 2;
-    K = (int)std::max(1L, std::min(32L, kk));
+    K = (int)std::max(1L, std::min(63L, kk));
   }
   if (K > 63) K = 63;
   const int R = K + 1;

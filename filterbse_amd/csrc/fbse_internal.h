@@ -74,10 +74,14 @@ template <typename T> struct ProjTables {
   cplx_t<T> *y1 = nullptr;  // [3][npts]
   int *sign = nullptr;      // [n_atoms*5]
   cplx_t<T> *ls = nullptr;  // 36
+  // atoms grouped by colour: atoms of one colour share no grid point
+  int *color_atoms = nullptr;          // atom indices, colour by colour
+  std::vector<int> color_off;          // host: start of each colour in color_atoms
 };
 
 template <typename T> struct ProjArgs {
   int n_atoms, npts, so, nl, n_so, n_nl;
+  const int *atoms;         // atoms of the colour processed by this launch
   const int *a_off, *a_cnt, *jxyz;
   const T *proj, *nlproj;
   const cplx_t<T> *y1;
@@ -137,6 +141,17 @@ struct fbse_ctx {
   size_t h_red_bytes = 0;
 
   int window = 0;               // forced window depth (0 = auto)
+
+  // FFT autotuner: per precision (0 FP64, 1 FP32), batch size class (0 below
+  // 32M elements, 1 above), axis and pass class (0 plain, 1 z kinetic or
+  // preconditioner, 2 x combine) the chosen kernel (0 ping-pong, 1 in-place)
+  // and tile lines; lines 0 = not tuned yet.
+  int tune_kernel[2][2][3][3] = {};
+  int tune_lines[2][2][3][3] = {};
+  double tune_ms[2][2][3][3] = {};
+  int autotune = 1;
+  int force_variant = -1;       // fbse_set_fft_kernel
+  int force_lines = 0;
 
   // profiling
   int profile = 0;

@@ -14,6 +14,7 @@
 // Sign convention is FFTW's: forward exp(-2 pi i jk/n), backward exp(+...).
 #include "fbse_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -229,6 +230,8 @@ __device__ cplx_t<T> *run_passes(cplx_t<T> *a, cplx_t<T> *b, const cplx_t<T> *tw
   return a;
 }
 
+constexpr int kBatchPP = 8;  // global loads in flight per thread (two-buffer kernel)
+
 template <typename T, int KIND>
 __global__ __launch_bounds__(kFftThreads) void fft_axis_kernel(AxisArgs<T> A) {
   using C = cplx_t<T>;
@@ -260,14 +263,38 @@ __global__ __launch_bounds__(kFftThreads) void fft_axis_kernel(AxisArgs<T> A) {
 
   const int tot = L * n;
   if (A.xaxis) {
-    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-      const int l = e / n, p = e - l * n;
-      if (l < nl) buf0[l * LS + p] = A.src[base + (long)e];
+    const int totv = nl * n;
+    for (int e0 = threadIdx.x; e0 < totv; e0 += kFftThreads * kBatchPP) {
+      C v[kBatchPP];
+#pragma unroll
+      for (int u = 0; u < kBatchPP; ++u) {
+        const int e = e0 + u * kFftThreads;
+        if (e < totv) v[u] = A.src[base + (long)e];
+      }
+#pragma unroll
+      for (int u = 0; u < kBatchPP; ++u) {
+        const int e = e0 + u * kFftThreads;
+        if (e < totv) {
+          const int l = e / n, p = e - l * n;
+          buf0[l * LS + p] = v[u];
+        }
+      }
     }
   } else {
-    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-      const int p = e / L, l = e - p * L;
-      if (l < nl) buf0[l * LS + p] = A.src[base + (long)p * A.inner + l];
+    for (int e0 = threadIdx.x; e0 < tot; e0 += kFftThreads * kBatchPP) {
+      C v[kBatchPP];
+#pragma unroll
+      for (int u = 0; u < kBatchPP; ++u) {
+        const int e = e0 + u * kFftThreads;
+        const int p = e / L, l = e - p * L;
+        if (e < tot && l < nl) v[u] = A.src[base + (long)p * A.inner + l];
+      }
+#pragma unroll
+      for (int u = 0; u < kBatchPP; ++u) {
+        const int e = e0 + u * kFftThreads;
+        const int p = e / L, l = e - p * L;
+        if (e < tot && l < nl) buf0[l * LS + p] = v[u];
+      }
     }
   }
   __syncthreads();
@@ -302,18 +329,32 @@ __global__ __launch_bounds__(kFftThreads) void fft_axis_kernel(AxisArgs<T> A) {
   }
 
   if (KIND == kPassBwdCombine) {
-    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-      const int l = e / n, p = e - l * n;
-      if (l >= nl) continue;
-      const long g = base + (long)e;
-      const long row = (line0 + l) % A.vloc_rows;
-      const T v = A.vloc[row * n + p];
-      const C pin = A.psi_in[g];
-      C t = res[l * LS + p];
-      C o;
-      o.x = A.a1 * (t.x + v * pin.x) + A.a0 * pin.x;
-      o.y = A.a1 * (t.y + v * pin.y) + A.a0 * pin.y;
-      A.dst[g] = o;
+    const int totv = nl * n;
+    for (int e0 = threadIdx.x; e0 < totv; e0 += kFftThreads * kBatchPP) {
+      C pin[kBatchPP];
+      T vv[kBatchPP];
+#pragma unroll
+      for (int u = 0; u < kBatchPP; ++u) {
+        const int e = e0 + u * kFftThreads;
+        if (e < totv) {
+          const int l = e / n, p = e - l * n;
+          const long row = (line0 + l) % A.vloc_rows;
+          vv[u] = A.vloc[row * n + p];
+          pin[u] = A.psi_in[base + (long)e];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kBatchPP; ++u) {
+        const int e = e0 + u * kFftThreads;
+        if (e < totv) {
+          const int l = e / n, p = e - l * n;
+          const C t = res[l * LS + p];
+          C o;
+          o.x = A.a1 * (t.x + vv[u] * pin[u].x) + A.a0 * pin[u].x;
+          o.y = A.a1 * (t.y + vv[u] * pin[u].y) + A.a0 * pin[u].y;
+          A.dst[base + (long)e] = o;
+        }
+      }
     }
   } else if (A.xaxis) {
     for (int e = threadIdx.x; e < tot; e += blockDim.x) {
@@ -328,14 +369,288 @@ __global__ __launch_bounds__(kFftThreads) void fft_axis_kernel(AxisArgs<T> A) {
   }
 }
 
+constexpr int kIpThreads = 512;  // threads of the in-place kernel
+constexpr int kBatch = 8;        // global loads in flight per thread while filling a tile
+
+// One in-place Stockham pass: every thread first reads (and twiddles) the R
+// inputs of each of its butterflies into registers, the work-group syncs, and
+// then the outputs are written to their self-sorted positions.  One LDS buffer
+// instead of two lets the tile hold twice the lines.
+template <typename T, int R, int KB>
+__device__ __forceinline__ void pass_inplace(cplx_t<T> *buf, const cplx_t<T> *tw, int n, int p, int LS, int nlines,
+                                             bool inv) {
+  using C = cplx_t<T>;
+  const int m = n / R;
+  const int tstep = n / (p * R);
+  const int nb = m * nlines;
+  const T sgn = inv ? (T)1 : (T)-1;
+  C x[KB][R];
+  int dsti[KB];
+#pragma unroll
+  for (int k = 0; k < KB; ++k) {
+    const int b = threadIdx.x + k * kIpThreads;
+    dsti[k] = -1;
+    if (b < nb) {
+      const int l = b / m;
+      const int i = b - l * m;
+      const int kk = i % p;
+      const C *src = buf + l * LS + i;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        C v = src[r * m];
+        if (r > 0 && kk > 0) {
+          C w = tw[r * kk * tstep];
+          if (inv) w = cconj(w);
+          v = cmul(v, w);
+        }
+        x[k][r] = v;
+      }
+      dsti[k] = l * LS + (i - kk) * R + kk;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < KB; ++k) {
+    if (dsti[k] >= 0) {
+      Bfly<T, R>::run(x[k], sgn);
+#pragma unroll
+      for (int q = 0; q < R; ++q) buf[dsti[k] + q * p] = x[k][q];
+    }
+  }
+  __syncthreads();
+}
+
+// KE = elements of the tile per thread (tile <= kIpThreads * KE elements).
+template <typename T, int KE>
+__device__ void run_inplace(cplx_t<T> *buf, const cplx_t<T> *tw, const AxisArgs<T> &A, int LS, int nlines, bool inv) {
+  int p = 1;
+  for (int s = 0; s < A.nf; ++s) {
+    const int R = A.f[s];
+    switch (R) {
+      case 2: pass_inplace<T, 2, (KE + 1) / 2>(buf, tw, A.n, p, LS, nlines, inv); break;
+      case 3: pass_inplace<T, 3, (KE + 2) / 3>(buf, tw, A.n, p, LS, nlines, inv); break;
+      case 4: pass_inplace<T, 4, (KE + 3) / 4>(buf, tw, A.n, p, LS, nlines, inv); break;
+      case 5: pass_inplace<T, 5, (KE + 4) / 5>(buf, tw, A.n, p, LS, nlines, inv); break;
+      default: pass_inplace<T, 8, (KE + 7) / 8>(buf, tw, A.n, p, LS, nlines, inv); break;
+    }
+    p *= R;
+  }
+}
+
+// Same contract as fft_axis_kernel, one LDS buffer, tile of up to
+// kIpThreads*KE elements; radices limited to 2,3,4,5,7,8,11,13.
+template <typename T, int KIND, int KE>
+__global__ __launch_bounds__(kIpThreads) void fft_axis_inplace_kernel(AxisArgs<T> A) {
+  using C = cplx_t<T>;
+  extern __shared__ unsigned char smem_raw[];
+  const int n = A.n;
+  const int L = A.lines;
+  const int LS = n | 1;
+  C *buf = reinterpret_cast<C *>(smem_raw);
+  C *tw = buf + L * LS;
+  for (int t = threadIdx.x; t < n; t += kIpThreads) tw[t] = A.tw[t];
+
+  const long tile = blockIdx.x;
+  long base, i0 = 0, line0 = 0;
+  int nl;
+  if (A.xaxis) {
+    line0 = tile * L;
+    base = line0 * n;
+    const long rem = A.total_lines - line0;
+    nl = rem < L ? (int)rem : L;
+  } else {
+    const long o = tile / A.tiles_per_outer;
+    i0 = (tile - o * A.tiles_per_outer) * L;
+    base = o * A.outer_stride + i0;
+    const long rem = A.inner_count - i0;
+    nl = rem < L ? (int)rem : L;
+  }
+  const int tot = L * n;
+  // tile load: kBatch independent global loads in flight per thread
+  if (A.xaxis) {
+    const int totv = nl * n;
+    for (int e0 = threadIdx.x; e0 < totv; e0 += kIpThreads * kBatch) {
+      C v[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int e = e0 + u * kIpThreads;
+        if (e < totv) v[u] = A.src[base + (long)e];
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int e = e0 + u * kIpThreads;
+        if (e < totv) {
+          const int l = e / n, p = e - l * n;
+          buf[l * LS + p] = v[u];
+        }
+      }
+    }
+  } else {
+    // L is a power of two for the strided passes
+    const int lsh = __ffs(L) - 1;
+    for (int e0 = threadIdx.x; e0 < tot; e0 += kIpThreads * kBatch) {
+      C v[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int e = e0 + u * kIpThreads;
+        const int p = e >> lsh, l = e & (L - 1);
+        if (e < tot && l < nl) v[u] = A.src[base + (long)p * A.inner + l];
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int e = e0 + u * kIpThreads;
+        const int p = e >> lsh, l = e & (L - 1);
+        if (e < tot && l < nl) buf[l * LS + p] = v[u];
+      }
+    }
+  }
+  __syncthreads();
+  const bool inv = (KIND == kPassBwd || KIND == kPassBwdCombine);
+  run_inplace<T, KE>(buf, tw, A, LS, nl, inv);
+  if (KIND == kPassKin || KIND == kPassPrec) {
+    const int lsh = __ffs(L) - 1;
+    for (int e = threadIdx.x; e < nl * n; e += kIpThreads) {
+      const int l = e % nl, p = e / nl;
+      const long inner = i0 + l;
+      T k2;
+      if (A.ksqr) {
+        k2 = A.ksqr[(long)p * A.inner + inner];
+      } else {
+        const long jy = inner / A.nx, jx = inner - jy * A.nx;
+        k2 = A.kx[jx] + A.ky[jy] + A.kz[p];
+        if (k2 > A.kcap) k2 = A.kcap;
+      }
+      if (KIND == kPassPrec) k2 = A.pa / ((T)1 + k2 * A.pb);
+      C v = buf[l * LS + p];
+      v.x *= k2;
+      v.y *= k2;
+      buf[l * LS + p] = v;
+    }
+    (void)lsh;
+    __syncthreads();
+    run_inplace<T, KE>(buf, tw, A, LS, nl, true);
+  }
+  if (KIND == kPassBwdCombine) {
+    const int totv = nl * n;
+    for (int e0 = threadIdx.x; e0 < totv; e0 += kIpThreads * kBatch) {
+      C pin[kBatch];
+      T vv[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int e = e0 + u * kIpThreads;
+        if (e < totv) {
+          const int l = e / n, p = e - l * n;
+          const long row = (line0 + l) % A.vloc_rows;
+          vv[u] = A.vloc[row * n + p];
+          pin[u] = A.psi_in[base + (long)e];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int e = e0 + u * kIpThreads;
+        if (e < totv) {
+          const int l = e / n, p = e - l * n;
+          const C t = buf[l * LS + p];
+          C o;
+          o.x = A.a1 * (t.x + vv[u] * pin[u].x) + A.a0 * pin[u].x;
+          o.y = A.a1 * (t.y + vv[u] * pin[u].y) + A.a0 * pin[u].y;
+          A.dst[base + (long)e] = o;
+        }
+      }
+    }
+  } else if (A.xaxis) {
+    const int totv = nl * n;
+    for (int e = threadIdx.x; e < totv; e += kIpThreads) {
+      const int l = e / n, p = e - l * n;
+      A.dst[base + (long)e] = buf[l * LS + p];
+    }
+  } else {
+    const int lsh = __ffs(L) - 1;
+    for (int e = threadIdx.x; e < tot; e += kIpThreads) {
+      const int p = e >> lsh, l = e & (L - 1);
+      if (l < nl) A.dst[base + (long)p * A.inner + l] = buf[l * LS + p];
+    }
+  }
+}
+
+static bool inplace_supported(const AxisPlan &pl) {
+  for (int i = 0; i < pl.nf; ++i) {
+    const int r = pl.f[i];
+    if (!(r == 2 || r == 3 || r == 4 || r == 5 || r == 8)) return false;
+  }
+  return true;
+}
+
 static size_t axis_lds_bytes(int n, int lines, size_t elem) {
   const int LS = n | 1;
   return (size_t)(2 * lines * LS + n) * elem;
 }
 
+// Tile shape of the in-place kernel: the largest tile within 64 KiB of LDS
+// and kIpThreads*KE elements; y/z tiles take a power-of-two number of
+// consecutive inner indices (512-byte rows when the length allows).
 template <typename T>
-int fft_pass(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src, cplx_t<T> *dst, long nbatch,
-             const cplx_t<T> *psi_in, T a1, T a0) {
+static int fft_pass_inplace(fbse_ctx *c, int axis, int kind, AxisArgs<T> A, long nbatch, int lines_req) {
+  constexpr int KE = sizeof(T) == 4 ? 16 : 8;
+  const int n = A.n, LS = n | 1;
+  const long cap_lds = (65536 / (long)sizeof(cplx_t<T>) - n) / LS;
+  const long cap_e = (long)kIpThreads * KE / n;
+  long L = std::min(cap_lds, cap_e);
+  if (L < 1) return set_err(c, FBSE_ERR_UNSUPPORTED, "axis length %d too long for one tile", n);
+  if (lines_req > 0) L = std::min(L, (long)lines_req);
+  long ntiles;
+  if (axis == 0) {
+    L = std::min(L, 64L);
+    A.xaxis = 1;
+    A.inner = 1;
+    A.total_lines = nbatch * c->ny * c->nz;
+    if (L > A.total_lines) L = A.total_lines;
+    ntiles = (A.total_lines + L - 1) / L;
+  } else {
+    const long lmax = sizeof(T) == 4 ? 64 : 32;
+    L = std::min(L, lmax);
+    long pw = 1;
+    while (pw * 2 <= L) pw *= 2;
+    L = pw;
+    A.xaxis = 0;
+    A.inner = axis == 1 ? c->nx : c->nx * c->ny;
+    A.inner_count = A.inner;
+    A.outer_stride = A.inner * n;
+    A.tiles_per_outer = (A.inner_count + L - 1) / L;
+    ntiles = (axis == 1 ? nbatch * c->nz : nbatch) * A.tiles_per_outer;
+  }
+  A.lines = (int)L;
+  const size_t lds = (size_t)(L * LS + n) * sizeof(cplx_t<T>);
+  hipEvent_t ev = nullptr;
+  const int pk = (kind == kPassBwdCombine) ? 1 : 0;
+  prof_begin(c, pk, &ev);
+  switch (kind) {
+    case kPassFwd:
+      hipLaunchKernelGGL((fft_axis_inplace_kernel<T, kPassFwd, KE>), dim3(ntiles), dim3(kIpThreads), lds, c->stream, A);
+      break;
+    case kPassBwd:
+      hipLaunchKernelGGL((fft_axis_inplace_kernel<T, kPassBwd, KE>), dim3(ntiles), dim3(kIpThreads), lds, c->stream, A);
+      break;
+    case kPassKin:
+      hipLaunchKernelGGL((fft_axis_inplace_kernel<T, kPassKin, KE>), dim3(ntiles), dim3(kIpThreads), lds, c->stream, A);
+      break;
+    case kPassPrec:
+      hipLaunchKernelGGL((fft_axis_inplace_kernel<T, kPassPrec, KE>), dim3(ntiles), dim3(kIpThreads), lds, c->stream,
+                         A);
+      break;
+    default:
+      hipLaunchKernelGGL((fft_axis_inplace_kernel<T, kPassBwdCombine, KE>), dim3(ntiles), dim3(kIpThreads), lds,
+                         c->stream, A);
+      break;
+  }
+  FBSE_HIP(c, hipGetLastError());
+  prof_end(c, pk, ev);
+  return FBSE_OK;
+}
+
+template <typename T>
+static int fft_pass_variant(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src, cplx_t<T> *dst, long nbatch,
+                            const cplx_t<T> *psi_in, T a1, T a0, int variant, int lines_req) {
   const AxisPlan &pl = c->plan[axis];
   AxisArgs<T> A{};
   A.src = src;
@@ -360,14 +675,15 @@ int fft_pass(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src, cplx_t<T> *d
   A.n = pl.n;
   A.nf = pl.nf;
   for (int i = 0; i < pl.nf; ++i) A.f[i] = pl.f[i];
-  // lines per tile: the tuned value, clamped so the tile never exceeds the data
-  int lines = pl.lines;
+  // lines per tile: the requested value, clamped so the tile never exceeds the data
+  int lines = lines_req > 0 ? lines_req : pl.lines;
   A.psi_in = psi_in;
   A.a1 = a1;
   A.a0 = a0;
   A.pa = a1;  // the preconditioner pass takes (pa, pb) in the (a1, a0) slots
   A.pb = a0;
   A.vloc_rows = c->nspin * c->nz * c->ny;
+  if (variant == 1) return fft_pass_inplace<T>(c, axis, kind, A, nbatch, lines_req);
   long ntiles;
   if (axis == 0) {
     A.xaxis = 1;
@@ -420,6 +736,95 @@ int fft_pass(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src, cplx_t<T> *d
   FBSE_HIP(c, hipGetLastError());
   prof_end(c, pk, ev);
   return FBSE_OK;
+}
+
+
+static int pass_class(int kind) { return kind == kPassKin || kind == kPassPrec ? 1 : (kind == kPassBwdCombine ? 2 : 0); }
+
+// Autotuner: the first call of each (precision, axis, pass class) times every
+// candidate kernel/tile on the real batch, writing into a scratch buffer so
+// the caller's data is untouched, and keeps the fastest.
+template <typename T>
+static int tune_pass(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src, long nbatch, const cplx_t<T> *psi_in,
+                     T a1, T a0) {
+  const int pi = sizeof(T) == 8 ? 0 : 1, cls = pass_class(kind);
+  const int bk_ = nbatch * c->ngrid >= (32L << 20) ? 1 : 0;
+  const AxisPlan &pl = c->plan[axis];
+  std::vector<std::pair<int, int>> cand;
+  const int base_lines[] = {4, 8, 16, 32, 64};
+  for (int L : base_lines) cand.push_back({0, L});
+  if (inplace_supported(pl))
+    for (int L : base_lines) cand.push_back({1, L});
+  const size_t bytes = (size_t)nbatch * c->ngrid * sizeof(cplx_t<T>);
+  cplx_t<T> *scratch = nullptr;
+  if (hipMalloc(&scratch, bytes) != hipSuccess) {
+    hipGetLastError();
+    c->tune_kernel[pi][bk_][axis][cls] = 0;
+    c->tune_lines[pi][bk_][axis][cls] = pl.lines;
+    return FBSE_OK;  // no room to tune: keep the default ping-pong tile
+  }
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  const int saved_profile = c->profile;
+  c->profile = 0;
+  double best = 1e30;
+  int bk = 0, bl = pl.lines;
+  for (auto &cd : cand) {
+    double tmin = 1e30;
+    bool ok = true;
+    for (int rep = 0; rep < 3 && ok; ++rep) {
+      hipEventRecord(e0, c->stream);
+      int rc = fft_pass_variant<T>(c, axis, kind, src, scratch, nbatch, psi_in, a1, a0, cd.first, cd.second);
+      hipEventRecord(e1, c->stream);
+      if (rc) {
+        ok = false;
+        c->err.clear();
+        break;
+      }
+      hipEventSynchronize(e1);
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, e0, e1);
+      tmin = std::min(tmin, (double)ms);
+    }
+    if (ok && tmin < best) {
+      best = tmin;
+      bk = cd.first;
+      bl = cd.second;
+    }
+  }
+  c->profile = saved_profile;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipStreamSynchronize(c->stream);
+  hipFree(scratch);
+  c->tune_kernel[pi][bk_][axis][cls] = bk;
+  c->tune_lines[pi][bk_][axis][cls] = bl;
+  c->tune_ms[pi][bk_][axis][cls] = best;
+  return FBSE_OK;
+}
+
+template <typename T>
+int fft_pass(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src, cplx_t<T> *dst, long nbatch,
+             const cplx_t<T> *psi_in, T a1, T a0) {
+  const int pi = sizeof(T) == 8 ? 0 : 1, cls = pass_class(kind);
+  const int bk = nbatch * c->ngrid >= (32L << 20) ? 1 : 0;
+  if (c->force_variant >= 0) {
+    const int v = (c->force_variant == 1 && inplace_supported(c->plan[axis])) ? 1 : 0;
+    return fft_pass_variant<T>(c, axis, kind, src, dst, nbatch, psi_in, a1, a0, v, c->force_lines);
+  }
+  if (c->tune_lines[pi][bk][axis][cls] == 0) {
+    if (c->autotune && nbatch * c->ngrid >= (1L << 20)) {
+      int rc = tune_pass<T>(c, axis, kind, src, nbatch, psi_in, a1, a0);
+      if (rc) return rc;
+    } else {
+      // untuned default: the in-place kernel when its radices allow
+      return fft_pass_variant<T>(c, axis, kind, src, dst, nbatch, psi_in, a1, a0,
+                                 inplace_supported(c->plan[axis]) ? 1 : 0, 0);
+    }
+  }
+  return fft_pass_variant<T>(c, axis, kind, src, dst, nbatch, psi_in, a1, a0, c->tune_kernel[pi][bk][axis][cls],
+                             c->tune_lines[pi][bk][axis][cls]);
 }
 
 template <typename T>

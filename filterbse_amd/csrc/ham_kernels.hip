@@ -18,169 +18,186 @@ template <typename C> __device__ __forceinline__ C cjm(C a, C b) {
   return r;
 }
 
-__device__ __forceinline__ void atomic_add_real(double *p, double v) { unsafeAtomicAdd(p, v); }
-__device__ __forceinline__ void atomic_add_real(float *p, float v) { unsafeAtomicAdd(p, v); }
 
 // ---------------------------------------------------------------------------
-// Projectors.  One work-group per (atom, chunk of kProjCols columns); a column
-// is one (state, spin) grid.  For every column it forms the 30 projections
-//   SO rows r = p*3+m : c = sum_g proj_p(g) conj(Y_m(g)) psi(g)
-//   NL rows r = 15+p*3+m : c = sum_g NL_p(g) conj(Y_m(g)) psi(g)
-// (phamiltonian.c:199-226, 372-404), mixes them (L.S for SO, +-1 for NL,
-// times dv), and scatters  out(g) += sum_rows beta_row(g) d_row  with
-// hardware float atomics (phamiltonian.c:230-306, 406-438: the reference
-// scatters with omp atomics as well).  Grid points are staged in LDS in
-// tiles of kProjPts; the 30 x 16 projections are a small GEMM over the tile.
+// Projectors (phamiltonian.c:140-444).  For every atom and every column (one
+// (state, spin) grid) the 30 projections
+//   SO rows r = p*3+m    : c = dv sum_g proj_p(g) conj(Y_m(g)) psi(g)
+//   NL rows r = 15+p*3+m : c = dv sum_g NL_p(g)   conj(Y_m(g)) psi(g)
+// are mixed (L.S across the two spins of a state for SO, the projector sign
+// for NL) and expanded back onto the sphere:
+//   out(g) += sum_{p,m} Y_m(g) (proj_p(g) d_so[p][m] + NL_p(g) d_nl[p][m]).
+// Atoms are processed colour by colour (one launch per colour); atoms of one
+// colour share no grid point, so the expansion is a plain read-modify-write
+// with a deterministic summation order (the reference scatters with omp
+// atomics, phamiltonian.c:295-303,427-435).
+// Work-group = one atom x kPjCols columns; thread = (column, point lane): it
+// walks every kPjLanes-th point of the sphere holding all 30 projections of
+// its column in registers; the lanes are then summed through LDS.
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(256) void proj_kernel(ProjArgs<T> P) {
+constexpr int kPjCols = 16;
+constexpr int kPjLanes = 16;  // kPjCols * kPjLanes = 256 threads
+
+template <typename T, int MAXP>
+__global__ __launch_bounds__(256) void proj_color_kernel(ProjArgs<T> P) {
   using C = cplx_t<T>;
-  __shared__ C s_psi[kProjPts][kProjCols + 1];
-  __shared__ T s_pj[5][kProjPts];
-  __shared__ T s_nl[5][kProjPts];
-  __shared__ C s_y[3][kProjPts];
-  __shared__ int s_j[kProjPts];
-  __shared__ C s_c[kProjRows][kProjCols];
-  __shared__ C s_d[kProjRows][kProjCols];
-
-  const int atom = blockIdx.x;
-  const int col0 = blockIdx.y * kProjCols;
-  const int ncol = min(kProjCols, P.ncols - col0);
+  __shared__ C s_d[kProjRows][kPjCols];
+  const int atom = P.atoms[blockIdx.x];
+  const int col0 = blockIdx.y * kPjCols;
+  const int lane = threadIdx.x % kPjLanes;   // point lane (fastest: consecutive points)
+  const int cc = threadIdx.x / kPjLanes;     // column within the chunk
+  const int col = col0 + cc;
+  const bool live = col < P.ncols;
   const int off = P.a_off[atom], cnt = P.a_cnt[atom];
-  const int tid = threadIdx.x;
-  const int nout = kProjRows * kProjCols;  // 480 outputs, two per thread
+  const int st = live ? col / P.nspin : 0, sp = live ? col - st * P.nspin : 0;
+  const C *psi = P.psi_in + (long)st * P.nsg + (long)sp * P.ngrid;
+  const int nso = P.so ? P.n_so : 0, nnl = P.nl ? P.n_nl : 0;
 
-  C acc0, acc1;
-  acc0.x = acc0.y = 0;
-  acc1.x = acc1.y = 0;
-  const int o0 = tid, o1 = tid + 256;
-  const int r0 = o0 / kProjCols, c0 = o0 % kProjCols;
-  const int r1 = o1 / kProjCols, c1 = o1 % kProjCols;
-
-  for (int t0 = 0; t0 < cnt; t0 += kProjPts) {
-    const int npt = min(kProjPts, cnt - t0);
-    for (int e = tid; e < kProjPts; e += 256) {
-      if (e < npt) {
-        const int idx = off + t0 + e;
-        s_j[e] = P.jxyz[idx];
-        for (int p = 0; p < 5; ++p) {
-          s_pj[p][e] = P.proj[(long)p * P.npts + idx];
-          s_nl[p][e] = P.nlproj[(long)p * P.npts + idx];
-        }
-        for (int m = 0; m < 3; ++m) s_y[m][e] = P.y1[(long)m * P.npts + idx];
-      }
-    }
-    __syncthreads();
-    for (int e = tid; e < kProjPts * kProjCols; e += 256) {
-      const int pt = e % kProjPts, cc = e / kProjPts;
-      C v;
-      v.x = v.y = 0;
-      if (pt < npt && cc < ncol) {
-        const int col = col0 + cc;
-        const int st = col / P.nspin, sp = col - st * P.nspin;
-        v = P.psi_in[(long)st * P.nsg + (long)sp * P.ngrid + s_j[pt]];
-      }
-      s_psi[pt][cc] = v;
-    }
-    __syncthreads();
-    // partial projections over this tile
-    {
-      const int rr[2] = {r0, r1};
-      const int cc[2] = {c0, c1};
+  // acc[p*3+m] (SO) and acc[3*MAXP + p*3+m] (NL) for p < MAXP radial projectors
+  constexpr int NR = 6 * MAXP;
+  C acc[NR];
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const int o = q ? o1 : o0;
-        if (o >= nout) continue;
-        const int r = rr[q];
-        const int so = r < 15;
-        const int p = (so ? r : r - 15) / 3, m = (so ? r : r - 15) % 3;
-        C a;
-        a.x = a.y = 0;
-        for (int pt = 0; pt < npt; ++pt) {
-          const T w = so ? s_pj[p][pt] : s_nl[p][pt];
-          const C t = cjm(s_y[m][pt], s_psi[pt][cc[q]]);  // conj(Y) psi
-          a.x += w * t.x;
-          a.y += w * t.y;
-        }
-        if (q) {
-          acc1.x += a.x;
-          acc1.y += a.y;
-        } else {
-          acc0.x += a.x;
-          acc0.y += a.y;
-        }
-      }
-    }
-    __syncthreads();
-  }
-  if (o0 < nout) s_c[r0][c0] = acc0;
-  if (o1 < nout) s_c[r1][c1] = acc1;
-  __syncthreads();
-
-  // mix: SO via L.S (coupling the two spins of one state), NL via the sign
-  for (int o = tid; o < nout; o += 256) {
-    const int r = o / kProjCols, cc = o % kProjCols;
-    C d;
-    d.x = d.y = 0;
-    if (r < 15) {
-      const int p = r / 3, m = r % 3;
-      if (P.so && p < P.n_so && P.nspin == 2) {
-        const int sp = (col0 + cc) % 2;  // output spin s of this column
-        const int cbase = cc - sp;       // column of spin 0 of the same state
-        const int j = sp * 3 + m;
-        for (int jp = 0; jp < 6; ++jp) {
-          const int spp = jp / 3, mpp = jp % 3;
-          const C ls = P.ls[jp * 6 + j];
-          if (ls.x == (T)0 && ls.y == (T)0) continue;
-          C t = cm(ls, s_c[p * 3 + mpp][cbase + spp]);
-          d.x += t.x;
-          d.y += t.y;
-        }
-      }
-    } else {
-      const int p = (r - 15) / 3;
-      if (P.nl && p < P.n_nl) {
-        const T sg = (T)P.sign[atom * 5 + p];
-        d = s_c[r][cc];
-        d.x *= sg;
-        d.y *= sg;
-      }
-    }
-    d.x *= P.scale;
-    d.y *= P.scale;
-    s_d[r][cc] = d;
-  }
-  __syncthreads();
-
-  // expand and scatter: out(g) += sum_{p,m} proj_p Y_m d_so + NL_p Y_m d_nl
-  for (int t0 = 0; t0 < cnt; t0 += kProjPts) {
-    const int npt = min(kProjPts, cnt - t0);
-    for (int e = tid; e < npt * kProjCols; e += 256) {
-      const int pt = e % npt, cc = e / npt;
-      if (cc >= ncol) continue;
-      const int idx = off + t0 + pt;
-      const int col = col0 + cc;
-      const int st = col / P.nspin, sp = col - st * P.nspin;
-      C acc;
-      acc.x = acc.y = 0;
+  for (int r = 0; r < NR; ++r) acc[r].x = acc[r].y = 0;
+  if (live) {
+    for (int g = lane; g < cnt; g += kPjLanes) {
+      const int idx = off + g;
+      const C v = psi[P.jxyz[idx]];
+      C u[3];
+#pragma unroll
       for (int m = 0; m < 3; ++m) {
-        C s;
-        s.x = s.y = 0;
-        for (int p = 0; p < 5; ++p) {
-          const T wso = P.proj[(long)p * P.npts + idx];
-          const T wnl = P.nlproj[(long)p * P.npts + idx];
-          const C dso = s_d[p * 3 + m][cc], dnl = s_d[15 + p * 3 + m][cc];
-          s.x += wso * dso.x + wnl * dnl.x;
-          s.y += wso * dso.y + wnl * dnl.y;
-        }
-        const C t = cm(P.y1[(long)m * P.npts + idx], s);
-        acc.x += t.x;
-        acc.y += t.y;
+        const C y = P.y1[(long)m * P.npts + idx];
+        u[m].x = y.x * v.x + y.y * v.y;  // conj(Y) psi
+        u[m].y = y.x * v.y - y.y * v.x;
       }
-      T *dst = reinterpret_cast<T *>(&P.psi_out[(long)st * P.nsg + (long)sp * P.ngrid + P.jxyz[idx]]);
-      atomic_add_real(dst, acc.x);
-      atomic_add_real(dst + 1, acc.y);
+#pragma unroll
+      for (int p = 0; p < MAXP; ++p) {
+        if (p < nso) {
+          const T w = P.proj[(long)p * P.npts + idx];
+#pragma unroll
+          for (int m = 0; m < 3; ++m) {
+            acc[p * 3 + m].x += w * u[m].x;
+            acc[p * 3 + m].y += w * u[m].y;
+          }
+        }
+        if (p < nnl) {
+          const T w = P.nlproj[(long)p * P.npts + idx];
+#pragma unroll
+          for (int m = 0; m < 3; ++m) {
+            acc[3 * MAXP + p * 3 + m].x += w * u[m].x;
+            acc[3 * MAXP + p * 3 + m].y += w * u[m].y;
+          }
+        }
+      }
     }
+  }
+  // sum the point lanes: the kPjLanes lanes of a column are consecutive lanes
+  // of one wavefront, reduced with butterfly shuffles
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+#pragma unroll
+    for (int o = kPjLanes / 2; o > 0; o >>= 1) {
+      acc[r].x += __shfl_xor(acc[r].x, o, kPjLanes);
+      acc[r].y += __shfl_xor(acc[r].y, o, kPjLanes);
+    }
+  }
+  // raw projections into s_d (30 rows x columns; rows of absent projectors 0)
+  for (int r = lane; r < kProjRows; r += kPjLanes) s_d[r][cc] = C{};
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 3 * MAXP; ++r)
+    if (lane == r % kPjLanes) {
+      s_d[r][cc] = acc[r];
+      s_d[15 + r][cc] = acc[3 * MAXP + r];
+    }
+  __syncthreads();
+  // mix: SO with L.S (needs both spins of the state), NL with the sign; x dv*a1
+  C dmix[2];
+  if (lane < kProjRows / 2) {
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      const int r = half * 15 + lane;
+      C d;
+      d.x = d.y = 0;
+      if (r < 15) {
+        const int p = r / 3, m = r % 3;
+        if (p < nso && P.nspin == 2) {
+          const int spo = (col0 + cc) % 2;
+          const int cbase = cc - spo;
+          const int j = spo * 3 + m;
+          for (int jp = 0; jp < 6; ++jp) {
+            const C ls = P.ls[jp * 6 + j];
+            if (ls.x == (T)0 && ls.y == (T)0) continue;
+            const C c = s_d[p * 3 + jp % 3][cbase + jp / 3];
+            d.x += ls.x * c.x - ls.y * c.y;
+            d.y += ls.x * c.y + ls.y * c.x;
+          }
+        }
+      } else {
+        const int p = (r - 15) / 3;
+        if (p < nnl) {
+          const T sg = (T)P.sign[atom * 5 + p];
+          d = s_d[r][cc];
+          d.x *= sg;
+          d.y *= sg;
+        }
+      }
+      d.x *= P.scale;
+      d.y *= P.scale;
+      dmix[half] = d;
+    }
+  }
+  __syncthreads();
+  if (lane < kProjRows / 2) {
+    s_d[lane][cc] = dmix[0];
+    s_d[15 + lane][cc] = dmix[1];
+  }
+  __syncthreads();
+  if (!live) return;
+  // expansion: this thread's column, every kPjLanes-th point
+  C d[NR];
+#pragma unroll
+  for (int r = 0; r < 3 * MAXP; ++r) {
+    d[r] = s_d[r][cc];
+    d[3 * MAXP + r] = s_d[15 + r][cc];
+  }
+  C *out = P.psi_out + (long)st * P.nsg + (long)sp * P.ngrid;
+  for (int g = lane; g < cnt; g += kPjLanes) {
+    const int idx = off + g;
+    C s3[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) s3[m].x = s3[m].y = 0;
+#pragma unroll
+    for (int p = 0; p < MAXP; ++p) {
+      if (p < nso) {
+        const T w = P.proj[(long)p * P.npts + idx];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          s3[m].x += w * d[p * 3 + m].x;
+          s3[m].y += w * d[p * 3 + m].y;
+        }
+      }
+      if (p < nnl) {
+        const T w = P.nlproj[(long)p * P.npts + idx];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+          s3[m].x += w * d[3 * MAXP + p * 3 + m].x;
+          s3[m].y += w * d[3 * MAXP + p * 3 + m].y;
+        }
+      }
+    }
+    C a;
+    a.x = a.y = 0;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const C y = P.y1[(long)m * P.npts + idx];
+      a.x += y.x * s3[m].x - y.y * s3[m].y;
+      a.y += y.x * s3[m].y + y.y * s3[m].x;
+    }
+    const int j = P.jxyz[idx];
+    C o = out[j];
+    o.x += a.x;
+    o.y += a.y;
+    out[j] = o;
   }
 }
 
@@ -212,10 +229,20 @@ int proj_apply(fbse_ctx *c, const cplx_t<T> *psi_in, cplx_t<T> *psi_out, long ns
   P.ncols = (int)(nstates * c->nspin);
   P.scale = (T)(c->dv * a1);
   if (P.n_atoms == 0 || P.ncols == 0) return FBSE_OK;
-  dim3 grid(P.n_atoms, (P.ncols + kProjCols - 1) / kProjCols);
   hipEvent_t ev = nullptr;
   prof_begin(c, 2, &ev);
-  hipLaunchKernelGGL(proj_kernel<T>, grid, dim3(256), 0, c->stream, P);
+  const int nchunk = (P.ncols + kPjCols - 1) / kPjCols;
+  const int ncol = (int)pt->color_off.size() - 1;
+  for (int k = 0; k < ncol; ++k) {
+    const int na = pt->color_off[k + 1] - pt->color_off[k];
+    if (na == 0) continue;
+    P.atoms = pt->color_atoms + pt->color_off[k];
+    const int nso = P.so ? P.n_so : 0, nnl = P.nl ? P.n_nl : 0;
+    if (nso <= 1 && nnl <= 1)
+      hipLaunchKernelGGL((proj_color_kernel<T, 1>), dim3(na, nchunk), dim3(256), 0, c->stream, P);
+    else
+      hipLaunchKernelGGL((proj_color_kernel<T, 5>), dim3(na, nchunk), dim3(256), 0, c->stream, P);
+  }
   FBSE_HIP(c, hipGetLastError());
   prof_end(c, 2, ev);
   return FBSE_OK;

@@ -168,6 +168,11 @@ int fbse_profile_read(fbse_ctx *ctx, int kind, long *launches, double *total_ms)
  * chosen by the built-in autotuner at the first call; this forces K (0 = auto). */
 int fbse_set_window(fbse_ctx *ctx, int window);
 
+/* FFT kernel selection: variant -1 = autotune at first use (default), 0 = the
+ * two-buffer LDS kernel, 1 = the in-place register-staged kernel (lengths with
+ * radices 2,3,4,5,8 only; others fall back to 0); lines > 0 fixes the tile width. */
+int fbse_set_fft_kernel(fbse_ctx *ctx, int variant, int lines);
+
 /* Device memory helpers for callers without a GPU runtime of their own. */
 int fbse_dev_alloc(fbse_ctx *ctx, size_t bytes, void **d_ptr);
 int fbse_dev_free(fbse_ctx *ctx, void *d_ptr);

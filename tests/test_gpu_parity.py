@@ -23,11 +23,15 @@ def system(request):
     return request.param, s, oracle_ham(s), gpu_context(s)
 
 
-def test_apply_h(system):
+@pytest.mark.parametrize("variant,lines", [(0, 0), (0, 4), (1, 0), (1, 8), (-1, 0)])
+def test_apply_h(system, variant, lines):
+    """Both FFT kernels and several tile widths give the same H."""
     name, s, h, g = system
     psi = random_states(s, 3, seed=11)
     ref = h.apply(psi)
+    g.set_fft_kernel(variant, lines)
     out = g.apply_h(psi)
+    g.set_fft_kernel(-1, 0)
     assert rel(out, ref) < 1e-11, name
 
 
