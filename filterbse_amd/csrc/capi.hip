@@ -794,6 +794,35 @@ struct LinAlg {
     }
     return FBSE_OK;
   }
+  // Out (m x nout) = A[:, 0:nin] * V (nin x nout, ldv)
+  int product(const double *A, long nin, const double *V, long ldv, long nout, double *Out) {
+    if (cplx) {
+      const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
+      FBSE_BLAS(c, rocblas_zgemm(c->blas, rocblas_operation_none, rocblas_operation_none, (int)m, (int)nout,
+                                 (int)nin, &one, (const rocblas_double_complex *)A, (int)m,
+                                 (const rocblas_double_complex *)V, (int)ldv, &zero, (rocblas_double_complex *)Out,
+                                 (int)m));
+    } else {
+      const double one = 1.0, zero = 0.0;
+      FBSE_BLAS(c, rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, (int)m, (int)nout,
+                                 (int)nin, &one, A, (int)m, V, (int)ldv, &zero, Out, (int)m));
+    }
+    return FBSE_OK;
+  }
+  // Out (r x nout) = X (r x r2, ldx = r) * Y (r2 x nout, ldy = r2): small dense products
+  int small_product(const double *X, long r, long r2, const double *Y, long nout, double *Out) {
+    if (cplx) {
+      const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
+      FBSE_BLAS(c, rocblas_zgemm(c->blas, rocblas_operation_none, rocblas_operation_none, (int)r, (int)nout, (int)r2,
+                                 &one, (const rocblas_double_complex *)X, (int)r, (const rocblas_double_complex *)Y,
+                                 (int)r2, &zero, (rocblas_double_complex *)Out, (int)r));
+    } else {
+      const double one = 1.0, zero = 0.0;
+      FBSE_BLAS(c, rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, (int)r, (int)nout, (int)r2,
+                                 &one, X, (int)r, Y, (int)r2, &zero, Out, (int)r));
+    }
+    return FBSE_OK;
+  }
   // A[:, 0:nout] = A[:, 0:nin] * V (nin x nout, ldv), in place by row blocks
   int rotate(double *A, long nin, const double *V, long ldv, long nout) {
     long rb = 32768;
@@ -878,17 +907,18 @@ static int ortho_dev_impl(fbse_ctx *c, double *A, long n, double svd_eps, long *
     hipFree(Z);
     hipFree(w);
   };
-  // pass 1: G = A^H A -> V ; B = A V
-  if ((rc = la.gram(A, n, A, n, G, n)) || (rc = la.eigh(G, n, w)) || (rc = la.rotate(A, n, G, n, n))) {
+  // pass 1: G = A^H A -> V (columns of B = A V are orthogonal, |B_i| = s_i)
+  if ((rc = la.gram(A, n, A, n, G, n)) || (rc = la.eigh(G, n, w))) {
     cleanup();
     return rc;
   }
   // The first Gram eigensolve resolves eigenvalues only to eps*||G||, i.e.
   // singular values down to ~1e-8 s_0.  Columns of B = A V whose eigenvalue is
-  // below kSmall*lambda_max (the leading, ascending ones) are re-measured by a
-  // second Gram over that block alone, where the eigensolver's error scales
-  // with the block's own norm; this resolves s_i/s_0 to ~1e-12, enough for
-  // the 1e-10 cut of ortho.c:68-75.
+  // below kSmall*lambda_max (the leading, ascending ones) are formed
+  // explicitly, B_s = A V_s, and re-measured by a second Gram over that block
+  // alone, where the eigensolver's error scales with the block's own norm;
+  // this resolves s_i/s_0 to ~1e-12, enough for the 1e-10 cut of
+  // ortho.c:68-75.
   FBSE_HIP(c, hipMemcpyAsync(hw.data(), w, n * 8, hipMemcpyDeviceToHost, c->stream));
   FBSE_HIP(c, hipStreamSynchronize(c->stream));
   const double kSmall = 1e-8;
@@ -897,7 +927,16 @@ static int ortho_dev_impl(fbse_ctx *c, double *A, long n, double svd_eps, long *
   while (ns < n && hw[ns] < kSmall * lmax) ++ns;
   std::vector<double> mu;
   if (ns > 0) {
-    if ((rc = la.gram(A, ns, A, ns, Z, ns))) {
+    double *Bs = nullptr;
+    if (hipMalloc(&Bs, (size_t)la.m * ns * es) != hipSuccess) {
+      cleanup();
+      return set_err(c, FBSE_ERR_NOMEM, "ortho: %ld x %ld block of small singular directions", la.m, ns);
+    }
+    rc = la.product(A, n, G, n, ns, Bs);
+    if (!rc) rc = la.gram(Bs, ns, Bs, ns, Z, ns);
+    hipStreamSynchronize(c->stream);
+    hipFree(Bs);
+    if (rc) {
       cleanup();
       return rc;
     }
@@ -928,7 +967,8 @@ static int ortho_dev_impl(fbse_ctx *c, double *A, long n, double svd_eps, long *
     cleanup();
     return set_err(c, FBSE_ERR_SOLVER, "ortho: all singular values vanish");
   }
-  // Zsel (n x k): large column i -> e_{col}/s ; small kept j -> W_s[:, j]/s
+  // Zsel (n x k) in the basis of B's columns: large column i -> e_col/s ;
+  // small kept j -> W_s[:, j]/s.  U = B Zsel = A (V Zsel): one pass over A.
   {
     std::vector<double> zh((size_t)n * k * (la.cplx ? 2 : 1), 0.0);
     std::vector<double> ws;
@@ -950,11 +990,18 @@ static int ortho_dev_impl(fbse_ctx *c, double *A, long n, double svd_eps, long *
           for (int t = 0; t < cw; ++t) zh[((size_t)i * n + r) * cw + t] = ws[((size_t)j * ns + r) * cw + t] * inv;
       }
     }
-    FBSE_HIP(c, hipMemcpyAsync(G, zh.data(), zh.size() * 8, hipMemcpyHostToDevice, c->stream));
-  }
-  if ((rc = la.rotate(A, n, G, n, k))) {
-    cleanup();
-    return rc;
+    FBSE_HIP(c, hipMemcpyAsync(Z, zh.data(), zh.size() * 8, hipMemcpyHostToDevice, c->stream));
+    // W = V Zsel (n x k)
+    double *W = nullptr;
+    FBSE_HIP(c, hipMalloc(&W, (size_t)n * k * es));
+    rc = la.small_product(G, n, n, Z, k, W);
+    if (!rc) rc = la.rotate(A, n, W, n, k);
+    hipStreamSynchronize(c->stream);
+    hipFree(W);
+    if (rc) {
+      cleanup();
+      return rc;
+    }
   }
   // pass 3: Cholesky re-orthogonalisation U <- U R^-1 with U^H U = R^H R
   if ((rc = la.gram(A, k, A, k, G, k))) {
@@ -1016,13 +1063,23 @@ extern "C" int fbse_ortho(fbse_ctx *c, double *psitot, long n_cols, double svd_e
   });
 }
 
+// States of per_state bytes that fit in frac of the free device memory (>= 1).
+static long states_that_fit(const fbse_ctx *c, size_t per_state, double frac) {
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 1;
+  return std::max(1L, (long)(frac * (double)fr / (double)per_state));
+}
+
 static int rr_dev_impl(fbse_ctx *c, double *U, long n, double *eval) {
   LinAlg la{c, c->is_complex != 0, c->nsg};
   const long sl = stlen_of(c);
   const size_t es = la.es();
   int rc = FBSE_OK;
   double *Hs = nullptr, *w = nullptr, *HU = nullptr;
-  const long ch = std::max(1L, std::min(n, h_chunk(c)));
+  // H U is held for as many columns as memory allows: U^H (H U) over a narrow
+  // block of columns runs far below the GEMM peak, so one wide product is used
+  // whenever it fits (1024 states of a 96^3 spinor grid take 29 GB).
+  const long ch = std::min(n, states_that_fit(c, (size_t)sl * 8, 0.5));
   FBSE_HIP(c, hipMalloc(&Hs, (size_t)n * n * es));
   FBSE_HIP(c, hipMalloc(&w, (size_t)n * 8));
   FBSE_HIP(c, hipMalloc(&HU, (size_t)ch * sl * 8));

@@ -230,11 +230,17 @@ __device__ cplx_t<T> *run_passes(cplx_t<T> *a, cplx_t<T> *b, const cplx_t<T> *tw
   return a;
 }
 
-constexpr int kBatchPP = 8;  // global loads in flight per thread (two-buffer kernel)
+// Global loads a thread keeps in flight while it fills or drains a tile.  The
+// batch is held as separate real and imaginary arrays: an array of 16-byte
+// complex<double> structs is not split into registers by the compiler and
+// would live in scratch memory.
+template <typename T>
+constexpr int load_batch() { return 8; }
 
 template <typename T, int KIND>
 __global__ __launch_bounds__(kFftThreads) void fft_axis_kernel(AxisArgs<T> A) {
   using C = cplx_t<T>;
+  constexpr int NB = load_batch<T>();
   extern __shared__ unsigned char smem_raw[];
   const int n = A.n;
   const int L = A.lines;
@@ -264,36 +270,44 @@ __global__ __launch_bounds__(kFftThreads) void fft_axis_kernel(AxisArgs<T> A) {
   const int tot = L * n;
   if (A.xaxis) {
     const int totv = nl * n;
-    for (int e0 = threadIdx.x; e0 < totv; e0 += kFftThreads * kBatchPP) {
-      C v[kBatchPP];
+    for (int e0 = threadIdx.x; e0 < totv; e0 += kFftThreads * NB) {
+      T vr[NB], vi[NB];
 #pragma unroll
-      for (int u = 0; u < kBatchPP; ++u) {
+      for (int u = 0; u < NB; ++u) {
         const int e = e0 + u * kFftThreads;
-        if (e < totv) v[u] = A.src[base + (long)e];
+        if (e < totv) {
+          const C t = A.src[base + (long)e];
+          vr[u] = t.x;
+          vi[u] = t.y;
+        }
       }
 #pragma unroll
-      for (int u = 0; u < kBatchPP; ++u) {
+      for (int u = 0; u < NB; ++u) {
         const int e = e0 + u * kFftThreads;
         if (e < totv) {
           const int l = e / n, p = e - l * n;
-          buf0[l * LS + p] = v[u];
+          buf0[l * LS + p] = C{vr[u], vi[u]};
         }
       }
     }
   } else {
-    for (int e0 = threadIdx.x; e0 < tot; e0 += kFftThreads * kBatchPP) {
-      C v[kBatchPP];
+    for (int e0 = threadIdx.x; e0 < tot; e0 += kFftThreads * NB) {
+      T vr[NB], vi[NB];
 #pragma unroll
-      for (int u = 0; u < kBatchPP; ++u) {
+      for (int u = 0; u < NB; ++u) {
         const int e = e0 + u * kFftThreads;
         const int p = e / L, l = e - p * L;
-        if (e < tot && l < nl) v[u] = A.src[base + (long)p * A.inner + l];
+        if (e < tot && l < nl) {
+          const C t = A.src[base + (long)p * A.inner + l];
+          vr[u] = t.x;
+          vi[u] = t.y;
+        }
       }
 #pragma unroll
-      for (int u = 0; u < kBatchPP; ++u) {
+      for (int u = 0; u < NB; ++u) {
         const int e = e0 + u * kFftThreads;
         const int p = e / L, l = e - p * L;
-        if (e < tot && l < nl) buf0[l * LS + p] = v[u];
+        if (e < tot && l < nl) buf0[l * LS + p] = C{vr[u], vi[u]};
       }
     }
   }
@@ -330,28 +344,30 @@ __global__ __launch_bounds__(kFftThreads) void fft_axis_kernel(AxisArgs<T> A) {
 
   if (KIND == kPassBwdCombine) {
     const int totv = nl * n;
-    for (int e0 = threadIdx.x; e0 < totv; e0 += kFftThreads * kBatchPP) {
-      C pin[kBatchPP];
-      T vv[kBatchPP];
+    for (int e0 = threadIdx.x; e0 < totv; e0 += kFftThreads * NB) {
+      T pr[NB], pim[NB];
+      T vv[NB];
 #pragma unroll
-      for (int u = 0; u < kBatchPP; ++u) {
+      for (int u = 0; u < NB; ++u) {
         const int e = e0 + u * kFftThreads;
         if (e < totv) {
           const int l = e / n, p = e - l * n;
           const long row = (line0 + l) % A.vloc_rows;
           vv[u] = A.vloc[row * n + p];
-          pin[u] = A.psi_in[base + (long)e];
+          const C t = A.psi_in[base + (long)e];
+          pr[u] = t.x;
+          pim[u] = t.y;
         }
       }
 #pragma unroll
-      for (int u = 0; u < kBatchPP; ++u) {
+      for (int u = 0; u < NB; ++u) {
         const int e = e0 + u * kFftThreads;
         if (e < totv) {
           const int l = e / n, p = e - l * n;
           const C t = res[l * LS + p];
           C o;
-          o.x = A.a1 * (t.x + vv[u] * pin[u].x) + A.a0 * pin[u].x;
-          o.y = A.a1 * (t.y + vv[u] * pin[u].y) + A.a0 * pin[u].y;
+          o.x = A.a1 * (t.x + vv[u] * pr[u]) + A.a0 * pr[u];
+          o.y = A.a1 * (t.y + vv[u] * pim[u]) + A.a0 * pim[u];
           A.dst[base + (long)e] = o;
         }
       }
@@ -370,7 +386,6 @@ __global__ __launch_bounds__(kFftThreads) void fft_axis_kernel(AxisArgs<T> A) {
 }
 
 constexpr int kIpThreads = 512;  // threads of the in-place kernel
-constexpr int kBatch = 8;        // global loads in flight per thread while filling a tile
 
 // One in-place Stockham pass: every thread first reads (and twiddles) the R
 // inputs of each of its butterflies into registers, the work-group syncs, and
@@ -438,10 +453,11 @@ __device__ void run_inplace(cplx_t<T> *buf, const cplx_t<T> *tw, const AxisArgs<
 }
 
 // Same contract as fft_axis_kernel, one LDS buffer, tile of up to
-// kIpThreads*KE elements; radices limited to 2,3,4,5,7,8,11,13.
+// kIpThreads*KE elements; radices limited to 2, 3, 4, 5 and 8.
 template <typename T, int KIND, int KE>
 __global__ __launch_bounds__(kIpThreads) void fft_axis_inplace_kernel(AxisArgs<T> A) {
   using C = cplx_t<T>;
+  constexpr int NB = load_batch<T>();
   extern __shared__ unsigned char smem_raw[];
   const int n = A.n;
   const int L = A.lines;
@@ -466,41 +482,49 @@ __global__ __launch_bounds__(kIpThreads) void fft_axis_inplace_kernel(AxisArgs<T
     nl = rem < L ? (int)rem : L;
   }
   const int tot = L * n;
-  // tile load: kBatch independent global loads in flight per thread
+  // tile load: NB independent global loads in flight per thread
   if (A.xaxis) {
     const int totv = nl * n;
-    for (int e0 = threadIdx.x; e0 < totv; e0 += kIpThreads * kBatch) {
-      C v[kBatch];
+    for (int e0 = threadIdx.x; e0 < totv; e0 += kIpThreads * NB) {
+      T vr[NB], vi[NB];
 #pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
+      for (int u = 0; u < NB; ++u) {
         const int e = e0 + u * kIpThreads;
-        if (e < totv) v[u] = A.src[base + (long)e];
+        if (e < totv) {
+          const C t = A.src[base + (long)e];
+          vr[u] = t.x;
+          vi[u] = t.y;
+        }
       }
 #pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
+      for (int u = 0; u < NB; ++u) {
         const int e = e0 + u * kIpThreads;
         if (e < totv) {
           const int l = e / n, p = e - l * n;
-          buf[l * LS + p] = v[u];
+          buf[l * LS + p] = C{vr[u], vi[u]};
         }
       }
     }
   } else {
     // L is a power of two for the strided passes
     const int lsh = __ffs(L) - 1;
-    for (int e0 = threadIdx.x; e0 < tot; e0 += kIpThreads * kBatch) {
-      C v[kBatch];
+    for (int e0 = threadIdx.x; e0 < tot; e0 += kIpThreads * NB) {
+      T vr[NB], vi[NB];
 #pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
+      for (int u = 0; u < NB; ++u) {
         const int e = e0 + u * kIpThreads;
         const int p = e >> lsh, l = e & (L - 1);
-        if (e < tot && l < nl) v[u] = A.src[base + (long)p * A.inner + l];
+        if (e < tot && l < nl) {
+          const C t = A.src[base + (long)p * A.inner + l];
+          vr[u] = t.x;
+          vi[u] = t.y;
+        }
       }
 #pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
+      for (int u = 0; u < NB; ++u) {
         const int e = e0 + u * kIpThreads;
         const int p = e >> lsh, l = e & (L - 1);
-        if (e < tot && l < nl) buf[l * LS + p] = v[u];
+        if (e < tot && l < nl) buf[l * LS + p] = C{vr[u], vi[u]};
       }
     }
   }
@@ -532,28 +556,30 @@ __global__ __launch_bounds__(kIpThreads) void fft_axis_inplace_kernel(AxisArgs<T
   }
   if (KIND == kPassBwdCombine) {
     const int totv = nl * n;
-    for (int e0 = threadIdx.x; e0 < totv; e0 += kIpThreads * kBatch) {
-      C pin[kBatch];
-      T vv[kBatch];
+    for (int e0 = threadIdx.x; e0 < totv; e0 += kIpThreads * NB) {
+      T pr[NB], pim[NB];
+      T vv[NB];
 #pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
+      for (int u = 0; u < NB; ++u) {
         const int e = e0 + u * kIpThreads;
         if (e < totv) {
           const int l = e / n, p = e - l * n;
           const long row = (line0 + l) % A.vloc_rows;
           vv[u] = A.vloc[row * n + p];
-          pin[u] = A.psi_in[base + (long)e];
+          const C t = A.psi_in[base + (long)e];
+          pr[u] = t.x;
+          pim[u] = t.y;
         }
       }
 #pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
+      for (int u = 0; u < NB; ++u) {
         const int e = e0 + u * kIpThreads;
         if (e < totv) {
           const int l = e / n, p = e - l * n;
           const C t = buf[l * LS + p];
           C o;
-          o.x = A.a1 * (t.x + vv[u] * pin[u].x) + A.a0 * pin[u].x;
-          o.y = A.a1 * (t.y + vv[u] * pin[u].y) + A.a0 * pin[u].y;
+          o.x = A.a1 * (t.x + vv[u] * pr[u]) + A.a0 * pr[u];
+          o.y = A.a1 * (t.y + vv[u] * pim[u]) + A.a0 * pim[u];
           A.dst[base + (long)e] = o;
         }
       }

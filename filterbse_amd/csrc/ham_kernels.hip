@@ -435,6 +435,8 @@ int axpby_states(fbse_ctx *c, const double *x, double *y, long n, double a, doub
 constexpr int kMaxWin = 64;
 constexpr int kMaxMs = 16;
 
+constexpr int kAccGroup = 4;  // ring slots loaded per group in accumulate_kernel
+
 template <typename T>
 __global__ __launch_bounds__(256) void accumulate_kernel(const cplx_t<T> *ring, int ring_slots, long slot_stride,
                                                          long j_lo, int nj, const double *an, int ms, long ncheby,
@@ -447,27 +449,52 @@ __global__ __launch_bounds__(256) void accumulate_kernel(const cplx_t<T> *ring, 
   }
   __syncthreads();
   const long item = blockIdx.y;  // complex: start state; packed: start pair
+  const long slot0 = j_lo % ring_slots;
   for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < nsg; g += (long)gridDim.x * blockDim.x) {
+    const cplx_t<T> *src = ring + item * nsg + g;
     double2 acc[kMaxMs];
 #pragma unroll
     for (int t = 0; t < kMaxMs; ++t) acc[t] = make_double2(0.0, 0.0);
-    for (int j = 0; j < nj; ++j) {
-      const long slot = (j_lo + j) % ring_slots;
-      const cplx_t<T> x = ring[slot * slot_stride + item * nsg + g];
-      const double vx = (double)x.x, vy = (double)x.y;
+    // The window's slots follow each other modulo the ring.  Loads run one
+    // group of kAccGroup slots ahead of the arithmetic so that the HBM latency
+    // is covered by the previous group's FMAs.
+    cplx_t<T> cur[kAccGroup], nxt[kAccGroup];
+    long slot = slot0;
 #pragma unroll
-      for (int t = 0; t < kMaxMs; ++t) {
-        if (t < ms) {
-          const double2 a = s_a[t * kMaxWin + j];
-          if (!packed) {
-            acc[t].x += a.x * vx - a.y * vy;
-            acc[t].y += a.x * vy + a.y * vx;
-          } else {
-            acc[t].x += a.x * vx;
-            acc[t].y += a.x * vy;
+    for (int u = 0; u < kAccGroup; ++u) {
+      if (u < nj) cur[u] = src[slot * slot_stride];
+      if (++slot == ring_slots) slot = 0;
+    }
+    for (int j0 = 0; j0 < nj; j0 += kAccGroup) {
+      if (j0 + kAccGroup < nj) {
+#pragma unroll
+        for (int u = 0; u < kAccGroup; ++u) {
+          if (j0 + kAccGroup + u < nj) nxt[u] = src[slot * slot_stride];
+          if (++slot == ring_slots) slot = 0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kAccGroup; ++u) {
+        if (j0 + u < nj) {
+          const double vx = (double)cur[u].x, vy = (double)cur[u].y;
+          const double2 *sa = s_a + j0 + u;
+#pragma unroll
+          for (int t = 0; t < kMaxMs; ++t) {
+            if (t < ms) {
+              const double2 a = sa[t * kMaxWin];
+              if (!packed) {
+                acc[t].x += a.x * vx - a.y * vy;
+                acc[t].y += a.x * vy + a.y * vx;
+              } else {
+                acc[t].x += a.x * vx;
+                acc[t].y += a.x * vy;
+              }
+            }
           }
         }
       }
+#pragma unroll
+      for (int u = 0; u < kAccGroup; ++u) cur[u] = nxt[u];
     }
     if (!packed) {
       const long stlen = 2 * nsg;
