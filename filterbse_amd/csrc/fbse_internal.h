@@ -142,15 +142,15 @@ struct fbse_ctx {
 
   int window = 0;               // forced window depth (0 = auto)
 
-  // FFT autotuner: per precision (0 FP64, 1 FP32), batch size class (0 below
-  // 32M elements, 1 above), axis and pass class (0 plain, 1 z kinetic or
-  // preconditioner, 2 x combine) the chosen kernel (0 ping-pong, 1 in-place)
-  // and tile lines; lines 0 = not tuned yet.
-  int tune_kernel[2][2][3][3] = {};
+  // FFT autotuner of the per-axis passes: per precision (0 FP64, 1 FP32),
+  // batch size class (0 below 32M elements, 1 above), axis and pass class
+  // (0 plain, 1 z kinetic or preconditioner, 2 x combine) the tile lines;
+  // 0 = not tuned yet.
   int tune_lines[2][2][3][3] = {};
+  int tile_variant[2] = {-1, -1};  // plane/slab kernel instance per precision (-1 = not tuned)
   double tune_ms[2][2][3][3] = {};
   int autotune = 1;
-  int force_variant = -1;       // fbse_set_fft_kernel
+  int force_variant = -1;       // fbse_set_fft_kernel: -1 auto, 0 per-axis passes, 1 plane/slab passes
   int force_lines = 0;
 
   // profiling
@@ -192,6 +192,16 @@ void prof_collect(fbse_ctx *c);
 template <typename T>
 int fft_pass(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src, cplx_t<T> *dst, long nbatch,
              const cplx_t<T> *psi_in, T a1, T a0);
+// Plane/slab FFT passes (fft_tile_kernels.hip): available when every axis
+// factors into radices 2, 3, 4, 5, 8 and an xy plane and an xz slab fit the
+// 160 KiB LDS of a work-group.  out = a1*(T + V_loc) psi + a0*psi.
+template <typename T> bool tiles_supported(const fbse_ctx *c);
+template <typename T>
+int kinetic_local_tiles(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *work, long nstates, T a1,
+                        T a0);
+template <typename T>
+int precondition_tiles(fbse_ctx *c, const cplx_t<T> *r, cplx_t<T> *out, cplx_t<T> *work, long nstates, T pa, T pb);
+
 // Full H (or Newton step) for nstates complex states: out = a1*H psi + a0*psi.
 // work must hold nstates*nsg complex values.
 template <typename T>

@@ -1,0 +1,542 @@
+// fft_tile_kernels.hip -- the kinetic part of H in three HBM passes.
+//
+// A gfx950 work-group can hold 160 KiB of LDS, enough for a whole nx x ny
+// plane of a 96^3 grid in FP32 or FP64.  The 3D FFT pair of the kinetic
+// operator (hamiltonian.c:62-99) therefore runs as
+//   pass 1  xy planes : forward x FFT, forward y FFT          (plane in, plane out)
+//   pass 2  xz slabs  : forward z FFT, x k^2 (or the preconditioner), backward z FFT
+//   pass 3  xy planes : backward y FFT, backward x FFT, then
+//                       out = a1*(T psi + V psi_in) + a0*psi_in
+// instead of the five single-axis passes of fft_kernels.hip.  Each work-group
+// loads one tile into LDS with rows of nx contiguous elements, transforms it
+// in place with register-staged Stockham passes (every thread reads its
+// butterflies' inputs into registers, the group syncs, outputs are written to
+// their self-sorted positions), and writes it back.  Radices 2, 3, 4, 5 and 8.
+//
+// Tile geometry (element (r, c) of tile t of batch item b):
+//   xy plane : rows r = y (ny), cols c = x (nx), tile t = z,
+//              global index b*ngrid + z*nx*ny + r*nx + c
+//   xz slab  : rows r = z (nz), cols c = x (nx), tile t = y,
+//              global index b*ngrid + y*nx + r*nx*ny + c
+#include "fbse_internal.h"
+#include "fft_common.h"
+
+#include <algorithm>
+
+namespace fbse {
+
+enum TileKind { kTileFwdXY = 0, kTileBwdXY = 1, kTileBwdXYCombine = 2, kTileKinZ = 3, kTilePrecZ = 4 };
+
+// A thread stages ceil(E / R) butterflies of radix R per pass: a work-group
+// of NT threads covers tiles of up to NT * E elements.  E is a compile-time
+// kernel parameter; the autotuner picks among the instantiated values.
+constexpr int kTileLoadBatch = 8;  // global loads in flight per thread
+
+template <typename T> struct TileArgs {
+  const cplx_t<T> *src;
+  cplx_t<T> *dst;
+  const cplx_t<T> *twc, *twr;  // twiddles of the row (length nc) and column (length nr) transforms
+  int nc, nr, ls;              // tile columns, rows, LDS row stride (elements)
+  int nfc, nfr;
+  unsigned long long fc, fr;   // radix schedules, 4 bits per factor, first factor lowest
+  long row_stride;             // global elements between tile rows
+  long tile_stride;            // global elements between tiles of one batch item
+  long item_stride;            // global elements between batch items (ngrid)
+  int tiles_per_item;
+  // z passes: k2 = min(kx[x] + ky[y] + kz[z], kcap) when separable, else ksqr[x + nx*y + nx*ny*z]
+  const T *kx, *ky, *kz, *ksqr;
+  T kcap;
+  long nxny;
+  T pa, pb;                    // preconditioner g = pa / (1 + k2 * pb)
+  // combine: V of spin s starts at vloc + s*ngrid (items run spin-fastest)
+  const cplx_t<T> *psi_in;
+  const T *vloc;
+  int nspin;
+  T a1, a0;
+};
+
+// i mod p for 0 <= i < 2^22, p >= 1: float quotient, then one correction.
+__device__ __forceinline__ int mod_small(int i, int p, float inv_p) {
+  int q = (int)((float)i * inv_p);
+  int r = i - q * p;
+  if (r >= p) r -= p;
+  if (r < 0) r += p;
+  return r;
+}
+
+// One in-place Stockham pass of radix R over `nlines` lines of length n.
+// Element j of line l sits at l*lstride + j*estride.  The work-group size is
+// a multiple of nlines, so a thread keeps one line l0 = tid % nlines for the
+// whole transform and takes butterflies i0, i0 + di, ... of it; consecutive
+// threads are on consecutive lines: for column transforms (lstride 1) a
+// wavefront touches consecutive LDS words, for row transforms (lstride = odd
+// row pitch) words 2*pitch apart, which spread over all banks.
+template <typename T, int R, int KB>
+__device__ __forceinline__ void tile_radix_pass(cplx_t<T> *buf, const cplx_t<T> *tw, int n, int p, int estride,
+                                                int lstride, int l0, int i0, int di, bool inv) {
+  using C = cplx_t<T>;
+  const int m = n / R;
+  const int tstep = n / (p * R);
+  const float inv_p = 1.0f / (float)p;
+  const T sgn = inv ? (T)1 : (T)-1;
+  const int lbase = l0 * lstride;
+  C x[KB][R];
+  int dsti[KB];
+#pragma unroll
+  for (int k = 0; k < KB; ++k) {
+    const int i = i0 + k * di;
+    dsti[k] = -1;
+    if (i < m) {
+      const int kk = mod_small(i, p, inv_p);
+      const C *src = buf + lbase + i * estride;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        C v = src[r * m * estride];
+        if (r > 0 && kk > 0) {
+          C w = tw[r * kk * tstep];
+          if (inv) w = cconj(w);
+          v = cmul(v, w);
+        }
+        x[k][r] = v;
+      }
+      dsti[k] = lbase + ((i - kk) * R + kk) * estride;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < KB; ++k) {
+    if (dsti[k] >= 0) {
+      Bfly<T, R>::run(x[k], sgn);
+#pragma unroll
+      for (int q = 0; q < R; ++q) buf[dsti[k] + q * p * estride] = x[k][q];
+    }
+  }
+  __syncthreads();
+}
+
+// 1D transforms of length n over nlines lines (nlines divides blockDim.x).
+template <typename T, int E>
+__device__ __forceinline__ void tile_fft(cplx_t<T> *buf, const cplx_t<T> *tw, int n, int nf, unsigned long long f,
+                                         int estride, int lstride, int nlines, bool inv) {
+  const int l0 = threadIdx.x % nlines, i0 = threadIdx.x / nlines, di = blockDim.x / nlines;
+  int p = 1;
+  for (int s = 0; s < nf; ++s) {
+    const int R = (int)((f >> (4 * s)) & 15u);
+    switch (R) {
+      case 2: tile_radix_pass<T, 2, (E + 1) / 2>(buf, tw, n, p, estride, lstride, l0, i0, di, inv); break;
+      case 3: tile_radix_pass<T, 3, (E + 2) / 3>(buf, tw, n, p, estride, lstride, l0, i0, di, inv); break;
+      case 4: tile_radix_pass<T, 4, (E + 3) / 4>(buf, tw, n, p, estride, lstride, l0, i0, di, inv); break;
+      case 5: tile_radix_pass<T, 5, (E + 4) / 5>(buf, tw, n, p, estride, lstride, l0, i0, di, inv); break;
+      default: tile_radix_pass<T, 8, (E + 7) / 8>(buf, tw, n, p, estride, lstride, l0, i0, di, inv); break;
+    }
+    p *= R;
+  }
+}
+
+template <typename T, int KIND, int E>
+__device__ __forceinline__ void tile_body(const TileArgs<T> &A) {
+  using C = cplx_t<T>;
+  constexpr int NB = kTileLoadBatch;
+  extern __shared__ unsigned char smem_raw[];
+  const int nc = A.nc, nr = A.nr, ls = A.ls;
+  const int NT = blockDim.x;
+  C *buf = reinterpret_cast<C *>(smem_raw);
+  C *twc = buf + nr * ls;
+  C *twr = twc + nc;
+  for (int t = threadIdx.x; t < nc; t += NT) twc[t] = A.twc[t];
+  for (int t = threadIdx.x; t < nr; t += NT) twr[t] = A.twr[t];
+
+  const long item = blockIdx.x / A.tiles_per_item;
+  const int w = (int)(blockIdx.x - item * A.tiles_per_item);
+  const long base = item * A.item_stride + (long)w * A.tile_stride;
+  // blockDim.x is a multiple of nc: thread -> column c0, rows r0, r0 + dr, ...
+  const int c0 = threadIdx.x % nc, r0 = threadIdx.x / nc, dr = NT / nc;
+  const cplx_t<T> *src = A.src + base + c0;
+
+  for (int rb = r0; rb < nr; rb += dr * NB) {
+    T vr[NB], vi[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int r = rb + u * dr;
+      if (r < nr) {
+        const C t = src[r * A.row_stride];
+        vr[u] = t.x;
+        vi[u] = t.y;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int r = rb + u * dr;
+      if (r < nr) buf[r * ls + c0] = C{vr[u], vi[u]};
+    }
+  }
+  __syncthreads();
+
+  if (KIND == kTileFwdXY) {
+    tile_fft<T, E>(buf, twc, nc, A.nfc, A.fc, 1, ls, nr, false);  // x rows
+    tile_fft<T, E>(buf, twr, nr, A.nfr, A.fr, ls, 1, nc, false);  // y columns
+  } else if (KIND == kTileBwdXY || KIND == kTileBwdXYCombine) {
+    tile_fft<T, E>(buf, twr, nr, A.nfr, A.fr, ls, 1, nc, true);
+    tile_fft<T, E>(buf, twc, nc, A.nfc, A.fc, 1, ls, nr, true);
+  } else {  // z columns: forward, scale by k^2 (or the preconditioner), backward
+    tile_fft<T, E>(buf, twr, nr, A.nfr, A.fr, ls, 1, nc, false);
+    T kxy = 0;
+    const T *ksq = nullptr;
+    if (A.ksqr)
+      ksq = A.ksqr + c0 + (long)w * nc;
+    else
+      kxy = A.kx[c0] + A.ky[w];
+    for (int r = r0; r < nr; r += dr) {
+      T k2;
+      if (ksq) {
+        k2 = ksq[(long)r * A.nxny];
+      } else {
+        k2 = kxy + A.kz[r];
+        if (k2 > A.kcap) k2 = A.kcap;
+      }
+      if (KIND == kTilePrecZ) k2 = A.pa / ((T)1 + k2 * A.pb);
+      C v = buf[r * ls + c0];
+      v.x *= k2;
+      v.y *= k2;
+      buf[r * ls + c0] = v;
+    }
+    __syncthreads();
+    tile_fft<T, E>(buf, twr, nr, A.nfr, A.fr, ls, 1, nc, true);
+  }
+
+  cplx_t<T> *dst = A.dst + base + c0;
+  if (KIND == kTileBwdXYCombine) {
+    // V is shared by the states: batch item b holds spin b % nspin
+    const long vofs = (item % A.nspin) * A.item_stride + (long)w * A.tile_stride + c0;
+    const T *vl = A.vloc + vofs;
+    const cplx_t<T> *pin = A.psi_in + base + c0;
+    for (int rb = r0; rb < nr; rb += dr * NB) {
+      T pr[NB], pim[NB], vv[NB];
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        const int r = rb + u * dr;
+        if (r < nr) {
+          vv[u] = vl[r * A.row_stride];
+          const C t = pin[r * A.row_stride];
+          pr[u] = t.x;
+          pim[u] = t.y;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        const int r = rb + u * dr;
+        if (r < nr) {
+          const C t = buf[r * ls + c0];
+          C o;
+          o.x = A.a1 * (t.x + vv[u] * pr[u]) + A.a0 * pr[u];
+          o.y = A.a1 * (t.y + vv[u] * pim[u]) + A.a0 * pim[u];
+          dst[r * A.row_stride] = o;
+        }
+      }
+    }
+  } else {
+    for (int r = r0; r < nr; r += dr) dst[r * A.row_stride] = buf[r * ls + c0];
+  }
+}
+
+// Kernel instances: (precision, E, block-size limit, waves per SIMD).  Two
+// FP32 work-groups of a 96 x 96 tile (2 x 76 KiB of LDS) share a CU; the
+// waves-per-SIMD hint keeps the VGPRs low enough for both to be resident.
+// An FP64 96 x 96 tile takes 152 KiB, one work-group per CU.
+#define FBSE_TILE_KERNEL(NAME, T, E, MAXT, WAVES)                                                      \
+  template <int KIND>                                                                                \
+  __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WAVES))) void NAME(TileArgs<T> A) { \
+    tile_body<T, KIND, E>(A);                                                                        \
+  }
+FBSE_TILE_KERNEL(fft_tile_f32_e12, float, 12, 1024, 6)
+FBSE_TILE_KERNEL(fft_tile_f32_e16, float, 16, 1024, 5)
+FBSE_TILE_KERNEL(fft_tile_f32_e32, float, 32, 512, 2)
+FBSE_TILE_KERNEL(fft_tile_f64_e12, double, 12, 1024, 1)
+FBSE_TILE_KERNEL(fft_tile_f64_e16, double, 16, 768, 1)
+#undef FBSE_TILE_KERNEL
+
+struct TileVariant {
+  int e;         // elements per thread
+  int max_nt;    // block-size limit of the instance
+};
+static const TileVariant kTileF32[] = {{12, 1024}, {16, 1024}, {32, 512}};
+static const TileVariant kTileF64[] = {{12, 1024}, {16, 768}};
+template <typename T> static const TileVariant *tile_variants(int *n) {
+  if (sizeof(T) == 4) {
+    *n = 3;
+    return kTileF32;
+  }
+  *n = 2;
+  return kTileF64;
+}
+
+template <typename T, int KIND> static const void *tile_fn(int v) {
+  if constexpr (sizeof(T) == 4) {
+    return v == 0 ? (const void *)fft_tile_f32_e12<KIND>
+                  : (v == 1 ? (const void *)fft_tile_f32_e16<KIND> : (const void *)fft_tile_f32_e32<KIND>);
+  } else {
+    return v == 0 ? (const void *)fft_tile_f64_e12<KIND> : (const void *)fft_tile_f64_e16<KIND>;
+  }
+}
+template <typename T, int KIND>
+static void tile_launch(int v, dim3 g, dim3 b, size_t lds, hipStream_t st, const TileArgs<T> &A) {
+  if constexpr (sizeof(T) == 4) {
+    if (v == 0)
+      hipLaunchKernelGGL((fft_tile_f32_e12<KIND>), g, b, lds, st, A);
+    else if (v == 1)
+      hipLaunchKernelGGL((fft_tile_f32_e16<KIND>), g, b, lds, st, A);
+    else
+      hipLaunchKernelGGL((fft_tile_f32_e32<KIND>), g, b, lds, st, A);
+  } else {
+    if (v == 0)
+      hipLaunchKernelGGL((fft_tile_f64_e12<KIND>), g, b, lds, st, A);
+    else
+      hipLaunchKernelGGL((fft_tile_f64_e16<KIND>), g, b, lds, st, A);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static bool radices_ok(const AxisPlan &pl) {
+  for (int i = 0; i < pl.nf; ++i) {
+    const int r = pl.f[i];
+    if (r != 2 && r != 3 && r != 4 && r != 5 && r != 8) return false;
+  }
+  return pl.nf > 0 && pl.nf <= 16;
+}
+
+// LDS row pitch: odd, so that row transforms (threads on consecutive rows)
+// hit distinct banks.
+static int tile_pitch(int nc) { return nc | 1; }
+
+template <typename T> static size_t tile_lds_bytes(int nr, int nc) {
+  return ((size_t)nr * tile_pitch(nc) + nc + nr) * sizeof(cplx_t<T>);
+}
+
+static int gcd_int(int a, int b) { return b ? gcd_int(b, a % b) : a; }
+
+// Work-group size: a multiple of both line counts (nr for row transforms, nc
+// for column transforms and the load/store mapping) with at least
+// ceil(nr*nc / e) threads; 0 when none fits within max_nt.
+static int tile_threads(int nr, int nc, int e, int max_nt) {
+  const long lcm = (long)nr / gcd_int(nr, nc) * nc;
+  const long need = ((long)nr * nc + e - 1) / e;
+  long nt = (need + lcm - 1) / lcm * lcm;
+  if (nt < 64) nt = (64 + lcm - 1) / lcm * lcm;
+  return nt <= max_nt ? (int)nt : 0;
+}
+
+// first instance (in list order) that fits both tile shapes, -1 if none
+template <typename T> static int first_tile_variant(const fbse_ctx *c) {
+  int nv;
+  const TileVariant *tv = tile_variants<T>(&nv);
+  for (int v = 0; v < nv; ++v)
+    if (tile_threads((int)c->ny, (int)c->nx, tv[v].e, tv[v].max_nt) &&
+        tile_threads((int)c->nz, (int)c->nx, tv[v].e, tv[v].max_nt))
+      return v;
+  return -1;
+}
+
+template <typename T> bool tiles_supported(const fbse_ctx *c) {
+  if (!radices_ok(c->plan[0]) || !radices_ok(c->plan[1]) || !radices_ok(c->plan[2])) return false;
+  const int nx = (int)c->nx, ny = (int)c->ny, nz = (int)c->nz;
+  const size_t cap = 160 * 1024;
+  if (tile_lds_bytes<T>(ny, nx) > cap || tile_lds_bytes<T>(nz, nx) > cap) return false;
+  if (first_tile_variant<T>(c) < 0) return false;
+  return true;
+}
+template bool tiles_supported<double>(const fbse_ctx *);
+template bool tiles_supported<float>(const fbse_ctx *);
+
+template <typename T> static const cplx_t<T> *twiddles(const fbse_ctx *c, int axis);
+template <> const double2 *twiddles<double>(const fbse_ctx *c, int axis) { return c->d_tw[axis]; }
+template <> const float2 *twiddles<float>(const fbse_ctx *c, int axis) { return c->d_tw_f[axis]; }
+
+// Dynamic LDS above 64 KiB has to be enabled per kernel and device.
+template <typename T, int KIND> static void set_lds_limit(int device, int v) {
+  static bool done[64][3] = {};
+  if (device < 0 || device >= 64 || !done[device][v]) {
+    (void)hipFuncSetAttribute(tile_fn<T, KIND>(v), hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (device >= 0 && device < 64) done[device][v] = true;
+  }
+}
+
+template <typename T>
+static int tile_pass_v(fbse_ctx *c, int v, int kind, const cplx_t<T> *src, cplx_t<T> *dst, long nbatch,
+                       const cplx_t<T> *psi_in, T a1, T a0) {
+  TileArgs<T> A{};
+  A.src = src;
+  A.dst = dst;
+  A.nc = (int)c->nx;
+  A.ls = tile_pitch(A.nc);
+  A.item_stride = c->ngrid;
+  const bool xy = kind == kTileFwdXY || kind == kTileBwdXY || kind == kTileBwdXYCombine;
+  const AxisPlan &pc = c->plan[0], &pr = c->plan[xy ? 1 : 2];
+  A.nr = pr.n;
+  A.nfc = pc.nf;
+  A.nfr = pr.nf;
+  A.fc = 0;
+  A.fr = 0;
+  for (int i = 0; i < pc.nf; ++i) A.fc |= (unsigned long long)pc.f[i] << (4 * i);
+  for (int i = 0; i < pr.nf; ++i) A.fr |= (unsigned long long)pr.f[i] << (4 * i);
+  A.twc = twiddles<T>(c, 0);
+  A.twr = twiddles<T>(c, xy ? 1 : 2);
+  if (xy) {
+    A.row_stride = c->nx;
+    A.tile_stride = c->nx * c->ny;
+    A.tiles_per_item = (int)c->nz;
+  } else {
+    A.row_stride = c->nx * c->ny;
+    A.tile_stride = c->nx;
+    A.tiles_per_item = (int)c->ny;
+  }
+  A.nxny = c->nx * c->ny;
+  const bool f32 = sizeof(T) == 4;
+  if (c->ksqr_sep) {
+    A.kx = f32 ? (const T *)c->d_kx_f : (const T *)c->d_kx;
+    A.ky = f32 ? (const T *)c->d_ky_f : (const T *)c->d_ky;
+    A.kz = f32 ? (const T *)c->d_kz_f : (const T *)c->d_kz;
+    A.ksqr = nullptr;
+  } else {
+    A.ksqr = f32 ? (const T *)c->d_ksqr_f : (const T *)c->d_ksqr;
+  }
+  A.kcap = (T)c->kcap;
+  A.pa = a1;  // the preconditioner pass takes (pa, pb) in the (a1, a0) slots
+  A.pb = a0;
+  A.psi_in = psi_in;
+  A.vloc = f32 ? (const T *)c->d_vloc_f : (const T *)c->d_vloc;
+  A.nspin = (int)c->nspin;
+  A.a1 = a1;
+  A.a0 = a0;
+  const long ntiles = nbatch * A.tiles_per_item;
+  if (ntiles > 0x7fffffffL) return set_err(c, FBSE_ERR_UNSUPPORTED, "%ld FFT tiles in one launch", ntiles);
+  int nv;
+  const TileVariant *tv = tile_variants<T>(&nv);
+  const int nt = tile_threads(A.nr, A.nc, tv[v].e, tv[v].max_nt);
+  if (nt == 0) return set_err(c, FBSE_ERR_UNSUPPORTED, "tile %d x %d does not fit FFT instance %d", A.nr, A.nc, v);
+  const size_t lds = tile_lds_bytes<T>(A.nr, A.nc);
+  hipEvent_t ev = nullptr;
+  const int pk = kind == kTileBwdXYCombine ? 1 : 0;
+  prof_begin(c, pk, &ev);
+  const dim3 g((unsigned)ntiles), b((unsigned)nt);
+  switch (kind) {
+    case kTileFwdXY:
+      set_lds_limit<T, kTileFwdXY>(c->device, v);
+      tile_launch<T, kTileFwdXY>(v, g, b, lds, c->stream, A);
+      break;
+    case kTileBwdXY:
+      set_lds_limit<T, kTileBwdXY>(c->device, v);
+      tile_launch<T, kTileBwdXY>(v, g, b, lds, c->stream, A);
+      break;
+    case kTileBwdXYCombine:
+      set_lds_limit<T, kTileBwdXYCombine>(c->device, v);
+      tile_launch<T, kTileBwdXYCombine>(v, g, b, lds, c->stream, A);
+      break;
+    case kTileKinZ:
+      set_lds_limit<T, kTileKinZ>(c->device, v);
+      tile_launch<T, kTileKinZ>(v, g, b, lds, c->stream, A);
+      break;
+    default:
+      set_lds_limit<T, kTilePrecZ>(c->device, v);
+      tile_launch<T, kTilePrecZ>(v, g, b, lds, c->stream, A);
+      break;
+  }
+  FBSE_HIP(c, hipGetLastError());
+  prof_end(c, pk, ev);
+  return FBSE_OK;
+}
+
+// Instance choice: the autotuner times every instance that fits on the
+// first batch of at least 1M grid points (forward xy pass into a scratch
+// buffer, best of 3) and keeps the fastest per precision; smaller batches
+// before that use the first instance that fits.
+template <typename T>
+static int tile_pass(fbse_ctx *c, int kind, const cplx_t<T> *src, cplx_t<T> *dst, long nbatch,
+                     const cplx_t<T> *psi_in, T a1, T a0) {
+  const int pi = sizeof(T) == 8 ? 0 : 1;
+  if (c->force_variant == 1 && c->force_lines > 0) {  // forced instance (1-based)
+    int nv;
+    tile_variants<T>(&nv);
+    if (c->force_lines > nv) return set_err(c, FBSE_ERR_ARG, "FFT tile instance %d of %d", c->force_lines, nv);
+    return tile_pass_v<T>(c, c->force_lines - 1, kind, src, dst, nbatch, psi_in, a1, a0);
+  }
+  if (c->tile_variant[pi] < 0) {
+    int nv;
+    const TileVariant *tv = tile_variants<T>(&nv);
+    int v0 = first_tile_variant<T>(c);
+    if (!(c->autotune && nbatch * c->ngrid >= (1L << 20)))
+      return tile_pass_v<T>(c, v0, kind, src, dst, nbatch, psi_in, a1, a0);
+    cplx_t<T> *scratch = nullptr;
+    const size_t bytes = (size_t)nbatch * c->ngrid * sizeof(cplx_t<T>);
+    int best_v = v0;
+    if (hipMalloc(&scratch, bytes) == hipSuccess) {
+      hipEvent_t e0, e1;
+      (void)hipEventCreate(&e0);
+      (void)hipEventCreate(&e1);
+      const int saved_profile = c->profile;
+      c->profile = 0;
+      double best = 1e30;
+      for (int v = 0; v < nv; ++v) {
+        if (!tile_threads((int)c->ny, (int)c->nx, tv[v].e, tv[v].max_nt) ||
+            !tile_threads((int)c->nz, (int)c->nx, tv[v].e, tv[v].max_nt))
+          continue;
+        double tmin = 1e30;
+        for (int rep = 0; rep < 3; ++rep) {
+          (void)hipEventRecord(e0, c->stream);
+          int rc = tile_pass_v<T>(c, v, kTileFwdXY, src, scratch, nbatch, nullptr, 0, 0);
+          (void)hipEventRecord(e1, c->stream);
+          if (rc) {
+            c->err.clear();
+            tmin = 1e30;
+            break;
+          }
+          (void)hipEventSynchronize(e1);
+          float ms = 0.f;
+          (void)hipEventElapsedTime(&ms, e0, e1);
+          tmin = std::min(tmin, (double)ms);
+        }
+        if (tmin < best) {
+          best = tmin;
+          best_v = v;
+        }
+      }
+      c->profile = saved_profile;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      (void)hipStreamSynchronize(c->stream);
+      (void)hipFree(scratch);
+    } else {
+      (void)hipGetLastError();
+    }
+    c->tile_variant[pi] = best_v;
+  }
+  return tile_pass_v<T>(c, c->tile_variant[pi], kind, src, dst, nbatch, psi_in, a1, a0);
+}
+
+// out = a1*(T + V) psi + a0*psi over nstates*nspin grids (projectors are
+// added by the caller); work holds one batch of grids.
+template <typename T>
+int kinetic_local_tiles(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *work, long nstates, T a1,
+                        T a0) {
+  const long nb = nstates * c->nspin;
+  int rc;
+  if ((rc = tile_pass<T>(c, kTileFwdXY, psi, work, nb, nullptr, 0, 0))) return rc;
+  if ((rc = tile_pass<T>(c, kTileKinZ, work, work, nb, nullptr, 0, 0))) return rc;
+  return tile_pass<T>(c, kTileBwdXYCombine, work, out, nb, psi, a1, a0);
+}
+template int kinetic_local_tiles<double>(fbse_ctx *, const double2 *, double2 *, double2 *, long, double, double);
+template int kinetic_local_tiles<float>(fbse_ctx *, const float2 *, float2 *, float2 *, long, float, float);
+
+template <typename T>
+int precondition_tiles(fbse_ctx *c, const cplx_t<T> *r, cplx_t<T> *out, cplx_t<T> *work, long nstates, T pa,
+                       T pb) {
+  const long nb = nstates * c->nspin;
+  int rc;
+  if ((rc = tile_pass<T>(c, kTileFwdXY, r, work, nb, nullptr, 0, 0))) return rc;
+  if ((rc = tile_pass<T>(c, kTilePrecZ, work, work, nb, nullptr, pa, pb))) return rc;
+  return tile_pass<T>(c, kTileBwdXY, work, out, nb, nullptr, 0, 0);
+}
+template int precondition_tiles<double>(fbse_ctx *, const double2 *, double2 *, double2 *, long, double, double);
+
+}  // namespace fbse

@@ -168,9 +168,14 @@ int fbse_profile_read(fbse_ctx *ctx, int kind, long *launches, double *total_ms)
  * chosen by the built-in autotuner at the first call; this forces K (0 = auto). */
 int fbse_set_window(fbse_ctx *ctx, int window);
 
-/* FFT kernel selection: variant -1 = autotune at first use (default), 0 = the
- * two-buffer LDS kernel, 1 = the in-place register-staged kernel (lengths with
- * radices 2,3,4,5,8 only; others fall back to 0); lines > 0 fixes the tile width. */
+/* FFT pass selection for the kinetic operator: variant -1 = automatic
+ * (default: plane/slab passes when available, else per-axis passes),
+ * 0 = five per-axis passes (any length), 1 = three plane/slab passes (every
+ * axis length a product of 2, 3, 4, 5, 8 and an nx x max(ny, nz) tile within
+ * 160 KiB of LDS; otherwise the per-axis passes run).  lines > 0 fixes,
+ * instead of autotuning, the tile height of the per-axis passes (variant 0)
+ * or the plane/slab kernel instance, 1-based (variant 1: FP32 instances
+ * 1..3 stage 12, 16, 32 elements per thread, FP64 instances 1..2 stage 12, 16). */
 int fbse_set_fft_kernel(fbse_ctx *ctx, int variant, int lines);
 
 /* Device memory helpers for callers without a GPU runtime of their own. */
