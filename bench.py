@@ -66,7 +66,9 @@ def setup(cfg_name, dev):
                 sig_f=sig_f)
 
 
-def one_pass(S, precision, stage_times=None, refine=False):
+def one_pass(S, precision, stage_times=None, refine=False, profile=False):
+    """One pass of the path.  profile: per-kernel HIP events (on the library's
+    stream) of this pass's filter are left in S["prof"]."""
     import torch
     ctx = S["ctx"]
     n = S["n_starts"] * S["ms"]
@@ -74,7 +76,12 @@ def one_pass(S, precision, stage_times=None, refine=False):
     torch.cuda.synchronize()
     ptr = S["psi_rank"].data_ptr()
     t = [time.perf_counter()]
+    if profile:
+        ctx.profile(True)
     ctx.filter_dev(ptr, S["n_starts"], S["ms"], S["an"], S["zn"], S["Vmin"], 4.0 / S["dE"], precision)
+    if profile:
+        S["prof"] = {k: ctx.profile_read(k) for k in range(5)}
+        ctx.profile(False)
     t.append(time.perf_counter())
     ctx.normalize_all_dev(ptr, n)
     rank, sv = ctx.ortho_dev(ptr, n)
@@ -129,20 +136,6 @@ def cpu_baseline(S, budget_s=15.0):
                        % (n, S["cfg_name"], el))
 
 
-def profile_filter(S, precision):
-    """Per-kernel device times (HIP events on the library's stream) of one filter."""
-    import torch
-    ctx = S["ctx"]
-    S["psi_rank"].view(S["n_starts"], S["ms"], S["stlen"])[:, 0, :].copy_(S["starts_dev"])
-    torch.cuda.synchronize()
-    ctx.profile(True)
-    ctx.filter_dev(S["psi_rank"].data_ptr(), S["n_starts"], S["ms"], S["an"], S["zn"], S["Vmin"], 4.0 / S["dE"],
-                   precision)
-    res = {k: ctx.profile_read(k) for k in range(5)}
-    ctx.profile(False)
-    return res
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,12 +160,16 @@ def main():
     for _ in range(args.warmup):
         one_pass(S, mixed, refine=True)
     stages = []
+    ctx = S["ctx"]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ev, sig, rank = one_pass(S, mixed, stages, refine=True)
+    for i in range(args.steps):
+        # the last timed pass also records per-kernel HIP events of its filter
+        ev, sig, rank = one_pass(S, mixed, stages, refine=True, profile=(i == args.steps - 1))
     torch.cuda.synchronize()
     t_mixed = (time.perf_counter() - t0) / args.steps
+    prof = S["prof"]
+    fft_path, fft_inst = ctx.fft_choice(mixed)
     ev_c, sig_c = converged(ev, sig)
     st = np.mean(stages, axis=0)
     log("mixed: %.3fs/pass (filter %.3f ortho %.3f RR %.3f refine+sigma %.3f), n %d, converged %d"
@@ -194,20 +191,24 @@ def main():
         log("fp64: %.3fs/pass, rank %d, converged %d, max |e_mixed - e_fp64| %s"
             % (t64, rank64, len(ev64_c), fp["max_eig_err_mixed_vs_fp64"]))
 
-    prof = profile_filter(S, mixed)
     job = S["job"]
     nsg = job["nspin"] * job["nx"] * job["ny"] * job["nz"]
     n_items = S["n_starts"] if job["is_complex"] else (S["n_starts"] + 1) // 2
     elem = 8  # complex float
     fft_launches, fft_ms = prof[0]
     avg_ms = fft_ms / max(fft_launches, 1)
-    bytes_per_launch = 2.0 * n_items * nsg * elem  # read + write of the batch once
+    # algorithmic bytes of one FFT launch: the batch of n_items spinor grids
+    # (nsg complex FP32 values each) read once and written once
+    bytes_per_launch = 2.0 * n_items * nsg * elem
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    if fft_path == 1:
+        kname = "fft_tile_f32 instance %d (xy-plane forward and xz-slab kinetic passes)" % fft_inst
+    else:
+        kname = "fft_axis_kernel<float> (x/y forward, z kinetic, y backward passes)"
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="fft_axis_kernel<float> (x/y forward, z kinetic, y backward passes)",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel=kname,
                     avg_launch_ms=round(avg_ms, 4), bytes_per_launch=bytes_per_launch,
-                    launches=fft_launches)
+                    launches=fft_launches, measured="HIP events on the library stream, filter of the last timed pass")
     kernels = {name: dict(launches=prof[k][0], total_ms=round(prof[k][1], 3))
                for k, name in enumerate(["fft_passes", "x_backward_newton_combine", "projectors",
                                          "window_accumulate", "all"])}

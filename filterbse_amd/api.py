@@ -77,6 +77,7 @@ GPU_SIGNATURES = {
     "fbse_profile_read": (C.c_int, [C.c_void_p, C.c_int, LP, DP]),
     "fbse_set_window": (C.c_int, [C.c_void_p, C.c_int]),
     "fbse_set_fft_kernel": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "fbse_fft_choice": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "fbse_dev_alloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     "fbse_dev_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fbse_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
@@ -371,6 +372,12 @@ class Context:
 
     def set_fft_kernel(self, variant, lines=0):
         self._check(gpu().fbse_set_fft_kernel(self.ctx, variant, lines))
+
+    def fft_choice(self, precision=FBSE_PREC_FP64):
+        """(path, instance): path 1 = plane/slab passes, 0 = per-axis passes."""
+        p, i = C.c_int(), C.c_int()
+        self._check(gpu().fbse_fft_choice(self.ctx, precision, C.byref(p), C.byref(i)))
+        return p.value, i.value
 
     def set_window(self, k):
         self._check(gpu().fbse_set_window(self.ctx, k))

@@ -424,6 +424,11 @@ extern "C" int fbse_set_fft_kernel(fbse_ctx *c, int variant, int lines) {
   c->force_lines = lines;
   return FBSE_OK;
 }
+extern "C" int fbse_fft_choice(fbse_ctx *c, int precision, int *path, int *instance) {
+  if (!c || !path || (precision != FBSE_PREC_FP64 && precision != FBSE_PREC_MIXED)) return FBSE_ERR_ARG;
+  *path = fft_path(c, precision == FBSE_PREC_MIXED ? 1 : 0, instance);
+  return FBSE_OK;
+}
 extern "C" int fbse_dev_alloc(fbse_ctx *c, size_t bytes, void **p) {
   if (!c || !p) return FBSE_ERR_ARG;
   hipError_t e = hipMalloc(p, bytes ? bytes : 16);
@@ -692,6 +697,8 @@ static int filter_impl(fbse_ctx *c, double *d_psi_rank, long n_starts, long ms, 
   if ((rc = ensure_scratch(c, slot_bytes * (size_t)(R + 1)))) return rc;
   C *ring = (C *)c->d_scratch;
   C *work = ring + (size_t)R * slot_elems;
+  int real_an = 1;  // all an.im == 0: the accumulation needs only the real parts
+  for (long k = 0; k < ms * M && real_an; ++k) real_an = an[2 * k + 1] == 0.0;
   double *d_an = nullptr;
   FBSE_HIP(c, hipMalloc(&d_an, (size_t)ms * M * 2 * sizeof(double)));
   FBSE_HIP(c, hipMemcpyAsync(d_an, an, (size_t)ms * M * 2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -725,13 +732,14 @@ static int filter_impl(fbse_ctx *c, double *d_psi_rank, long n_starts, long ms, 
     if ((rc = apply_h_cplx<T>(c, in, out, work, n_items, (T)dE_1, (T)a0))) break;
     if (j + 1 - acc_lo == R || j == M - 1) {
       if ((rc = accumulate_window<T>(c, ring, R, slot_elems, acc_lo, j + 1, d_an, ms, M, d_psi_rank, n_starts,
-                                     n_items, packed)))
+                                     n_items, packed, real_an)))
         break;
       acc_lo = j + 1;
     }
   }
   if (!rc && M == 1)
-    rc = accumulate_window<T>(c, ring, R, slot_elems, 0, 1, d_an, ms, M, d_psi_rank, n_starts, n_items, packed);
+    rc = accumulate_window<T>(c, ring, R, slot_elems, 0, 1, d_an, ms, M, d_psi_rank, n_starts, n_items, packed,
+                              real_an);
   hipStreamSynchronize(c->stream);
   hipFree(d_an);
   if (!rc) {
@@ -1166,31 +1174,52 @@ extern "C" int fbse_refine_dev(fbse_ctx *c, double *d_psi, long n_ritz, const do
   std::vector<long> sel;
   for (long i = 0; i < n_ritz; ++i)
     if (eval[i] >= e_lo && eval[i] <= e_hi && (!sigma || sigma[i] < sigma_max)) sel.push_back(i);
-  long k = std::min((long)sel.size(), capacity / 2);
   *n_out = 0;
-  if (k == 0) return FBSE_OK;
-  // compact the selected Ritz vectors to the front (indices increase, so no overlap)
+  if (sel.empty()) return FBSE_OK;
+  // Full basis when [all n_ritz Ritz vectors, P r of every selected pair] fits
+  // in capacity: the corrections then stay orthogonal to the whole filtered
+  // subspace, which matters when the window holds many close eigenvalues.
+  // Otherwise the selected pairs are compacted to the front and the basis is
+  // [u_sel, P r_sel] (k <= capacity / 2).
+  const bool full = n_ritz + (long)sel.size() <= capacity;
+  const long k = full ? (long)sel.size() : std::min((long)sel.size(), capacity / 2);
+  const long nbase = full ? n_ritz : k;  // corrections go to columns nbase .. nbase+k-1
   std::vector<double> theta(k);
-  for (long j = 0; j < k; ++j) {
-    if (sel[j] != j)
-      FBSE_HIP(c, hipMemcpyAsync(d_psi + j * sl, d_psi + sel[j] * sl, sl * 8, hipMemcpyDeviceToDevice, c->stream));
-  }
   for (long j = 0; j < k; ++j) theta[j] = eval[sel[j]];
-  double *d_hu = nullptr, *d_th = nullptr;
+  double *d_u = nullptr, *d_hu = nullptr, *d_th = nullptr;
   FBSE_HIP(c, hipMalloc(&d_hu, (size_t)k * sl * 8));
   FBSE_HIP(c, hipMalloc(&d_th, (size_t)k * 8));
   int rc = FBSE_OK;
-  hipError_t e = hipMemcpyAsync(d_th, theta.data(), k * 8, hipMemcpyHostToDevice, c->stream);
-  if (e != hipSuccess) rc = set_err(c, FBSE_ERR_HIP, "copy: %s", hipGetErrorString(e));
-  if (!rc) rc = apply_h_dev_impl(c, d_psi, d_hu, k);
-  if (!rc) rc = states_sub_scaled(c, d_psi, d_hu, k, sl, d_th);
-  if (!rc) rc = precondition_dev(c, d_hu, d_psi + k * sl, k, e_p);
+  if (full) {
+    // gather the selected vectors into a work block (they stay in place too)
+    if (hipMalloc(&d_u, (size_t)k * sl * 8) != hipSuccess) rc = set_err(c, FBSE_ERR_NOMEM, "refine work block");
+    for (long j = 0; j < k && !rc; ++j) {
+      hipError_t e = hipMemcpyAsync(d_u + j * sl, d_psi + sel[j] * sl, sl * 8, hipMemcpyDeviceToDevice, c->stream);
+      if (e != hipSuccess) rc = set_err(c, FBSE_ERR_HIP, "copy: %s", hipGetErrorString(e));
+    }
+  } else {
+    // compact the selected Ritz vectors to the front (indices increase, so no overlap)
+    d_u = d_psi;
+    for (long j = 0; j < k && !rc; ++j) {
+      if (sel[j] == j) continue;
+      hipError_t e = hipMemcpyAsync(d_psi + j * sl, d_psi + sel[j] * sl, sl * 8, hipMemcpyDeviceToDevice, c->stream);
+      if (e != hipSuccess) rc = set_err(c, FBSE_ERR_HIP, "copy: %s", hipGetErrorString(e));
+    }
+  }
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(d_th, theta.data(), k * 8, hipMemcpyHostToDevice, c->stream);
+    if (e != hipSuccess) rc = set_err(c, FBSE_ERR_HIP, "copy: %s", hipGetErrorString(e));
+  }
+  if (!rc) rc = apply_h_dev_impl(c, d_u, d_hu, k);
+  if (!rc) rc = states_sub_scaled(c, d_u, d_hu, k, sl, d_th);
+  if (!rc) rc = precondition_dev(c, d_hu, d_psi + nbase * sl, k, e_p);
   hipStreamSynchronize(c->stream);
+  if (full) hipFree(d_u);
   hipFree(d_hu);
   hipFree(d_th);
   if (rc) return rc;
   long rank = 0;
-  if ((rc = ortho_dev_impl(c, d_psi, 2 * k, 1.0e-10, &rank, nullptr))) return rc;
+  if ((rc = ortho_dev_impl(c, d_psi, nbase + k, 1.0e-10, &rank, nullptr))) return rc;
   if ((rc = rr_dev_impl(c, d_psi, rank, eval_out))) return rc;
   std::vector<double> ev(rank);
   if ((rc = energies_dev_impl(c, d_psi, rank, ev.data(), sigma_out))) return rc;

@@ -202,6 +202,11 @@ int kinetic_local_tiles(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_
 template <typename T>
 int precondition_tiles(fbse_ctx *c, const cplx_t<T> *r, cplx_t<T> *out, cplx_t<T> *work, long nstates, T pa, T pb);
 
+// FFT path in use for a precision (f32 = 1: the mixed filter's FP32 passes):
+// 1 plane/slab passes, 0 per-axis passes; *instance = plane/slab kernel
+// instance (1-based, 0 = per-axis or not tuned yet).
+int fft_path(const fbse_ctx *c, int f32, int *instance);
+
 // Full H (or Newton step) for nstates complex states: out = a1*H psi + a0*psi.
 // work must hold nstates*nsg complex values.
 template <typename T>
@@ -236,6 +241,6 @@ int states_sub_scaled(fbse_ctx *c, const double *x, double *y, long nstates, lon
 template <typename T>
 int accumulate_window(fbse_ctx *c, const cplx_t<T> *ring, int ring_slots, long slot_stride, long j_lo,
                       long j_hi, const double *d_an, long ms, long ncheby, double *psi_rank, long n_starts,
-                      long n_items, int packed);
+                      long n_items, int packed, int real_coef);
 
 }  // namespace fbse

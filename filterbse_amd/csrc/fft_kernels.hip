@@ -446,6 +446,19 @@ template <typename T> static bool use_tiles(const fbse_ctx *c) {
   return tiles_supported<T>(c);
 }
 
+int fft_path(const fbse_ctx *c, int f32, int *instance) {
+  const bool tiles = f32 ? use_tiles<float>(c) : use_tiles<double>(c);
+  if (instance) {
+    if (!tiles)
+      *instance = 0;
+    else if (c->force_variant == 1 && c->force_lines > 0)
+      *instance = c->force_lines;
+    else
+      *instance = c->tile_variant[f32 ? 1 : 0] + 1;  // 0 while not tuned yet
+  }
+  return tiles ? 1 : 0;
+}
+
 template <typename T>
 int apply_h_cplx(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *work, long nstates, T a1, T a0) {
   const long nb = nstates * c->nspin;

@@ -437,27 +437,28 @@ constexpr int kMaxMs = 16;
 
 constexpr int kAccGroup = 4;  // ring slots loaded per group in accumulate_kernel
 
-template <typename T>
-__global__ __launch_bounds__(256) void accumulate_kernel(const cplx_t<T> *ring, int ring_slots, long slot_stride,
-                                                         long j_lo, int nj, const double *an, int ms, long ncheby,
-                                                         double *psi_rank, long n_starts, long n_items,
+// The coefficients are wave-uniform (same step j and target t in every lane):
+// read through a restrict-qualified pointer they become scalar loads and
+// enter the FMAs as SGPR operands, instead of one LDS broadcast per (t, j)
+// whose 16-byte-per-lane return would cost more than the FMAs it feeds.
+// REAL_A: every an.im is 0 (the Gaussian filter, coeff.c:89-105; always for
+// packed real states): 2 FMAs per (t, j) instead of 4.
+template <typename T, bool REAL_A>
+__global__ __launch_bounds__(256) void accumulate_kernel(const cplx_t<T> *__restrict__ ring, int ring_slots,
+                                                         long slot_stride, long j_lo, int nj,
+                                                         const double *__restrict__ an, int ms, long ncheby,
+                                                         double *__restrict__ psi_rank, long n_starts, long n_items,
                                                          int packed, long nsg) {
-  __shared__ double2 s_a[kMaxMs * kMaxWin];
-  for (int e = threadIdx.x; e < ms * nj; e += blockDim.x) {
-    const int t = e / nj, j = e % nj;
-    s_a[t * kMaxWin + j] = reinterpret_cast<const double2 *>(an)[(long)t * ncheby + j_lo + j];
-  }
-  __syncthreads();
   const long item = blockIdx.y;  // complex: start state; packed: start pair
   const long slot0 = j_lo % ring_slots;
+  const double *a_win = an + 2 * j_lo;  // an[(t*ncheby + j_lo + j)*2 + {0,1}]
   for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < nsg; g += (long)gridDim.x * blockDim.x) {
     const cplx_t<T> *src = ring + item * nsg + g;
     double2 acc[kMaxMs];
 #pragma unroll
     for (int t = 0; t < kMaxMs; ++t) acc[t] = make_double2(0.0, 0.0);
     // The window's slots follow each other modulo the ring.  Loads run one
-    // group of kAccGroup slots ahead of the arithmetic so that the HBM latency
-    // is covered by the previous group's FMAs.
+    // group of kAccGroup slots ahead of the arithmetic.
     cplx_t<T> cur[kAccGroup], nxt[kAccGroup];
     long slot = slot0;
 #pragma unroll
@@ -477,17 +478,18 @@ __global__ __launch_bounds__(256) void accumulate_kernel(const cplx_t<T> *ring, 
       for (int u = 0; u < kAccGroup; ++u) {
         if (j0 + u < nj) {
           const double vx = (double)cur[u].x, vy = (double)cur[u].y;
-          const double2 *sa = s_a + j0 + u;
+          const double *aj = a_win + 2 * (j0 + u);
 #pragma unroll
           for (int t = 0; t < kMaxMs; ++t) {
             if (t < ms) {
-              const double2 a = sa[t * kMaxWin];
-              if (!packed) {
-                acc[t].x += a.x * vx - a.y * vy;
-                acc[t].y += a.x * vy + a.y * vx;
+              const double ar = aj[2 * t * ncheby];
+              if (REAL_A) {
+                acc[t].x += ar * vx;
+                acc[t].y += ar * vy;
               } else {
-                acc[t].x += a.x * vx;
-                acc[t].y += a.x * vy;
+                const double ai = aj[2 * t * ncheby + 1];
+                acc[t].x += ar * vx - ai * vy;
+                acc[t].y += ar * vy + ai * vx;
               }
             }
           }
@@ -524,7 +526,7 @@ __global__ __launch_bounds__(256) void accumulate_kernel(const cplx_t<T> *ring, 
 template <typename T>
 int accumulate_window(fbse_ctx *c, const cplx_t<T> *ring, int ring_slots, long slot_stride, long j_lo, long j_hi,
                       const double *d_an, long ms, long ncheby, double *psi_rank, long n_starts, long n_items,
-                      int packed) {
+                      int packed, int real_coef) {
   const int nj = (int)(j_hi - j_lo);
   if (nj <= 0) return FBSE_OK;
   if (nj > kMaxWin || ms > kMaxMs)
@@ -533,16 +535,20 @@ int accumulate_window(fbse_ctx *c, const cplx_t<T> *ring, int ring_slots, long s
   prof_begin(c, 3, &ev);
   long blocks = (c->nsg + 255) / 256;
   if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(accumulate_kernel<T>, dim3((unsigned)blocks, (unsigned)n_items), dim3(256), 0, c->stream, ring,
-                     ring_slots, slot_stride, j_lo, nj, d_an, (int)ms, ncheby, psi_rank, n_starts, n_items, packed,
-                     c->nsg);
+  const dim3 grid((unsigned)blocks, (unsigned)n_items);
+  if (real_coef || packed)
+    hipLaunchKernelGGL((accumulate_kernel<T, true>), grid, dim3(256), 0, c->stream, ring, ring_slots, slot_stride,
+                       j_lo, nj, d_an, (int)ms, ncheby, psi_rank, n_starts, n_items, packed, c->nsg);
+  else
+    hipLaunchKernelGGL((accumulate_kernel<T, false>), grid, dim3(256), 0, c->stream, ring, ring_slots, slot_stride,
+                       j_lo, nj, d_an, (int)ms, ncheby, psi_rank, n_starts, n_items, packed, c->nsg);
   FBSE_HIP(c, hipGetLastError());
   prof_end(c, 3, ev);
   return FBSE_OK;
 }
 template int accumulate_window<double>(fbse_ctx *, const double2 *, int, long, long, long, const double *, long,
-                                       long, double *, long, long, int);
+                                       long, double *, long, long, int, int);
 template int accumulate_window<float>(fbse_ctx *, const float2 *, int, long, long, long, const double *, long, long,
-                                      double *, long, long, int);
+                                      double *, long, long, int, int);
 
 }  // namespace fbse

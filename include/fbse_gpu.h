@@ -133,14 +133,17 @@ int fbse_sigma(fbse_ctx *ctx, const double *psitot, long n, double *sigma_E, dou
 int fbse_sigma_dev(fbse_ctx *ctx, const double *d_psitot, long n, double *sigma_E, double *eval_aux);
 
 /* One FP64 residual-refinement sweep after a mixed-precision filter (no
- * counterpart in the reference, whose filter is FP64 throughout): the Ritz
+ * counterpart in the reference, whose filter is FP64 throughout).  The k Ritz
  * pairs of a preceding fbse_rayleigh_ritz with eval in [e_lo, e_hi] (and, when
- * sigma is not NULL, sigma[i] < sigma_max) are compacted to the front of psitot, their residuals r = H u - e u are
- * preconditioned in k-space (r / (1 + 0.5|k|^2 / e_p)), and the 2k vectors
- * [u, P r] are orthogonalised and diagonalised again in FP64.  psitot must hold
- * capacity >= 2k states.  On return psitot holds *n_out Ritz vectors with
- * eval_out / sigma_out (capacity-long host arrays) as from fbse_rayleigh_ritz
- * and fbse_sigma.  e_p <= 0 selects 1 Ha. */
+ * sigma is not NULL, sigma[i] < sigma_max) get their residuals r = H u - e u
+ * preconditioned in k-space (r / (1 + 0.5|k|^2 / e_p)).  When n_ritz + k <=
+ * capacity the k corrections are appended after all n_ritz Ritz vectors and
+ * the n_ritz + k vectors are orthogonalised and diagonalised again in FP64;
+ * otherwise the selected pairs are compacted to the front and the 2k vectors
+ * [u, P r] are used (k limited to capacity / 2).  psitot must hold capacity
+ * states.  On return psitot holds *n_out Ritz vectors with eval_out /
+ * sigma_out (capacity-long host arrays) as from fbse_rayleigh_ritz and
+ * fbse_sigma.  e_p <= 0 selects 1 Ha. */
 int fbse_refine(fbse_ctx *ctx, double *psitot, long n_ritz, const double *eval, const double *sigma,
                 double sigma_max, double e_lo, double e_hi, long capacity, double e_p, double *eval_out,
                 double *sigma_out, long *n_out);
@@ -177,6 +180,13 @@ int fbse_set_window(fbse_ctx *ctx, int window);
  * or the plane/slab kernel instance, 1-based (variant 1: FP32 instances
  * 1..3 stage 12, 16, 32 elements per thread, FP64 instances 1..2 stage 12, 16). */
 int fbse_set_fft_kernel(fbse_ctx *ctx, int variant, int lines);
+
+/* The FFT path the context uses for a precision: *path = 1 plane/slab
+ * passes, 0 per-axis passes; *instance (may be NULL) = the plane/slab kernel
+ * instance, 1-based, as picked by the autotuner or forced (0 when the
+ * per-axis passes run or the autotuner has not run yet).  No counterpart in
+ * the reference (its FFTW plans are opaque); for reporting. */
+int fbse_fft_choice(fbse_ctx *ctx, int precision, int *path, int *instance);
 
 /* Device memory helpers for callers without a GPU runtime of their own. */
 int fbse_dev_alloc(fbse_ctx *ctx, size_t bytes, void **d_ptr);
