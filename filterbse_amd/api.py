@@ -78,6 +78,8 @@ GPU_SIGNATURES = {
     "fbse_set_window": (C.c_int, [C.c_void_p, C.c_int]),
     "fbse_set_fft_kernel": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "fbse_fft_choice": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "fbse_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "fbse_graph_replays": (C.c_long, [C.c_void_p]),
     "fbse_dev_alloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     "fbse_dev_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fbse_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
@@ -372,6 +374,14 @@ class Context:
 
     def set_fft_kernel(self, variant, lines=0):
         self._check(gpu().fbse_set_fft_kernel(self.ctx, variant, lines))
+
+    def set_graph(self, enable):
+        """Newton-window hipGraph replay on (1, default) or off (0)."""
+        self._check(gpu().fbse_set_graph(self.ctx, int(enable)))
+
+    def graph_replays(self):
+        """hipGraph replays of the last filter call."""
+        return gpu().fbse_graph_replays(self.ctx)
 
     def fft_choice(self, precision=FBSE_PREC_FP64):
         """(path, instance): path 1 = plane/slab passes, 0 = per-axis passes."""

@@ -368,7 +368,7 @@ extern "C" void fbse_destroy(fbse_ctx *c) {
   void *ptrs[] = {c->d_vloc, c->d_vloc_f, c->d_kx, c->d_ky, c->d_kz, c->d_ksqr, c->d_kx_f, c->d_ky_f, c->d_kz_f,
                   c->d_ksqr_f, c->d_scratch, c->d_scratch2, c->d_red, c->pj.a_off, c->pj.a_cnt, c->pj.jxyz,
                   c->pj.sign, c->pj.proj, c->pj.nlproj, c->pj.y1, c->pj.ls, c->pj_f.proj, c->pj_f.nlproj,
-                  c->pj_f.y1, c->pj_f.ls, c->pj.color_atoms};
+                  c->pj_f.y1, c->pj_f.ls, c->pj.color_atoms, c->d_step};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (int a = 0; a < 3; ++a) {
@@ -424,6 +424,12 @@ extern "C" int fbse_set_fft_kernel(fbse_ctx *c, int variant, int lines) {
   c->force_lines = lines;
   return FBSE_OK;
 }
+extern "C" int fbse_set_graph(fbse_ctx *c, int enable) {
+  if (!c || enable < 0 || enable > 1) return FBSE_ERR_ARG;
+  c->use_graph = enable;
+  return FBSE_OK;
+}
+extern "C" long fbse_graph_replays(const fbse_ctx *c) { return c ? c->graph_replays : -1; }
 extern "C" int fbse_fft_choice(fbse_ctx *c, int precision, int *path, int *instance) {
   if (!c || !path || (precision != FBSE_PREC_FP64 && precision != FBSE_PREC_MIXED)) return FBSE_ERR_ARG;
   *path = fft_path(c, precision == FBSE_PREC_MIXED ? 1 : 0, instance);
@@ -725,23 +731,93 @@ static int filter_impl(fbse_ctx *c, double *d_psi_rank, long n_starts, long ms, 
 
   long acc_lo = 0;
   const double c2 = Vmin * dE_1;
-  for (long j = 1; j < M; ++j) {
+  // Windows: the first covers steps 1..R-1 (slot 0 holds the start states),
+  // window w >= 1 steps wR..wR+R-1, writing slots 0..R-1 in order.  Full
+  // windows w >= 1 therefore launch the same kernels on the same buffers; only
+  // the Newton shifts a0(j) and the coefficient offset differ, and those are
+  // read on the device from a0 table[step counter + i] and the counter.  Such
+  // windows are captured once into a hipGraph and replayed (the counter
+  // advances by R at the end of each replay).
+  const long n_full = M / R - 1;  // full windows w = 1 .. n_full
+  const bool graph = c->use_graph && !c->profile && n_full >= 2;
+  double *d_a0 = nullptr;
+  if (graph) {
+    std::vector<double> a0h(M, 0.0);
+    for (long j = 1; j < M; ++j) a0h[j] = -(2.0 + zn[j - 1] + c2);
+    if (!c->d_step) FBSE_HIP(c, hipMalloc(&c->d_step, sizeof(long)));
+    FBSE_HIP(c, hipMalloc(&d_a0, (size_t)M * sizeof(double)));
+    FBSE_HIP(c, hipMemcpyAsync(d_a0, a0h.data(), (size_t)M * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  }
+  auto step = [&](long j) -> int {
     const C *in = ring + (size_t)((j - 1) % R) * slot_elems;
     C *out = ring + (size_t)(j % R) * slot_elems;
     const double a0 = -(2.0 + zn[j - 1] + c2);
-    if ((rc = apply_h_cplx<T>(c, in, out, work, n_items, (T)dE_1, (T)a0))) break;
+    int r = apply_h_cplx<T>(c, in, out, work, n_items, (T)dE_1, (T)a0);
+    if (r) return r;
     if (j + 1 - acc_lo == R || j == M - 1) {
-      if ((rc = accumulate_window<T>(c, ring, R, slot_elems, acc_lo, j + 1, d_an, ms, M, d_psi_rank, n_starts,
-                                     n_items, packed, real_an)))
-        break;
+      if ((r = accumulate_window<T>(c, ring, R, slot_elems, acc_lo, j + 1, d_an, ms, M, d_psi_rank, n_starts,
+                                    n_items, packed, real_an)))
+        return r;
       acc_lo = j + 1;
     }
+    return FBSE_OK;
+  };
+  long j = 1;
+  c->graph_replays = 0;
+  if (graph) {
+    for (; j < R && !rc; ++j) rc = step(j);  // window 0, eagerly (runs the autotuners)
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    if (!rc) {
+      // counter = R (start of window 1), set by a kernel so the stream order holds
+      if ((rc = fill_zero(c, c->d_step, sizeof(long))) || (rc = step_advance(c, c->d_step, R))) {
+      } else if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+      } else {
+        int crc = FBSE_OK;
+        c->dyn_a0_tab = d_a0;
+        c->dyn_step_base = c->d_step;
+        for (int i = 0; i < R && !crc; ++i) {
+          c->dyn_step_off = i;
+          const C *in = ring + (size_t)((i + R - 1) % R) * slot_elems;
+          C *out = ring + (size_t)i * slot_elems;
+          crc = apply_h_cplx<T>(c, in, out, work, n_items, (T)dE_1, (T)0);
+        }
+        c->dyn_a0_tab = nullptr;
+        c->dyn_step_base = nullptr;
+        if (!crc)
+          crc = accumulate_window<T>(c, ring, R, slot_elems, R, 2 * R, d_an, ms, M, d_psi_rank, n_starts, n_items,
+                                     packed, real_an, c->d_step);
+        if (!crc) crc = step_advance(c, c->d_step, R);
+        hipError_t ee = hipStreamEndCapture(c->stream, &g);
+        hipError_t ei = (crc || ee != hipSuccess) ? hipSuccess : hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (crc)
+          rc = crc;
+        else if (ee != hipSuccess || ei != hipSuccess)
+          rc = set_err(c, FBSE_ERR_HIP, "Newton window capture: %s (fbse_set_graph(ctx, 0) launches eagerly)",
+                       hipGetErrorString(ee != hipSuccess ? ee : ei));
+      }
+    }
+    if (!rc && ge) {
+      for (long w = 1; w <= n_full && !rc; ++w) {
+        hipError_t e = hipGraphLaunch(ge, c->stream);
+        if (e != hipSuccess) rc = set_err(c, FBSE_ERR_HIP, "graph launch: %s", hipGetErrorString(e));
+        else ++c->graph_replays;
+      }
+      j = (n_full + 1) * R;
+      acc_lo = j;
+    }
+    (void)hipStreamSynchronize(c->stream);  // replays done before the exec goes away
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
   }
+  for (; j < M && !rc; ++j) rc = step(j);
   if (!rc && M == 1)
     rc = accumulate_window<T>(c, ring, R, slot_elems, 0, 1, d_an, ms, M, d_psi_rank, n_starts, n_items, packed,
                               real_an);
   hipStreamSynchronize(c->stream);
   hipFree(d_an);
+  if (d_a0) hipFree(d_a0);
   if (!rc) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) rc = set_err(c, FBSE_ERR_HIP, "filter: %s", hipGetErrorString(e));

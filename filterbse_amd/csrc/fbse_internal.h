@@ -57,6 +57,10 @@ template <typename T> struct AxisArgs {
   const T *vloc;
   long vloc_rows;          // rows per V period (nspin*nz*ny)
   T a1, a0;
+  // when a0_tab is set, a0 = a0_tab[*step_base + step_off] (graph replays)
+  const double *a0_tab;
+  const long *step_base;
+  int step_off;
   // preconditioner (z pass): g = pa / (1 + k2 * pb)
   T pa, pb;
 };
@@ -148,6 +152,15 @@ struct fbse_ctx {
   // 0 = not tuned yet.
   int tune_lines[2][2][3][3] = {};
   int tile_variant[2] = {-1, -1};  // plane/slab kernel instance per precision (-1 = not tuned)
+
+  // Newton-window graphs: the Newton shift a0 of step (base + off) is read
+  // from a device table while a window is captured (dyn_a0_tab != nullptr)
+  int use_graph = 1;
+  const double *dyn_a0_tab = nullptr;
+  const long *dyn_step_base = nullptr;
+  int dyn_step_off = 0;
+  long *d_step = nullptr;       // device step counter of the replayed window
+  long graph_replays = 0;       // replays in the last filter
   double tune_ms[2][2][3][3] = {};
   int autotune = 1;
   int force_variant = -1;       // fbse_set_fft_kernel: -1 auto, 0 per-axis passes, 1 plane/slab passes
@@ -238,9 +251,13 @@ int axpby_states(fbse_ctx *c, const double *x, double *y, long n, double a, doub
 int states_sub_scaled(fbse_ctx *c, const double *x, double *y, long nstates, long elems, const double *d_theta);
 
 // window accumulation (ham_kernels.hip)
+// jlo_dev != nullptr: the window starts at step *jlo_dev (read on the device,
+// for graph replays); j_hi - j_lo still gives the window length.
 template <typename T>
 int accumulate_window(fbse_ctx *c, const cplx_t<T> *ring, int ring_slots, long slot_stride, long j_lo,
                       long j_hi, const double *d_an, long ms, long ncheby, double *psi_rank, long n_starts,
-                      long n_items, int packed, int real_coef);
+                      long n_items, int packed, int real_coef, const long *jlo_dev = nullptr);
+// *step += by (one thread; graph-safe counter update)
+int step_advance(fbse_ctx *c, long *step, long by);
 
 }  // namespace fbse

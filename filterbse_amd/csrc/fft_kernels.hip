@@ -228,6 +228,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_axis_kernel(AxisArgs<T> A) {
   }
 
   if (KIND == kPassBwdCombine) {
+    const T a0 = A.a0_tab ? (T)A.a0_tab[*A.step_base + A.step_off] : A.a0;
     const int totv = nl * n;
     for (int e0 = threadIdx.x; e0 < totv; e0 += kFftThreads * NB) {
       T pr[NB], pim[NB];
@@ -251,8 +252,8 @@ __global__ __launch_bounds__(kFftThreads) void fft_axis_kernel(AxisArgs<T> A) {
           const int l = e / n, p = e - l * n;
           const C t = res[l * LS + p];
           C o;
-          o.x = A.a1 * (t.x + vv[u] * pr[u]) + A.a0 * pr[u];
-          o.y = A.a1 * (t.y + vv[u] * pim[u]) + A.a0 * pim[u];
+          o.x = A.a1 * (t.x + vv[u] * pr[u]) + a0 * pr[u];
+          o.y = A.a1 * (t.y + vv[u] * pim[u]) + a0 * pim[u];
           A.dst[base + (long)e] = o;
         }
       }
@@ -308,6 +309,11 @@ static int fft_pass_lines(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src,
   A.psi_in = psi_in;
   A.a1 = a1;
   A.a0 = a0;
+  if (kind == kPassBwdCombine && c->dyn_a0_tab) {
+    A.a0_tab = c->dyn_a0_tab;
+    A.step_base = c->dyn_step_base;
+    A.step_off = c->dyn_step_off;
+  }
   A.pa = a1;  // the preconditioner pass takes (pa, pb) in the (a1, a0) slots
   A.pb = a0;
   A.vloc_rows = c->nspin * c->nz * c->ny;

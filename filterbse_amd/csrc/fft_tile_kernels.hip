@@ -53,6 +53,10 @@ template <typename T> struct TileArgs {
   const T *vloc;
   int nspin;
   T a1, a0;
+  // when a0_tab is set, a0 = a0_tab[*step_base + step_off] (graph replays)
+  const double *a0_tab;
+  const long *step_base;
+  int step_off;
 };
 
 // i mod p for 0 <= i < 2^22, p >= 1: float quotient, then one correction.
@@ -206,6 +210,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T> &A) {
 
   cplx_t<T> *dst = A.dst + base + c0;
   if (KIND == kTileBwdXYCombine) {
+    const T a0 = A.a0_tab ? (T)A.a0_tab[*A.step_base + A.step_off] : A.a0;
     // V is shared by the states: batch item b holds spin b % nspin
     const long vofs = (item % A.nspin) * A.item_stride + (long)w * A.tile_stride + c0;
     const T *vl = A.vloc + vofs;
@@ -228,8 +233,8 @@ __device__ __forceinline__ void tile_body(const TileArgs<T> &A) {
         if (r < nr) {
           const C t = buf[r * ls + c0];
           C o;
-          o.x = A.a1 * (t.x + vv[u] * pr[u]) + A.a0 * pr[u];
-          o.y = A.a1 * (t.y + vv[u] * pim[u]) + A.a0 * pim[u];
+          o.x = A.a1 * (t.x + vv[u] * pr[u]) + a0 * pr[u];
+          o.y = A.a1 * (t.y + vv[u] * pim[u]) + a0 * pim[u];
           dst[r * A.row_stride] = o;
         }
       }
@@ -409,6 +414,11 @@ static int tile_pass_v(fbse_ctx *c, int v, int kind, const cplx_t<T> *src, cplx_
   A.nspin = (int)c->nspin;
   A.a1 = a1;
   A.a0 = a0;
+  if (kind == kTileBwdXYCombine && c->dyn_a0_tab) {
+    A.a0_tab = c->dyn_a0_tab;
+    A.step_base = c->dyn_step_base;
+    A.step_off = c->dyn_step_off;
+  }
   const long ntiles = nbatch * A.tiles_per_item;
   if (ntiles > 0x7fffffffL) return set_err(c, FBSE_ERR_UNSUPPORTED, "%ld FFT tiles in one launch", ntiles);
   int nv;

@@ -448,8 +448,9 @@ __global__ __launch_bounds__(256) void accumulate_kernel(const cplx_t<T> *__rest
                                                          long slot_stride, long j_lo, int nj,
                                                          const double *__restrict__ an, int ms, long ncheby,
                                                          double *__restrict__ psi_rank, long n_starts, long n_items,
-                                                         int packed, long nsg) {
+                                                         int packed, long nsg, const long *__restrict__ jlo_dev) {
   const long item = blockIdx.y;  // complex: start state; packed: start pair
+  if (jlo_dev) j_lo = *jlo_dev;
   const long slot0 = j_lo % ring_slots;
   const double *a_win = an + 2 * j_lo;  // an[(t*ncheby + j_lo + j)*2 + {0,1}]
   for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < nsg; g += (long)gridDim.x * blockDim.x) {
@@ -526,7 +527,7 @@ __global__ __launch_bounds__(256) void accumulate_kernel(const cplx_t<T> *__rest
 template <typename T>
 int accumulate_window(fbse_ctx *c, const cplx_t<T> *ring, int ring_slots, long slot_stride, long j_lo, long j_hi,
                       const double *d_an, long ms, long ncheby, double *psi_rank, long n_starts, long n_items,
-                      int packed, int real_coef) {
+                      int packed, int real_coef, const long *jlo_dev) {
   const int nj = (int)(j_hi - j_lo);
   if (nj <= 0) return FBSE_OK;
   if (nj > kMaxWin || ms > kMaxMs)
@@ -538,17 +539,26 @@ int accumulate_window(fbse_ctx *c, const cplx_t<T> *ring, int ring_slots, long s
   const dim3 grid((unsigned)blocks, (unsigned)n_items);
   if (real_coef || packed)
     hipLaunchKernelGGL((accumulate_kernel<T, true>), grid, dim3(256), 0, c->stream, ring, ring_slots, slot_stride,
-                       j_lo, nj, d_an, (int)ms, ncheby, psi_rank, n_starts, n_items, packed, c->nsg);
+                       j_lo, nj, d_an, (int)ms, ncheby, psi_rank, n_starts, n_items, packed, c->nsg, jlo_dev);
   else
     hipLaunchKernelGGL((accumulate_kernel<T, false>), grid, dim3(256), 0, c->stream, ring, ring_slots, slot_stride,
-                       j_lo, nj, d_an, (int)ms, ncheby, psi_rank, n_starts, n_items, packed, c->nsg);
+                       j_lo, nj, d_an, (int)ms, ncheby, psi_rank, n_starts, n_items, packed, c->nsg, jlo_dev);
   FBSE_HIP(c, hipGetLastError());
   prof_end(c, 3, ev);
   return FBSE_OK;
 }
 template int accumulate_window<double>(fbse_ctx *, const double2 *, int, long, long, long, const double *, long,
-                                       long, double *, long, long, int, int);
+                                       long, double *, long, long, int, int, const long *);
 template int accumulate_window<float>(fbse_ctx *, const float2 *, int, long, long, long, const double *, long, long,
-                                      double *, long, long, int, int);
+                                      double *, long, long, int, int, const long *);
+
+__global__ void step_advance_kernel(long *step, long by) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) step[0] += by;
+}
+int step_advance(fbse_ctx *c, long *step, long by) {
+  hipLaunchKernelGGL(step_advance_kernel, dim3(1), dim3(64), 0, c->stream, step, by);
+  FBSE_HIP(c, hipGetLastError());
+  return FBSE_OK;
+}
 
 }  // namespace fbse

@@ -181,6 +181,15 @@ int fbse_set_window(fbse_ctx *ctx, int window);
  * 1..3 stage 12, 16, 32 elements per thread, FP64 instances 1..2 stage 12, 16). */
 int fbse_set_fft_kernel(fbse_ctx *ctx, int variant, int lines);
 
+/* Newton-window graphs (default on): the filter captures one full
+ * accumulation window (R Newton steps, their accumulation and a device step
+ * counter update) into a hipGraph and replays it for every further full
+ * window; 0 launches every kernel from the host instead.  Results are
+ * identical.  No counterpart in the reference. */
+int fbse_set_graph(fbse_ctx *ctx, int enable);
+/* Graph replays of the last filter call (0 when every window ran eagerly). */
+long fbse_graph_replays(const fbse_ctx *ctx);
+
 /* The FFT path the context uses for a precision: *path = 1 plane/slab
  * passes, 0 per-axis passes; *instance (may be NULL) = the plane/slab kernel
  * instance, 1-based, as picked by the autotuner or forced (0 when the

@@ -80,6 +80,32 @@ def test_filter_mixed(system, variant, lines):
     assert rel(out, ref) < 3e-5, name
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_filter_graph_replay(system, precision):
+    """Full accumulation windows replayed from one captured hipGraph give the
+    same bits as launching every kernel from the host; the graph path is run
+    twice with eager work and synchronisations in between (window 3 -> ring of
+    4 slots, M = 42 -> 9 replays per filter)."""
+    name, s, h, g = system
+    n_starts, ms, M = 2, 3, 42
+    psi_rank, stlen, an, zn, Vmin, dE_1 = filter_inputs(s, n_starts, ms, M)
+    g.set_window(3)
+    g.set_graph(0)
+    ref = g.filter(psi_rank, n_starts, ms, an, zn, Vmin, dE_1, precision=precision)
+    assert g.graph_replays() == 0
+    g.set_graph(1)
+    out1 = g.filter(psi_rank, n_starts, ms, an, zn, Vmin, dE_1, precision=precision)
+    assert g.graph_replays() == 9
+    eager = g.apply_h(random_states(s, 2, seed=3))  # unrelated eager work between replays
+    out2 = g.filter(psi_rank, n_starts, ms, an, zn, Vmin, dE_1, precision=precision)
+    g.set_window(0)
+    assert np.isfinite(eager).all()
+    assert np.array_equal(out1, ref), (name, rel(out1, ref))
+    assert np.array_equal(out2, ref), (name, rel(out2, ref))
+    if precision == 0:
+        assert rel(ref, h.filter_cycles(psi_rank, n_starts, ms, an, zn, Vmin, dE_1)) < 1e-10
+
+
 def test_normalize_energy_sigma(system):
     name, s, h, g = system
     psi = random_states(s, 3, seed=5)
