@@ -136,6 +136,58 @@ def cpu_baseline(S, budget_s=15.0):
                        % (n, S["cfg_name"], el))
 
 
+def measured_traffic(cfg_name, kname):
+    """HBM bytes per launch of the roofline kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload, when it covers
+    the kernel instance in use (else None)."""
+    path = os.path.join(ROOT, "profiles", "%s_fft_traffic.json" % cfg_name)
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if not kname.startswith(t.get("kernel", "?")):
+        return None
+    return t.get("traffic_bytes_per_launch")
+
+
+def traffic_sample(cfg_name, dev, M):
+    """A short mixed filter of the workload (its FFT launches are the ones the
+    roofline names) followed by a device-to-device copy of the bytes one FFT
+    launch reads (and writes): the copy's FETCH_SIZE / WRITE_SIZE per byte
+    calibrates the counters.  The spectrum bounds are a coarse Weyl-type bound
+    (the traffic of a launch does not depend on them)."""
+    import torch
+    from filterbse_amd import api, dots
+
+    cfg = dict(dots.CONFIGS[cfg_name])
+    job = dots.build_dot(cfg["radius_nm"], cfg["n_req"], cfg["dx"], so=cfg["so"], nl=cfg["nl"])
+    ctx = api.Context(job["nx"], job["ny"], job["nz"], job["nspin"], job["is_complex"], job["dv"], job["ksqr"],
+                      job["pot_local"], job["proj"])
+    ngrid = job["nx"] * job["ny"] * job["nz"]
+    Vmin = float(job["pot_local"].min()) - 1.0
+    dE = float(job["ksqr"].max()) * ngrid + float(job["pot_local"].max() - job["pot_local"].min()) + 2.0
+    ms, n_starts = cfg["ms"], cfg["n_starts"]
+    dt = (cfg["ncheby"] / (2.5 * dE)) ** 2
+    zn = api.leja_points(M)
+    an = api.newton_coeff(zn, api.ene_targets(*cfg["targets"]), Vmin, dE, dt)
+    stlen = (job["is_complex"] + 1) * job["nspin"] * ngrid
+    starts, _ = api.init_filter_states(n_starts, ms, -1, job["nx"], job["ny"], job["nz"], job["nspin"],
+                                       job["is_complex"], job["dv"])
+    psi = torch.from_numpy(starts).to(dev)
+    del starts
+    ctx.filter_dev(psi.data_ptr(), n_starts, ms, an, zn, Vmin, 4.0 / dE, api.FBSE_PREC_MIXED)
+    torch.cuda.synchronize()
+    n_items = n_starts if job["is_complex"] else (n_starts + 1) // 2
+    nbytes = n_items * job["nspin"] * ngrid * 8
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log("traffic sample: %d-step mixed filter; copy of %d bytes (one FFT launch reads and writes %d each)"
+        % (M, nbytes, nbytes))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +196,9 @@ def main():
     ap.add_argument("--config", default="c1_so_96")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--traffic-sample", type=int, default=0, metavar="M",
+                    help="only run an M-step mixed filter of the workload and one device copy of the bytes of "
+                         "one FFT launch (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the filter on a single MI355X; "
@@ -153,6 +208,9 @@ def main():
     import torch
     from filterbse_amd import api
     dev = torch.device("cuda:0")
+    if args.traffic_sample:
+        traffic_sample(args.config, dev, args.traffic_sample)
+        return
     S = setup(args.config, dev)
     S["cfg_name"] = args.config
     mixed, fp64 = api.FBSE_PREC_MIXED, api.FBSE_PREC_FP64
@@ -206,7 +264,8 @@ def main():
     else:
         kname = "fft_axis_kernel<float> (x/y forward, z kinetic, y backward passes)"
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel=kname,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(args.config, kname),
+                    kernel=kname,
                     avg_launch_ms=round(avg_ms, 4), bytes_per_launch=bytes_per_launch,
                     launches=fft_launches, measured="HIP events on the library stream, filter of the last timed pass")
     kernels = {name: dict(launches=prof[k][0], total_ms=round(prof[k][1], 3))

@@ -43,6 +43,7 @@ template <typename T> struct TileArgs {
   long tile_stride;            // global elements between tiles of one batch item
   long item_stride;            // global elements between batch items (ngrid)
   int tiles_per_item;
+  long nbatch;                 // batch items (state x spin grids)
   // z passes: k2 = min(kx[x] + ky[y] + kz[z], kcap) when separable, else ksqr[x + nx*y + nx*ny*z]
   const T *kx, *ky, *kz, *ksqr;
   T kcap;
@@ -150,8 +151,12 @@ __device__ __forceinline__ void tile_body(const TileArgs<T> &A) {
   for (int t = threadIdx.x; t < nc; t += NT) twc[t] = A.twc[t];
   for (int t = threadIdx.x; t < nr; t += NT) twr[t] = A.twr[t];
 
-  const long item = blockIdx.x / A.tiles_per_item;
-  const int w = (int)(blockIdx.x - item * A.tiles_per_item);
+  // batch items vary fastest: work-groups that run together (and share an
+  // XCD's L2: the dispatcher deals work-groups round-robin over the 8 XCDs)
+  // transform the same plane of different states and read the same plane of
+  // V_loc (or k^2) once per XCD
+  const int w = (int)(blockIdx.x / A.nbatch);
+  const long item = blockIdx.x - (long)w * A.nbatch;
   const long base = item * A.item_stride + (long)w * A.tile_stride;
   // blockDim.x is a multiple of nc: thread -> column c0, rows r0, r0 + dr, ...
   const int c0 = threadIdx.x % nc, r0 = threadIdx.x / nc, dr = NT / nc;
@@ -420,6 +425,7 @@ static int tile_pass_v(fbse_ctx *c, int v, int kind, const cplx_t<T> *src, cplx_
     A.step_off = c->dyn_step_off;
   }
   const long ntiles = nbatch * A.tiles_per_item;
+  A.nbatch = nbatch;
   if (ntiles > 0x7fffffffL) return set_err(c, FBSE_ERR_UNSUPPORTED, "%ld FFT tiles in one launch", ntiles);
   int nv;
   const TileVariant *tv = tile_variants<T>(&nv);
