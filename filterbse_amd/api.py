@@ -16,6 +16,8 @@ HOST_LIB = os.path.join(HERE, "libfbse_host.so")
 
 FBSE_PREC_FP64 = 0
 FBSE_PREC_MIXED = 1
+FBSE_OK, FBSE_ERR_ARG, FBSE_ERR_HIP, FBSE_ERR_NORM_ZERO, FBSE_ERR_NORM_NAN, FBSE_ERR_SOLVER, FBSE_ERR_NOMEM, \
+    FBSE_ERR_UNSUPPORTED = 0, -1, -2, -3, -4, -5, -6, -7
 
 DP = C.POINTER(C.c_double)
 LP = C.POINTER(C.c_long)
@@ -79,6 +81,9 @@ GPU_SIGNATURES = {
     "fbse_set_fft_kernel": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "fbse_fft_choice": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "fbse_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "fbse_filter_many": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_long),
+                                   C.c_long, C.c_long, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]),
     "fbse_graph_replays": (C.c_long, [C.c_void_p]),
     "fbse_dev_alloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     "fbse_dev_free": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -399,3 +404,29 @@ class Context:
         n, ms = C.c_long(0), C.c_double(0)
         self._check(gpu().fbse_profile_read(self.ctx, kind, C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+
+def filter_many(contexts, d_psi_ranks, n_starts, ms, ans, zns, Vmins, dE_1s, precision=FBSE_PREC_FP64):
+    """fbse_filter_many: the Newton filters of several dots (one Context each,
+    device pointers to each dot's psi_rank) in one call, their launches
+    interleaved over the contexts' streams.  ms and the Newton length are
+    common; an / zn / Vmin / dE_1 / n_starts are per dot."""
+    n = len(contexts)
+    if not (len(d_psi_ranks) == len(n_starts) == len(ans) == len(zns) == len(Vmins) == len(dE_1s) == n):
+        raise ValueError("per-dot arguments must have one entry per context")
+    ans = [_f64(a) for a in ans]
+    zns = [_f64(z) for z in zns]
+    M = len(zns[0])
+    if any(len(z) != M for z in zns):
+        raise ValueError("every dot needs the same Newton length")
+    P = C.c_void_p * n
+    ctxs = P(*[c.ctx for c in contexts])
+    psis = P(*[C.c_void_p(p) for p in d_psi_ranks])
+    a_p = P(*[a.ctypes.data for a in ans])
+    z_p = P(*[z.ctypes.data for z in zns])
+    ns = (C.c_long * n)(*n_starts)
+    vm = (C.c_double * n)(*Vmins)
+    de = (C.c_double * n)(*dE_1s)
+    rc = gpu().fbse_filter_many(n, ctxs, psis, ns, ms, M, a_p, z_p, vm, de, precision)
+    if rc != 0:
+        raise FbseError(rc, gpu().fbse_last_error(contexts[0].ctx).decode())

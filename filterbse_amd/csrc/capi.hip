@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 using namespace fbse;
@@ -678,77 +679,101 @@ extern "C" int fbse_sigma(fbse_ctx *c, const double *psi, long n, double *sigma,
 // ---------------------------------------------------------------------------
 // Newton filter
 // ---------------------------------------------------------------------------
-template <typename T>
-static int filter_impl(fbse_ctx *c, double *d_psi_rank, long n_starts, long ms, long M, const double *an,
-                       const double *zn, double Vmin, double dE_1) {
+// One filter run (one context, one batch of start states).  The run is a
+// small state machine so that several runs (dots) can be interleaved on their
+// own streams by fbse_filter_many.
+//
+// Windows: the first covers steps 1..R-1 (slot 0 holds the start states),
+// window w >= 1 steps wR..wR+R-1, writing slots 0..R-1 in order.  Full windows
+// w >= 1 therefore launch the same kernels on the same buffers; only the
+// Newton shifts a0(j) and the coefficient offset differ, and those are read on
+// the device from a0 table[step counter + i] and the counter.  Such windows are
+// captured once into a hipGraph and replayed (the counter advances by R at the
+// end of each replay).
+template <typename T> struct FilterRun {
   using C = cplx_t<T>;
-  const int packed = !c->is_complex;
-  const long n_items = packed ? (n_starts + 1) / 2 : n_starts;
-  const long nsg = c->nsg;
-  const long sl = stlen_of(c);
-  const long slot_elems = n_items * nsg;
-  const size_t slot_bytes = (size_t)slot_elems * sizeof(C);
-  int rc;
-  // window depth: forced, or as deep as ~40% of free memory allows (<= 63)
-  int K = c->window;
-  if (K <= 0) {
-    size_t fr = 0, tot = 0;
-    hipMemGetInfo(&fr, &tot);
-    long kk = (long)((0.4 * (double)fr) / (double)slot_bytes) - 2;
-    K = (int)std::max(1L, std::min(63L, kk));
-  }
-  if (K > 63) K = 63;
-  const int R = K + 1;
-  // ring (R slots) + one work slot + the double-precision staging for loading
-  if ((rc = ensure_scratch(c, slot_bytes * (size_t)(R + 1)))) return rc;
-  C *ring = (C *)c->d_scratch;
-  C *work = ring + (size_t)R * slot_elems;
-  int real_an = 1;  // all an.im == 0: the accumulation needs only the real parts
-  for (long k = 0; k < ms * M && real_an; ++k) real_an = an[2 * k + 1] == 0.0;
-  double *d_an = nullptr;
-  FBSE_HIP(c, hipMalloc(&d_an, (size_t)ms * M * 2 * sizeof(double)));
-  FBSE_HIP(c, hipMemcpyAsync(d_an, an, (size_t)ms * M * 2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  fbse_ctx *c = nullptr;
+  double *d_psi_rank = nullptr;
+  long n_starts = 0, ms = 0, M = 0;
+  const double *an = nullptr, *zn = nullptr;
+  double dE_1 = 0.0, c2 = 0.0;
+  int packed = 0, real_an = 1, R = 2;
+  long n_items = 0, nsg = 0, sl = 0, slot_elems = 0;
+  C *ring = nullptr, *work = nullptr;
+  double *d_an = nullptr, *d_a0 = nullptr;
+  long acc_lo = 0, j = 1, n_full = 0, w_next = 1;
+  bool graph = false;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
 
-  // start states -> ring slot 0 (packed pairs for real states)
-  {
-    double2 *stage;
-    if constexpr (sizeof(T) == 8) stage = (double2 *)ring;
-    else {
-      if ((rc = ensure_scratch2(c, (size_t)slot_elems * sizeof(double2)))) return rc;
-      stage = (double2 *)c->d_scratch2;
-    }
-    if (packed) {
-      if ((rc = pack_real(c, d_psi_rank, ms * sl, n_starts, stage))) return rc;
-    } else {
-      FBSE_HIP(c, hipMemcpy2DAsync(stage, nsg * sizeof(double2), d_psi_rank, ms * sl * sizeof(double),
-                                   nsg * sizeof(double2), n_starts, hipMemcpyDeviceToDevice, c->stream));
-    }
-    if constexpr (sizeof(T) == 4) {
-      if ((rc = to_f32(c, stage, (float2 *)ring, slot_elems))) return rc;
-    }
+  FilterRun(fbse_ctx *c_, double *psi, long ns, long ms_, long M_, const double *an_, const double *zn_, double Vmin,
+            double dE_1_)
+      : c(c_), d_psi_rank(psi), n_starts(ns), ms(ms_), M(M_), an(an_), zn(zn_), dE_1(dE_1_) {
+    c2 = Vmin * dE_1;
+    packed = !c->is_complex;
+    n_items = packed ? (n_starts + 1) / 2 : n_starts;
+    nsg = c->nsg;
+    sl = stlen_of(c);
+    slot_elems = n_items * nsg;
   }
-  if ((rc = fill_zero(c, d_psi_rank, (size_t)n_starts * ms * sl * sizeof(double)))) return rc;
+  size_t slot_bytes() const { return (size_t)slot_elems * sizeof(C); }
+  ~FilterRun() { release(); }
+  void release() {
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
+    if (d_an) (void)hipFree(d_an);
+    if (d_a0) (void)hipFree(d_a0);
+    ge = nullptr;
+    g = nullptr;
+    d_an = d_a0 = nullptr;
+  }
 
-  long acc_lo = 0;
-  const double c2 = Vmin * dE_1;
-  // Windows: the first covers steps 1..R-1 (slot 0 holds the start states),
-  // window w >= 1 steps wR..wR+R-1, writing slots 0..R-1 in order.  Full
-  // windows w >= 1 therefore launch the same kernels on the same buffers; only
-  // the Newton shifts a0(j) and the coefficient offset differ, and those are
-  // read on the device from a0 table[step counter + i] and the counter.  Such
-  // windows are captured once into a hipGraph and replayed (the counter
-  // advances by R at the end of each replay).
-  const long n_full = M / R - 1;  // full windows w = 1 .. n_full
-  const bool graph = c->use_graph && !c->profile && n_full >= 2;
-  double *d_a0 = nullptr;
-  if (graph) {
-    std::vector<double> a0h(M, 0.0);
-    for (long j = 1; j < M; ++j) a0h[j] = -(2.0 + zn[j - 1] + c2);
-    if (!c->d_step) FBSE_HIP(c, hipMalloc(&c->d_step, sizeof(long)));
-    FBSE_HIP(c, hipMalloc(&d_a0, (size_t)M * sizeof(double)));
-    FBSE_HIP(c, hipMemcpyAsync(d_a0, a0h.data(), (size_t)M * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  // ring (R = K+1 slots) + one work slot; coefficients; start states -> slot 0
+  int prepare(int K) {
+    int rc;
+    R = K + 1;
+    if ((rc = ensure_scratch(c, slot_bytes() * (size_t)(R + 1)))) return rc;
+    ring = (C *)c->d_scratch;
+    work = ring + (size_t)R * slot_elems;
+    real_an = 1;  // all an.im == 0: the accumulation needs only the real parts
+    for (long k = 0; k < ms * M && real_an; ++k) real_an = an[2 * k + 1] == 0.0;
+    FBSE_HIP(c, hipMalloc(&d_an, (size_t)ms * M * 2 * sizeof(double)));
+    FBSE_HIP(c, hipMemcpyAsync(d_an, an, (size_t)ms * M * 2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    {
+      double2 *stage;
+      if constexpr (sizeof(T) == 8) stage = (double2 *)ring;
+      else {
+        if ((rc = ensure_scratch2(c, (size_t)slot_elems * sizeof(double2)))) return rc;
+        stage = (double2 *)c->d_scratch2;
+      }
+      if (packed) {
+        if ((rc = pack_real(c, d_psi_rank, ms * sl, n_starts, stage))) return rc;
+      } else {
+        FBSE_HIP(c, hipMemcpy2DAsync(stage, nsg * sizeof(double2), d_psi_rank, ms * sl * sizeof(double),
+                                     nsg * sizeof(double2), n_starts, hipMemcpyDeviceToDevice, c->stream));
+      }
+      if constexpr (sizeof(T) == 4) {
+        if ((rc = to_f32(c, stage, (float2 *)ring, slot_elems))) return rc;
+      }
+    }
+    if ((rc = fill_zero(c, d_psi_rank, (size_t)n_starts * ms * sl * sizeof(double)))) return rc;
+    n_full = M / R - 1;  // full windows w = 1 .. n_full
+    graph = c->use_graph && !c->profile && n_full >= 2;
+    c->graph_replays = 0;
+    if (graph) {
+      std::vector<double> a0h(M, 0.0);
+      for (long jj = 1; jj < M; ++jj) a0h[jj] = -(2.0 + zn[jj - 1] + c2);
+      if (!c->d_step) FBSE_HIP(c, hipMalloc(&c->d_step, sizeof(long)));
+      FBSE_HIP(c, hipMalloc(&d_a0, (size_t)M * sizeof(double)));
+      FBSE_HIP(c, hipMemcpyAsync(d_a0, a0h.data(), (size_t)M * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      // a0h must outlive the copy
+      FBSE_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return FBSE_OK;
   }
-  auto step = [&](long j) -> int {
+
+  // Newton step j (eager launches), with the window's accumulation when due
+  int step() {
     const C *in = ring + (size_t)((j - 1) % R) * slot_elems;
     C *out = ring + (size_t)(j % R) * slot_elems;
     const double a0 = -(2.0 + zn[j - 1] + c2);
@@ -760,68 +785,146 @@ static int filter_impl(fbse_ctx *c, double *d_psi_rank, long n_starts, long ms, 
         return r;
       acc_lo = j + 1;
     }
+    ++j;
     return FBSE_OK;
-  };
-  long j = 1;
-  c->graph_replays = 0;
-  if (graph) {
-    for (; j < R && !rc; ++j) rc = step(j);  // window 0, eagerly (runs the autotuners)
-    hipGraph_t g = nullptr;
-    hipGraphExec_t ge = nullptr;
-    if (!rc) {
-      // counter = R (start of window 1), set by a kernel so the stream order holds
-      if ((rc = fill_zero(c, c->d_step, sizeof(long))) || (rc = step_advance(c, c->d_step, R))) {
-      } else if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-        (void)hipGetLastError();
-      } else {
-        int crc = FBSE_OK;
-        c->dyn_a0_tab = d_a0;
-        c->dyn_step_base = c->d_step;
-        for (int i = 0; i < R && !crc; ++i) {
-          c->dyn_step_off = i;
-          const C *in = ring + (size_t)((i + R - 1) % R) * slot_elems;
-          C *out = ring + (size_t)i * slot_elems;
-          crc = apply_h_cplx<T>(c, in, out, work, n_items, (T)dE_1, (T)0);
-        }
-        c->dyn_a0_tab = nullptr;
-        c->dyn_step_base = nullptr;
-        if (!crc)
-          crc = accumulate_window<T>(c, ring, R, slot_elems, R, 2 * R, d_an, ms, M, d_psi_rank, n_starts, n_items,
-                                     packed, real_an, c->d_step);
-        if (!crc) crc = step_advance(c, c->d_step, R);
-        hipError_t ee = hipStreamEndCapture(c->stream, &g);
-        hipError_t ei = (crc || ee != hipSuccess) ? hipSuccess : hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-        if (crc)
-          rc = crc;
-        else if (ee != hipSuccess || ei != hipSuccess)
-          rc = set_err(c, FBSE_ERR_HIP, "Newton window capture: %s (fbse_set_graph(ctx, 0) launches eagerly)",
-                       hipGetErrorString(ee != hipSuccess ? ee : ei));
-      }
+  }
+  bool in_window0() const { return j < R && j < M; }
+
+  // capture one full window (call after window 0)
+  int capture() {
+    if (!graph) return FBSE_OK;
+    int rc;
+    // counter = R (start of window 1), set by kernels so the stream order holds
+    if ((rc = fill_zero(c, c->d_step, sizeof(long))) || (rc = step_advance(c, c->d_step, R))) return rc;
+    FBSE_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int crc = FBSE_OK;
+    c->dyn_a0_tab = d_a0;
+    c->dyn_step_base = c->d_step;
+    for (int i = 0; i < R && !crc; ++i) {
+      c->dyn_step_off = i;
+      const C *in = ring + (size_t)((i + R - 1) % R) * slot_elems;
+      C *out = ring + (size_t)i * slot_elems;
+      crc = apply_h_cplx<T>(c, in, out, work, n_items, (T)dE_1, (T)0);
     }
-    if (!rc && ge) {
-      for (long w = 1; w <= n_full && !rc; ++w) {
-        hipError_t e = hipGraphLaunch(ge, c->stream);
-        if (e != hipSuccess) rc = set_err(c, FBSE_ERR_HIP, "graph launch: %s", hipGetErrorString(e));
-        else ++c->graph_replays;
-      }
+    c->dyn_a0_tab = nullptr;
+    c->dyn_step_base = nullptr;
+    if (!crc)
+      crc = accumulate_window<T>(c, ring, R, slot_elems, R, 2 * R, d_an, ms, M, d_psi_rank, n_starts, n_items,
+                                 packed, real_an, c->d_step);
+    if (!crc) crc = step_advance(c, c->d_step, R);
+    hipError_t ee = hipStreamEndCapture(c->stream, &g);
+    hipError_t ei = (crc || ee != hipSuccess) ? hipSuccess : hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (crc) return crc;
+    if (ee != hipSuccess || ei != hipSuccess)
+      return set_err(c, FBSE_ERR_HIP, "Newton window capture: %s (fbse_set_graph(ctx, 0) launches eagerly)",
+                     hipGetErrorString(ee != hipSuccess ? ee : ei));
+    return FBSE_OK;
+  }
+  bool replays_left() const { return graph && ge && w_next <= n_full; }
+  int replay() {
+    FBSE_HIP(c, hipGraphLaunch(ge, c->stream));
+    ++c->graph_replays;
+    ++w_next;
+    if (w_next > n_full) {  // all full windows queued: the tail continues eagerly
       j = (n_full + 1) * R;
       acc_lo = j;
     }
-    (void)hipStreamSynchronize(c->stream);  // replays done before the exec goes away
-    if (ge) (void)hipGraphExecDestroy(ge);
-    if (g) (void)hipGraphDestroy(g);
+    return FBSE_OK;
   }
-  for (; j < M && !rc; ++j) rc = step(j);
-  if (!rc && M == 1)
-    rc = accumulate_window<T>(c, ring, R, slot_elems, 0, 1, d_an, ms, M, d_psi_rank, n_starts, n_items, packed,
-                              real_an);
-  hipStreamSynchronize(c->stream);
-  hipFree(d_an);
-  if (d_a0) hipFree(d_a0);
-  if (!rc) {
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) rc = set_err(c, FBSE_ERR_HIP, "filter: %s", hipGetErrorString(e));
+  bool tail_left() const { return j < M; }
+  int finish() {
+    int rc = FBSE_OK;
+    if (M == 1)
+      rc = accumulate_window<T>(c, ring, R, slot_elems, 0, 1, d_an, ms, M, d_psi_rank, n_starts, n_items, packed,
+                                real_an);
+    (void)hipStreamSynchronize(c->stream);  // replays done before the graph exec goes away
+    release();
+    if (!rc) {
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) rc = set_err(c, FBSE_ERR_HIP, "filter: %s", hipGetErrorString(e));
+    }
+    return rc;
   }
+};
+
+// window depth: forced, or as deep as ~40% of free memory allows for all the
+// runs' rings together (<= 63)
+static int window_depth(int forced, size_t ring_bytes_per_slot) {
+  if (forced > 0) return std::min(forced, 63);
+  size_t fr = 0, tot = 0;
+  (void)hipMemGetInfo(&fr, &tot);
+  long kk = (long)((0.4 * (double)fr) / (double)ring_bytes_per_slot) - 2;
+  return (int)std::max(1L, std::min(63L, kk));
+}
+
+template <typename T>
+static int filter_impl(fbse_ctx *c, double *d_psi_rank, long n_starts, long ms, long M, const double *an,
+                       const double *zn, double Vmin, double dE_1) {
+  FilterRun<T> r(c, d_psi_rank, n_starts, ms, M, an, zn, Vmin, dE_1);
+  int rc = r.prepare(window_depth(c->window, r.slot_bytes()));
+  while (!rc && r.in_window0()) rc = r.step();  // eager (runs the autotuners)
+  if (!rc) rc = r.capture();
+  while (!rc && r.replays_left()) rc = r.replay();
+  while (!rc && r.tail_left()) rc = r.step();
+  int rc2 = r.finish();
+  return rc ? rc : rc2;
+}
+
+// Several dots at once: every run on its own context's stream, launches
+// interleaved step by step and window by window so that the dots' kernels
+// overlap on the device's hardware queues.
+template <typename T>
+static int filter_many_impl(int n, fbse_ctx *const *ctx, double *const *psi, const long *n_starts, long ms, long M,
+                            const double *const *an, const double *const *zn, const double *Vmin,
+                            const double *dE_1) {
+  std::vector<std::unique_ptr<FilterRun<T>>> runs;
+  size_t ring_slot = 0;
+  int forced = 0;
+  for (int d = 0; d < n; ++d) {
+    runs.emplace_back(new FilterRun<T>(ctx[d], psi[d], n_starts[d], ms, M, an[d], zn[d], Vmin[d], dE_1[d]));
+    ring_slot += runs.back()->slot_bytes();
+    if (ctx[d]->window > 0) forced = forced ? std::min(forced, ctx[d]->window) : ctx[d]->window;
+  }
+  const int K = window_depth(forced, ring_slot);
+  int rc = FBSE_OK, bad = -1;
+  auto fail = [&](int d, int r) {
+    if (r && !rc) {
+      rc = r;
+      bad = d;
+    }
+  };
+  for (int d = 0; d < n && !rc; ++d) fail(d, runs[d]->prepare(K));
+  for (bool more = true; more && !rc;) {
+    more = false;
+    for (int d = 0; d < n && !rc; ++d)
+      if (runs[d]->in_window0()) {
+        fail(d, runs[d]->step());
+        more = true;
+      }
+  }
+  for (int d = 0; d < n && !rc; ++d) fail(d, runs[d]->capture());
+  for (bool more = true; more && !rc;) {
+    more = false;
+    for (int d = 0; d < n && !rc; ++d)
+      if (runs[d]->replays_left()) {
+        fail(d, runs[d]->replay());
+        more = true;
+      }
+  }
+  for (bool more = true; more && !rc;) {
+    more = false;
+    for (int d = 0; d < n && !rc; ++d)
+      if (runs[d]->tail_left()) {
+        fail(d, runs[d]->step());
+        more = true;
+      }
+  }
+  for (int d = 0; d < n; ++d) {
+    int r = runs[d]->finish();
+    if (r) fail(d, r);
+  }
+  if (rc && bad >= 0 && ctx[0] != ctx[bad])
+    set_err(ctx[0], rc, "dot %d: %s", bad, ctx[bad]->err.c_str());
   return rc;
 }
 
@@ -833,6 +936,27 @@ extern "C" int fbse_filter_dev(fbse_ctx *c, double *d_psi_rank, long n_starts, l
   FBSE_HIP(c, hipSetDevice(c->device));
   if (precision == FBSE_PREC_FP64) return filter_impl<double>(c, d_psi_rank, n_starts, ms, M, an, zn, Vmin, dE_1);
   if (precision == FBSE_PREC_MIXED) return filter_impl<float>(c, d_psi_rank, n_starts, ms, M, an, zn, Vmin, dE_1);
+  return FBSE_ERR_ARG;
+}
+extern "C" int fbse_filter_many(int n_dots, fbse_ctx *const *ctx, double *const *d_psi_rank, const long *n_starts,
+                                long ms, long M, const double *const *an, const double *const *zn,
+                                const double *Vmin, const double *dE_1, int precision) {
+  if (n_dots < 1 || !ctx || !d_psi_rank || !n_starts || !an || !zn || !Vmin || !dE_1 || ms < 1 || M < 1)
+    return FBSE_ERR_ARG;
+  for (int d = 0; d < n_dots; ++d) {
+    if (!ctx[d] || !d_psi_rank[d] || n_starts[d] < 1 || !an[d] || (M > 1 && !zn[d])) return FBSE_ERR_ARG;
+    if (ctx[d]->device != ctx[0]->device)
+      return set_err(ctx[0], FBSE_ERR_UNSUPPORTED, "dot %d is on device %d, dot 0 on %d", d, ctx[d]->device,
+                     ctx[0]->device);
+    for (int e = 0; e < d; ++e)
+      if (ctx[e] == ctx[d]) return set_err(ctx[0], FBSE_ERR_ARG, "dots %d and %d share a context", e, d);
+  }
+  if (ms > 16) return set_err(ctx[0], FBSE_ERR_UNSUPPORTED, "ms = %ld > 16 energy targets per start", ms);
+  FBSE_HIP(ctx[0], hipSetDevice(ctx[0]->device));
+  if (precision == FBSE_PREC_FP64)
+    return filter_many_impl<double>(n_dots, ctx, d_psi_rank, n_starts, ms, M, an, zn, Vmin, dE_1);
+  if (precision == FBSE_PREC_MIXED)
+    return filter_many_impl<float>(n_dots, ctx, d_psi_rank, n_starts, ms, M, an, zn, Vmin, dE_1);
   return FBSE_ERR_ARG;
 }
 extern "C" int fbse_filter(fbse_ctx *c, double *psi_rank, long n_starts, long ms, long M, const double *an,

@@ -102,6 +102,17 @@ int fbse_filter(fbse_ctx *ctx, double *psi_rank, long n_starts, long ms, long nc
 int fbse_filter_dev(fbse_ctx *ctx, double *d_psi_rank, long n_starts, long ms, long ncheby, const double *an,
                     const double *zn, double Vmin, double dE_1, int precision);
 
+/* Filters of several dots (one context each, e.g. of different grid sizes) in
+ * one call: each dot's Newton recurrence runs on its own context's stream, the
+ * host issues the dots' steps and graph replays interleaved so their kernels
+ * overlap on the GPU, and the window depth is sized for all rings together.
+ * Per dot d the arguments mean what they mean for fbse_filter_dev; ms and M are
+ * common.  Results equal n_dots separate fbse_filter_dev calls.  No counterpart
+ * in the reference (one dot per run); this is the batched uneven-dot workload. */
+int fbse_filter_many(int n_dots, fbse_ctx *const *ctx, double *const *d_psi_rank, const long *n_starts, long ms,
+                     long M, const double *const *an, const double *const *zn, const double *Vmin,
+                     const double *dE_1, int precision);
+
 /* normalize_all (norm.c:82-180): each state scaled to sum |psi|^2 dv = 1.
  * Returns FBSE_ERR_NORM_ZERO / FBSE_ERR_NORM_NAN where the reference exits. */
 int fbse_normalize_all(fbse_ctx *ctx, double *psitot, long n_states);
