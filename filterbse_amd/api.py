@@ -81,6 +81,24 @@ GPU_SIGNATURES = {
     "fbse_set_fft_kernel": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "fbse_fft_choice": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "fbse_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "fbse_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)]),
+    "fbse_mlp_destroy": (None, [C.c_void_p]),
+    "fbse_mlp_last_error": (C.c_char_p, [C.c_void_p]),
+    "fbse_mlp_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "fbse_mlp_input_width": (C.c_int, [C.c_void_p]),
+    "fbse_mlp_num_params": (C.c_long, [C.c_void_p]),
+    "fbse_mlp_get_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "fbse_mlp_set_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "fbse_mlp_forward_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_void_p]),
+    "fbse_mlp_loss_grad_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long,
+                                         C.POINTER(C.c_double), C.POINTER(C.c_float)]),
+    "fbse_mlp_adam_step": (C.c_int, [C.c_void_p, C.c_float]),
+    "fbse_mlp_features_dev": (C.c_int, [C.c_void_p, C.c_long, C.c_long, C.c_long, DP, C.c_double, DP,
+                                        C.POINTER(C.c_int), C.c_long, C.c_int, C.c_void_p, DP, C.c_double,
+                                        C.c_void_p]),
+    "fbse_state_density_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_long), C.c_long, C.c_void_p]),
+    "fbse_warm_start_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_long,
+                                      C.c_long]),
     "fbse_filter_many": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_long),
                                    C.c_long, C.c_long, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                    C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]),
@@ -388,6 +406,17 @@ class Context:
         """hipGraph replays of the last filter call."""
         return gpu().fbse_graph_replays(self.ctx)
 
+    def state_density_dev(self, d_psi, idx, d_rho):
+        """d_rho (ngrid doubles) = sum over the listed states and spins of |psi|^2."""
+        ix = np.ascontiguousarray(idx, dtype=np.int64)
+        self._check(gpu().fbse_state_density_dev(self.ctx, C.c_void_p(d_psi), ix.ctypes.data_as(C.POINTER(C.c_long)),
+                                                 len(ix), C.c_void_p(d_rho)))
+
+    def warm_start_dev(self, d_y, ld, channel, d_psi_rank, n_starts, ms):
+        """Start states jns of psi_rank *= sqrt(exp(y[:, channel])), renormalised."""
+        self._check(gpu().fbse_warm_start_dev(self.ctx, C.c_void_p(d_y), ld, channel, C.c_void_p(d_psi_rank),
+                                              n_starts, ms))
+
     def fft_choice(self, precision=FBSE_PREC_FP64):
         """(path, instance): path 1 = plane/slab passes, 0 = per-axis passes."""
         p, i = C.c_int(), C.c_int()
@@ -430,3 +459,71 @@ def filter_many(contexts, d_psi_ranks, n_starts, ms, ans, zns, Vmins, dE_1s, pre
     rc = gpu().fbse_filter_many(n, ctxs, psis, ns, ms, M, a_p, z_p, vm, de, precision)
     if rc != 0:
         raise FbseError(rc, gpu().fbse_last_error(contexts[0].ctx).decode())
+
+
+class Mlp:
+    """The warm-start network (include/fbse_mlp.h): n_in -> width x3 -> n_out,
+    BF16 weights and activations, FP32 accumulation, FP32 master weights."""
+
+    def __init__(self, n_in, width=128, n_out=2, seed=1, device=0):
+        p = C.c_void_p()
+        rc = gpu().fbse_mlp_create(device, n_in, width, n_out, seed, C.byref(p))
+        if rc != 0:
+            raise FbseError(rc, "fbse_mlp_create(%d, %d, %d)" % (n_in, width, n_out))
+        self.m = p
+        self.n_in, self.width, self.n_out = n_in, width, n_out
+        self.k_in = gpu().fbse_mlp_input_width(self.m)
+        self.n_params = gpu().fbse_mlp_num_params(self.m)
+
+    def __del__(self):
+        if getattr(self, "m", None):
+            gpu().fbse_mlp_destroy(self.m)
+            self.m = None
+
+    def _check(self, rc):
+        if rc != 0:
+            raise FbseError(rc, gpu().fbse_mlp_last_error(self.m).decode())
+
+    def params(self):
+        out = np.zeros(self.n_params, dtype=np.float32)
+        self._check(gpu().fbse_mlp_get_params(self.m, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._check(gpu().fbse_mlp_set_params(self.m, p.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def layers(self, p=None):
+        """[(W, b)] views of a packed parameter vector (W is fan_in x fan_out)."""
+        p = self.params() if p is None else p
+        dims = [(self.n_in, self.width), (self.width, self.width), (self.width, self.width), (self.width, self.n_out)]
+        out, o = [], 0
+        for k, n in dims:
+            w = p[o:o + k * n].reshape(k, n)
+            o += k * n
+            b = p[o:o + n]
+            o += n
+            out.append((w, b))
+        return out
+
+    def forward_dev(self, d_x, n, d_y):
+        self._check(gpu().fbse_mlp_forward_dev(self.m, C.c_void_p(d_x), n, C.c_void_p(d_y)))
+
+    def loss_grad_dev(self, d_x, d_t, d_w, n, want_grads=False):
+        loss = C.c_double(0)
+        g = np.zeros(self.n_params, dtype=np.float32) if want_grads else None
+        self._check(gpu().fbse_mlp_loss_grad_dev(self.m, C.c_void_p(d_x), C.c_void_p(d_t),
+                                                 C.c_void_p(d_w) if d_w else None, n, C.byref(loss),
+                                                 g.ctypes.data_as(C.POINTER(C.c_float)) if want_grads else None))
+        return (loss.value, g) if want_grads else loss.value
+
+    def adam_step(self, lr):
+        self._check(gpu().fbse_mlp_adam_step(self.m, lr))
+
+    def features_dev(self, nx, ny, nz, origin, dx, atoms_xyz, species, n_species, d_vloc, center, r_scale, d_x):
+        xyz = _f64(atoms_xyz).reshape(-1)
+        sp = np.ascontiguousarray(species, dtype=np.int32)
+        org, cen = _f64(origin), _f64(center)
+        self._check(gpu().fbse_mlp_features_dev(self.m, nx, ny, nz, _dp(org), dx, _dp(xyz),
+                                                sp.ctypes.data_as(C.POINTER(C.c_int)), len(sp), n_species,
+                                                C.c_void_p(d_vloc), _dp(cen), r_scale, C.c_void_p(d_x)))
