@@ -97,8 +97,8 @@ GPU_SIGNATURES = {
                                         C.POINTER(C.c_int), C.c_long, C.c_int, C.c_void_p, DP, C.c_double,
                                         C.c_void_p]),
     "fbse_state_density_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_long), C.c_long, C.c_void_p]),
-    "fbse_warm_start_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_long,
-                                      C.c_long]),
+    "fbse_warm_start_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_void_p,
+                                      C.c_long, C.c_long]),
     "fbse_filter_many": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_long),
                                    C.c_long, C.c_long, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                    C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]),
@@ -412,10 +412,11 @@ class Context:
         self._check(gpu().fbse_state_density_dev(self.ctx, C.c_void_p(d_psi), ix.ctypes.data_as(C.POINTER(C.c_long)),
                                                  len(ix), C.c_void_p(d_rho)))
 
-    def warm_start_dev(self, d_y, ld, channel, d_psi_rank, n_starts, ms):
-        """Start states jns of psi_rank *= sqrt(exp(y[:, channel])), renormalised."""
-        self._check(gpu().fbse_warm_start_dev(self.ctx, C.c_void_p(d_y), ld, channel, C.c_void_p(d_psi_rank),
-                                              n_starts, ms))
+    def warm_start_dev(self, d_y, ld, channel, d_psi_rank, n_starts, ms, smooth_ep=0.0):
+        """Start states jns of psi_rank: low-passed (smooth_ep > 0), times
+        sqrt(exp(y[:, channel])), renormalised."""
+        self._check(gpu().fbse_warm_start_dev(self.ctx, C.c_void_p(d_y), ld, channel, smooth_ep,
+                                              C.c_void_p(d_psi_rank), n_starts, ms))
 
     def fft_choice(self, precision=FBSE_PREC_FP64):
         """(path, instance): path 1 = plane/slab passes, 0 = per-axis passes."""
@@ -476,8 +477,11 @@ class Mlp:
         self.n_params = gpu().fbse_mlp_num_params(self.m)
 
     def __del__(self):
-        if getattr(self, "m", None):
-            gpu().fbse_mlp_destroy(self.m)
+        if getattr(self, "m", None) and _gpu is not None:
+            try:
+                _gpu.fbse_mlp_destroy(self.m)
+            except Exception:  # interpreter shutdown
+                pass
             self.m = None
 
     def _check(self, rc):

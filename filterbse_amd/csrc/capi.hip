@@ -1335,7 +1335,7 @@ extern "C" int fbse_rayleigh_ritz(fbse_ctx *c, double *psitot, long n, double *e
 // ---------------------------------------------------------------------------
 // FP64 residual refinement of the Ritz pairs in a window (mixed-precision path)
 // ---------------------------------------------------------------------------
-static int precondition_dev(fbse_ctx *c, const double *d_r, double *d_out, long n, double e_p) {
+int fbse::precondition_dev(fbse_ctx *c, const double *d_r, double *d_out, long n, double e_p) {
   int rc;
   const long ch = h_chunk(c);
   const size_t per = (size_t)c->nsg * sizeof(double2);

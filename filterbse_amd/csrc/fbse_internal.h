@@ -226,6 +226,10 @@ template <typename T>
 int apply_h_cplx(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *work, long nstates, T a1,
                  T a0);
 
+// n contiguous states (reference layout, FP64): out = F^-1 [ F[r] / (N (1 + 0.5|k|^2 / e_p)) ]
+// (capi.hip; out may equal r)
+int precondition_dev(fbse_ctx *c, const double *d_r, double *d_out, long n, double e_p);
+
 // kinetic-energy preconditioner out = F^-1 [ F[r] / (N (1 + 0.5|k|^2 / e_p)) ]
 template <typename T>
 int precondition_cplx(fbse_ctx *c, const cplx_t<T> *r, cplx_t<T> *out, cplx_t<T> *work, long nstates, double e_p);

@@ -787,14 +787,21 @@ extern "C" int fbse_state_density_dev(fbse_ctx *c, const double *d_psi, const lo
   return FBSE_OK;
 }
 
-extern "C" int fbse_warm_start_dev(fbse_ctx *c, const float *d_y, int ld, int channel, double *d_psi_rank,
-                                   long n_starts, long ms) {
-  if (!c || !d_y || ld < 1 || channel < 0 || channel >= ld || !d_psi_rank || n_starts < 0 || ms < 1)
+extern "C" int fbse_warm_start_dev(fbse_ctx *c, const float *d_y, int ld, int channel, double smooth_ep,
+                                   double *d_psi_rank, long n_starts, long ms) {
+  if (!c || !d_y || ld < 1 || channel < 0 || channel >= ld || !d_psi_rank || n_starts < 0 || ms < 1 ||
+      smooth_ep < 0.0)
     return FBSE_ERR_ARG;
   if (n_starts == 0) return FBSE_OK;
   if (n_starts > 65535) return set_err(c, FBSE_ERR_UNSUPPORTED, "%ld start states in one call", n_starts);
   FBSE_HIP(c, hipSetDevice(c->device));
   const long sl = c->nsg * (c->is_complex + 1);
+  if (smooth_ep > 0.0) {  // low-pass the start states first (one state at a time: they are ms*stlen apart)
+    for (long j = 0; j < n_starts; ++j) {
+      int rc = precondition_dev(c, d_psi_rank + j * ms * sl, d_psi_rank + j * ms * sl, 1, smooth_ep);
+      if (rc) return rc;
+    }
+  }
   double *part = nullptr;
   FBSE_HIP(c, hipMalloc(&part, (size_t)n_starts * kWarmBlocks * sizeof(double)));
   hipLaunchKernelGGL(warm_envelope_kernel, dim3(kWarmBlocks, (unsigned)n_starts), dim3(256), 0, c->stream, d_y, ld,

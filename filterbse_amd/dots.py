@@ -220,7 +220,8 @@ def build_dot(radius_nm, n_req, dx, so=True, nl=True, seed=1, ke_max=20.0, poten
     nspin = 2 if so else 1
     pot_local = np.tile(vloc, nspin)
     out = dict(nx=nx, ny=ny, nz=nz, nspin=nspin, is_complex=1 if so else 0, dv=dx ** 3, ksqr=ksqr,
-               pot_local=pot_local, proj=None, syms=syms, R=R, n_core=n_core, mins=mins, dk=dk)
+               pot_local=pot_local, proj=None, syms=syms, R=R, n_core=n_core, mins=mins, dk=dk, dx=dx,
+               origin=(float(gx[0]), float(gy[0]), float(gz[0])), radius_bohr=radius_nm * 10.0 * ANG)
     if so or nl:
         out["proj"] = projector_tables(R[:n_core], syms[:n_core], gaxes, (nx, ny, nz), dx, so, nl)
     return out

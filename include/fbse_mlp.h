@@ -76,12 +76,15 @@ int fbse_mlp_features_dev(fbse_mlp *m, long nx, long ny, long nz, const double *
  * over the state indices idx (host array). */
 int fbse_state_density_dev(struct fbse_ctx *ctx, const double *d_psi, const long *idx, long n, double *d_rho);
 
-/* Warm start: multiplies start state jns of psi_rank (offset jns*ms*stlen, the
- * layout fbse_filter reads) at every grid point and spin by
- * exp(0.5 * clamp(y[g*ld + channel], -30, 30)), i.e. by the square root of the
- * predicted density, and renormalises it to unit norm (sum |psi|^2 dv = 1). */
-int fbse_warm_start_dev(struct fbse_ctx *ctx, const float *d_y, int ld, int channel, double *d_psi_rank,
-                        long n_starts, long ms);
+/* Warm start: start state jns of psi_rank (offset jns*ms*stlen, the layout
+ * fbse_filter reads) is, when smooth_ep > 0, first low-pass filtered in
+ * k-space, F^-1[F[psi] / (1 + 0.5|k|^2 / smooth_ep)] (the refinement
+ * preconditioner's FFT pair), which leaves smooth random envelopes on the
+ * length scale of the dot; then every grid point and spin is multiplied by
+ * exp(0.5 * clamp(y[g*ld + channel], -30, 30)), the square root of the
+ * predicted density, and the state is renormalised (sum |psi|^2 dv = 1). */
+int fbse_warm_start_dev(struct fbse_ctx *ctx, const float *d_y, int ld, int channel, double smooth_ep,
+                        double *d_psi_rank, long n_starts, long ms);
 
 #ifdef __cplusplus
 }
