@@ -188,6 +188,130 @@ def traffic_sample(cfg_name, dev, M):
         % (M, nbytes, nbytes))
 
 
+# Batched uneven dots: 32 spin-orbit dots with seeded radii uniform in
+# 1.0-3.0 nm at dx = 1.0 bohr (the reference grid rule needs ~52^3 ... ~120^3;
+# each axis is rounded up to a size with prime factors 2, 3, 5), 32 starts x 8
+# targets each, Newton length 1024, mixed precision; at most 8 dots per
+# fbse_filter_many call (progress is logged between calls).
+C2 = dict(n_dots=32, r_lo=1.0, r_hi=3.0, dx=1.0, n_starts=32, ms=8, ncheby=1024,
+          targets=(4, 4, -0.32, -0.22, -0.14, -0.04), seed=2, group_max=8)
+
+
+def bench_uneven(args, dev):
+    """One step = the filters of all dots (fbse_filter_many: every dot on its
+    own stream, launches interleaved) plus normalize / ortho / RR / sigma_E of
+    each dot, start states in HBM.  Dots are processed in as few groups as the
+    HBM holds (psi_rank of a group + its FP32 rings)."""
+    import torch
+    from filterbse_amd import api, dots
+
+    cfg = C2
+    rng = np.random.default_rng(cfg["seed"])
+    radii = rng.uniform(cfg["r_lo"], cfg["r_hi"], cfg["n_dots"])
+    D = []
+    t_setup = time.perf_counter()
+    t_range = 0.0
+    targets = api.ene_targets(*cfg["targets"])
+    M, ms, ns = cfg["ncheby"], cfg["ms"], cfg["n_starts"]
+    zn = api.leja_points(M)
+    for i, r in enumerate(radii):
+        # the grid the reference's rule needs, rounded up to FFT-friendly axes
+        n_req = dots.friendly_grid(float(r), cfg["dx"], seed=100 + i)
+        job = dots.build_dot(float(r), n_req, cfg["dx"], so=True, nl=True, seed=100 + i)
+        ctx = api.Context(job["nx"], job["ny"], job["nz"], job["nspin"], job["is_complex"], job["dv"], job["ksqr"],
+                          job["pot_local"], job["proj"])
+        t0 = time.perf_counter()
+        p0, p1 = api.energy_range_start_states(job["nx"], job["ny"], job["nz"], job["nspin"], job["is_complex"],
+                                               job["dv"])
+        rr = ctx.energy_range(p0, p1, max_iter=1500)
+        t_range += time.perf_counter() - t0
+        ctx.trim()
+        Vmin, dE = rr[2], rr[3]
+        an = api.newton_coeff(zn, targets, Vmin, dE, (M / (2.5 * dE)) ** 2)
+        stlen = 2 * job["nspin"] * job["nx"] * job["ny"] * job["nz"]
+        st, _ = api.init_filter_states(ns, 1, -(100 + i), job["nx"], job["ny"], job["nz"], job["nspin"], 1,
+                                       job["dv"])
+        D.append(dict(r=float(r), n=(job["nx"], job["ny"], job["nz"]), atoms=len(job["syms"]), ctx=ctx, an=an,
+                      Vmin=Vmin, dE=dE, stlen=stlen, starts=torch.from_numpy(st).to(dev).view(ns, stlen)))
+        log("dot %2d: r %.2f nm, grid %dx%dx%d, %d atoms" % (i, r, job["nx"], job["ny"], job["nz"], len(job["syms"])))
+        del job, st
+    t_setup = time.perf_counter() - t_setup
+    # groups that fit: psi_rank (FP64) + about four FP32 ring slots per dot
+    free, _ = torch.cuda.mem_get_info()
+    budget = 0.55 * free
+    groups, cur, used = [], [], 0.0
+    for k, d in enumerate(D):
+        need = ns * ms * d["stlen"] * 8 + 4 * ns * d["stlen"] * 4
+        if cur and (used + need > budget or len(cur) == cfg["group_max"]):
+            groups.append(cur)
+            cur, used = [], 0.0
+        cur.append(k)
+        used += need
+    if cur:
+        groups.append(cur)
+    log("%d dots in %d group(s); setup %.1fs (energy ranges %.1fs)" % (len(D), len(groups), t_setup, t_range))
+    mixed = api.FBSE_PREC_MIXED
+    conv_total, t_filter = 0, 0.0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for gi, grp in enumerate(groups):
+        psis = []
+        for k in grp:
+            d = D[k]
+            p = torch.zeros(ns * ms * d["stlen"], dtype=torch.float64, device=dev)
+            p.view(ns, ms, d["stlen"])[:, 0, :].copy_(d["starts"])
+            psis.append(p)
+        torch.cuda.synchronize()
+        tf = time.perf_counter()
+        api.filter_many([D[k]["ctx"] for k in grp], [p.data_ptr() for p in psis], [ns] * len(grp), ms,
+                        [D[k]["an"] for k in grp], [zn] * len(grp), [D[k]["Vmin"] for k in grp],
+                        [4.0 / D[k]["dE"] for k in grp], mixed)
+        t_filter += time.perf_counter() - tf
+        for k, p in zip(grp, psis):
+            c = D[k]["ctx"]
+            c.normalize_all_dev(p.data_ptr(), ns * ms)
+            rank, _ = c.ortho_dev(p.data_ptr(), ns * ms)
+            ev = c.rayleigh_ritz_dev(p.data_ptr(), rank)
+            sig, _ = c.sigma_dev(p.data_ptr(), rank)
+            D[k]["conv"] = int((sig < RESID).sum())
+            D[k]["rank"] = int(rank)
+            conv_total += D[k]["conv"]
+            c.trim()
+        del psis
+        log("group %d (%d dots) done: %.1fs so far" % (gi, len(grp), time.perf_counter() - t0))
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t0
+    hpsi = sum(ns * (M - 1) for _ in D) / t_filter
+    pts = sum(ns * 2 * d["n"][0] * d["n"][1] * d["n"][2] for d in D) * (M - 1) / t_filter
+    line = {
+        "metric": METRIC,
+        "value": round(conv_total / t_total, 4),
+        "unit": "eigenpairs/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": round(1e3 * t_total, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-complex filter, f64 ortho/RR/sigma",
+        "data": "synthetic (seeded zincblende dots, ran2 start states)",
+        "config": {"workload": "c2_uneven: %d spin-orbit dots, radii %.2f-%.2f nm, grids %s, %d starts x %d targets "
+                               "each, Newton length %d" % (len(D), min(radii), max(radii),
+                                                           "%d^3..%d^3" % (min(d["n"][0] for d in D),
+                                                                          max(d["n"][0] for d in D)), ns, ms, M),
+                   "parallelism": "single GPU; dots interleaved over per-dot streams, %d group(s)" % len(groups)},
+        "converged": conv_total,
+        "hpsi_per_s": round(hpsi, 1),
+        "grid_points_per_s": round(pts, 1),
+        "filter_s": round(t_filter, 3),
+        "energy_range_s": round(t_range, 2),
+        "dots": [dict(r_nm=round(d["r"], 3), grid=d["n"], atoms=d["atoms"], rank=d["rank"], converged=d["conv"])
+                 for d in D],
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,6 +334,9 @@ def main():
     dev = torch.device("cuda:0")
     if args.traffic_sample:
         traffic_sample(args.config, dev, args.traffic_sample)
+        return
+    if args.config == "c2_uneven":
+        bench_uneven(args, dev)
         return
     S = setup(args.config, dev)
     S["cfg_name"] = args.config

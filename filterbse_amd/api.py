@@ -103,6 +103,7 @@ GPU_SIGNATURES = {
                                    C.c_long, C.c_long, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                    C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]),
     "fbse_graph_replays": (C.c_long, [C.c_void_p]),
+    "fbse_trim": (C.c_int, [C.c_void_p]),
     "fbse_dev_alloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     "fbse_dev_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fbse_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
@@ -401,6 +402,10 @@ class Context:
     def set_graph(self, enable):
         """Newton-window hipGraph replay on (1, default) or off (0)."""
         self._check(gpu().fbse_set_graph(self.ctx, int(enable)))
+
+    def trim(self):
+        """Release the context's work buffers (allocated again on demand)."""
+        self._check(gpu().fbse_trim(self.ctx))
 
     def graph_replays(self):
         """hipGraph replays of the last filter call."""

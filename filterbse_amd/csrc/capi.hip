@@ -430,6 +430,16 @@ extern "C" int fbse_set_graph(fbse_ctx *c, int enable) {
   c->use_graph = enable;
   return FBSE_OK;
 }
+extern "C" int fbse_trim(fbse_ctx *c) {
+  if (!c) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipSetDevice(c->device));
+  FBSE_HIP(c, hipStreamSynchronize(c->stream));
+  if (c->d_scratch) FBSE_HIP(c, hipFree(c->d_scratch));
+  if (c->d_scratch2) FBSE_HIP(c, hipFree(c->d_scratch2));
+  c->d_scratch = c->d_scratch2 = nullptr;
+  c->scratch_bytes = c->scratch2_bytes = 0;
+  return FBSE_OK;
+}
 extern "C" long fbse_graph_replays(const fbse_ctx *c) { return c ? c->graph_replays : -1; }
 extern "C" int fbse_fft_choice(fbse_ctx *c, int precision, int *path, int *instance) {
   if (!c || !path || (precision != FBSE_PREC_FP64 && precision != FBSE_PREC_MIXED)) return FBSE_ERR_ARG;

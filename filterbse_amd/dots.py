@@ -187,6 +187,27 @@ def cdse_tables():
     return out
 
 
+def fft_friendly(n):
+    """Smallest even m >= n whose prime factors are 2, 3 and 5."""
+    m = max(2, int(n) + (int(n) & 1))
+    while True:
+        k = m
+        for p in (2, 3, 5):
+            while k % p == 0:
+                k //= p
+        if k == 1:
+            return m
+        m += 2
+
+
+def friendly_grid(radius_nm, dx, seed=1, ligands="reference"):
+    """The grid the reference's sizing rule needs for this dot, each axis
+    rounded up to an FFT-friendly size (a grid request the rule keeps)."""
+    syms, R, n_core = zincblende_dot(radius_nm, seed=seed, scheme=ligands)
+    n, _, _, _ = api.grid_params(R, (8, 8, 8), (dx, dx, dx))
+    return tuple(fft_friendly(v) for v in n)
+
+
 def build_dot(radius_nm, n_req, dx, so=True, nl=True, seed=1, ke_max=20.0, potential="cdse", ligands="reference"):
     """Returns the Context arguments plus metadata for a seeded dot.
     potential = "cdse": the reference's CdSe tables; "vq": the V(q) form.

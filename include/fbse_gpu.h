@@ -198,6 +198,11 @@ int fbse_set_fft_kernel(fbse_ctx *ctx, int variant, int lines);
  * window; 0 launches every kernel from the host instead.  Results are
  * identical.  No counterpart in the reference. */
 int fbse_set_graph(fbse_ctx *ctx, int enable);
+/* Releases the context's work buffers (the filter's ring of Newton terms,
+ * FFT work space); they are allocated again on demand.  For callers that keep
+ * many contexts alive (one per dot) and filter them in turns. */
+int fbse_trim(fbse_ctx *ctx);
+
 /* Graph replays of the last filter call (0 when every window ran eagerly). */
 long fbse_graph_replays(const fbse_ctx *ctx);
 
