@@ -139,6 +139,15 @@ def _load(path, sigs):
 def gpu():
     global _gpu
     if _gpu is None:
+        # One HIP runtime per process: PyTorch ships its own libamdhip64 with
+        # the same soname.  Loading torch first lets this library bind to that
+        # runtime (device buffers and streams are then shared); loading
+        # /opt/rocm's first would leave torch with a second runtime that sees
+        # no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _gpu = _load(GPU_LIB, GPU_SIGNATURES)
     return _gpu
 

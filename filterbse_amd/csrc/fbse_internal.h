@@ -93,6 +93,7 @@ template <typename T> struct ProjArgs {
   const cplx_t<T> *ls;
   const cplx_t<T> *psi_in;  // columns: state*nsg + spin*ngrid
   cplx_t<T> *psi_out;
+  cplx_t<T> *dout;          // projection only: mixed coefficients [column][atom][6*MAXP]
   long ngrid, nsg;
   int nspin, ncols;
   T scale;                  // dv * a1
@@ -133,6 +134,15 @@ struct fbse_ctx {
   fbse::ProjTables<double> pj;
   fbse::ProjTables<float> pj_f;
   int has_proj = 0;
+  // projector entries by grid point (spin-0 index): entries of point g are
+  // pt_off[g] .. pt_off[g+1]-1 with atom pt_atom[e] and values pt_val[8e ..]:
+  // proj, nl_proj, Re/Im Y_1m (m = 0..2), FP64 and FP32 copies
+  int *d_pt_off = nullptr, *d_pt_atom = nullptr;
+  double *d_pt_val = nullptr;
+  float *d_pt_val_f = nullptr;
+  void *d_pjd = nullptr;        // mixed projection coefficients (column x atom x 6), grown on demand
+  size_t pjd_bytes = 0;
+  int fuse_proj = 1;            // expansion inside the plane passes when the tables allow it
 
   // scratch (grown on demand)
   void *d_scratch = nullptr;
@@ -209,9 +219,11 @@ int fft_pass(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src, cplx_t<T> *d
 // factors into radices 2, 3, 4, 5, 8 and an xy plane and an xz slab fit the
 // 160 KiB LDS of a work-group.  out = a1*(T + V_loc) psi + a0*psi.
 template <typename T> bool tiles_supported(const fbse_ctx *c);
+// pjd != nullptr: the combine pass also adds the projector expansion with the
+// coefficients of proj_project
 template <typename T>
 int kinetic_local_tiles(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *work, long nstates, T a1,
-                        T a0);
+                        T a0, const cplx_t<T> *pjd = nullptr);
 template <typename T>
 int precondition_tiles(fbse_ctx *c, const cplx_t<T> *r, cplx_t<T> *out, cplx_t<T> *work, long nstates, T pa, T pb);
 
@@ -237,6 +249,14 @@ int precondition_cplx(fbse_ctx *c, const cplx_t<T> *r, cplx_t<T> *out, cplx_t<T>
 // projector launcher (ham_kernels.hip): psi_out += scale * Vproj psi_in
 template <typename T>
 int proj_apply(fbse_ctx *c, const cplx_t<T> *psi_in, cplx_t<T> *psi_out, long nstates, double a1);
+// projections only: d[(col*n_atoms + atom)*6 + r] = the L.S / sign mixed, dv*a1
+// scaled coefficients (r < 3 spin-orbit m, r >= 3 nonlocal m) of one radial
+// projector of each kind; returns FBSE_ERR_UNSUPPORTED when the tables have more
+template <typename T>
+int proj_project(fbse_ctx *c, const cplx_t<T> *psi_in, long nstates, double a1, cplx_t<T> *d);
+// true when the plane passes can add the projector expansion (one radial
+// projector of each kind, entries by point built)
+bool proj_fusable(const fbse_ctx *c);
 
 // element-wise helpers (ham_kernels.hip)
 int pack_real(fbse_ctx *c, const double *src, long src_stride, long nstates, double2 *dst);

@@ -470,6 +470,20 @@ int apply_h_cplx(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *w
   const long nb = nstates * c->nspin;
   int rc;
   if (use_tiles<T>(c)) {
+    if (proj_fusable(c)) {
+      // projections first (they read psi), the expansion rides on the combine pass
+      const size_t need = (size_t)c->pj.n_atoms * nb * 6 * sizeof(cplx_t<T>);
+      if (need > c->pjd_bytes) {
+        if (c->d_pjd) (void)hipFree(c->d_pjd);
+        c->d_pjd = nullptr;
+        c->pjd_bytes = 0;
+        FBSE_HIP(c, hipMalloc(&c->d_pjd, need));
+        c->pjd_bytes = need;
+      }
+      cplx_t<T> *d = (cplx_t<T> *)c->d_pjd;
+      if ((rc = proj_project<T>(c, psi, nstates, (double)a1, d))) return rc;
+      return kinetic_local_tiles<T>(c, psi, out, work, nstates, a1, a0, d);
+    }
     if ((rc = kinetic_local_tiles<T>(c, psi, out, work, nstates, a1, a0))) return rc;
   } else {
     if ((rc = fft_pass<T>(c, 0, kPassFwd, psi, work, nb, nullptr, 0, 0))) return rc;

@@ -58,7 +58,37 @@ template <typename T> struct TileArgs {
   const double *a0_tab;
   const long *step_base;
   int step_off;
+  // projector expansion in the combine pass (pt_off != nullptr): the entries
+  // of spin-grid point g are pt_off[g] .. pt_off[g+1]-1, atom pt_atom[e],
+  // values pt_val[8e ..] = proj, nl_proj, Re/Im Y_1m; coefficients of this
+  // work-group's column pjd[(item*pj_natoms + atom)*6 + r]
+  const int *pt_off, *pt_atom;
+  const T *pt_val;
+  const cplx_t<T> *pjd;
+  int pj_natoms;
 };
+
+// sum over the atoms whose projector sphere holds spin-grid point g of
+// Y_m(g) (proj(g) d_so[m] + nl_proj(g) d_nl[m]) (phamiltonian.c:140-444)
+template <typename T>
+__device__ __forceinline__ cplx_t<T> proj_expansion(const TileArgs<T> &A, long g, const cplx_t<T> *dcol) {
+  cplx_t<T> acc{(T)0, (T)0};
+  const int e1 = A.pt_off[g + 1];
+  for (int e = A.pt_off[g]; e < e1; ++e) {
+    const cplx_t<T> *dd = dcol + (long)A.pt_atom[e] * 6;
+    const T *v = A.pt_val + (long)e * 8;
+    const T wso = v[0], wnl = v[1];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const cplx_t<T> a = dd[m], b = dd[3 + m];
+      const T sx = wso * a.x + wnl * b.x, sy = wso * a.y + wnl * b.y;
+      const T yx = v[2 + 2 * m], yy = v[3 + 2 * m];
+      acc.x += yx * sx - yy * sy;
+      acc.y += yx * sy + yy * sx;
+    }
+  }
+  return acc;
+}
 
 // i mod p for 0 <= i < 2^22, p >= 1: float quotient, then one correction.
 __device__ __forceinline__ int mod_small(int i, int p, float inv_p) {
@@ -220,6 +250,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T> &A) {
     const long vofs = (item % A.nspin) * A.item_stride + (long)w * A.tile_stride + c0;
     const T *vl = A.vloc + vofs;
     const cplx_t<T> *pin = A.psi_in + base + c0;
+    const long gsp0 = (long)w * A.tile_stride + c0;  // spin-grid index of (row 0, c0)
     for (int rb = r0; rb < nr; rb += dr * NB) {
       T pr[NB], pim[NB], vv[NB];
 #pragma unroll
@@ -240,6 +271,11 @@ __device__ __forceinline__ void tile_body(const TileArgs<T> &A) {
           C o;
           o.x = A.a1 * (t.x + vv[u] * pr[u]) + a0 * pr[u];
           o.y = A.a1 * (t.y + vv[u] * pim[u]) + a0 * pim[u];
+          if (A.pt_off) {
+            const C pe = proj_expansion<T>(A, gsp0 + r * A.row_stride, A.pjd + item * A.pj_natoms * 6);
+            o.x += pe.x;
+            o.y += pe.y;
+          }
           dst[r * A.row_stride] = o;
         }
       }
@@ -374,7 +410,7 @@ template <typename T, int KIND> static void set_lds_limit(int device, int v) {
 
 template <typename T>
 static int tile_pass_v(fbse_ctx *c, int v, int kind, const cplx_t<T> *src, cplx_t<T> *dst, long nbatch,
-                       const cplx_t<T> *psi_in, T a1, T a0) {
+                       const cplx_t<T> *psi_in, T a1, T a0, const cplx_t<T> *pjd = nullptr) {
   TileArgs<T> A{};
   A.src = src;
   A.dst = dst;
@@ -424,6 +460,14 @@ static int tile_pass_v(fbse_ctx *c, int v, int kind, const cplx_t<T> *src, cplx_
     A.step_base = c->dyn_step_base;
     A.step_off = c->dyn_step_off;
   }
+  if (kind == kTileBwdXYCombine && pjd) {
+    A.pt_off = c->d_pt_off;
+    A.pt_atom = c->d_pt_atom;
+    if constexpr (sizeof(T) == 8) A.pt_val = c->d_pt_val;
+    else A.pt_val = c->d_pt_val_f;
+    A.pjd = pjd;
+    A.pj_natoms = c->pj.n_atoms;
+  }
   const long ntiles = nbatch * A.tiles_per_item;
   A.nbatch = nbatch;
   if (ntiles > 0x7fffffffL) return set_err(c, FBSE_ERR_UNSUPPORTED, "%ld FFT tiles in one launch", ntiles);
@@ -469,20 +513,20 @@ static int tile_pass_v(fbse_ctx *c, int v, int kind, const cplx_t<T> *src, cplx_
 // before that use the first instance that fits.
 template <typename T>
 static int tile_pass(fbse_ctx *c, int kind, const cplx_t<T> *src, cplx_t<T> *dst, long nbatch,
-                     const cplx_t<T> *psi_in, T a1, T a0) {
+                     const cplx_t<T> *psi_in, T a1, T a0, const cplx_t<T> *pjd = nullptr) {
   const int pi = sizeof(T) == 8 ? 0 : 1;
   if (c->force_variant == 1 && c->force_lines > 0) {  // forced instance (1-based)
     int nv;
     tile_variants<T>(&nv);
     if (c->force_lines > nv) return set_err(c, FBSE_ERR_ARG, "FFT tile instance %d of %d", c->force_lines, nv);
-    return tile_pass_v<T>(c, c->force_lines - 1, kind, src, dst, nbatch, psi_in, a1, a0);
+    return tile_pass_v<T>(c, c->force_lines - 1, kind, src, dst, nbatch, psi_in, a1, a0, pjd);
   }
   if (c->tile_variant[pi] < 0) {
     int nv;
     const TileVariant *tv = tile_variants<T>(&nv);
     int v0 = first_tile_variant<T>(c);
     if (!(c->autotune && nbatch * c->ngrid >= (1L << 20)))
-      return tile_pass_v<T>(c, v0, kind, src, dst, nbatch, psi_in, a1, a0);
+      return tile_pass_v<T>(c, v0, kind, src, dst, nbatch, psi_in, a1, a0, pjd);
     cplx_t<T> *scratch = nullptr;
     const size_t bytes = (size_t)nbatch * c->ngrid * sizeof(cplx_t<T>);
     int best_v = v0;
@@ -527,22 +571,24 @@ static int tile_pass(fbse_ctx *c, int kind, const cplx_t<T> *src, cplx_t<T> *dst
     }
     c->tile_variant[pi] = best_v;
   }
-  return tile_pass_v<T>(c, c->tile_variant[pi], kind, src, dst, nbatch, psi_in, a1, a0);
+  return tile_pass_v<T>(c, c->tile_variant[pi], kind, src, dst, nbatch, psi_in, a1, a0, pjd);
 }
 
 // out = a1*(T + V) psi + a0*psi over nstates*nspin grids (projectors are
 // added by the caller); work holds one batch of grids.
 template <typename T>
 int kinetic_local_tiles(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *work, long nstates, T a1,
-                        T a0) {
+                        T a0, const cplx_t<T> *pjd) {
   const long nb = nstates * c->nspin;
   int rc;
   if ((rc = tile_pass<T>(c, kTileFwdXY, psi, work, nb, nullptr, 0, 0))) return rc;
   if ((rc = tile_pass<T>(c, kTileKinZ, work, work, nb, nullptr, 0, 0))) return rc;
-  return tile_pass<T>(c, kTileBwdXYCombine, work, out, nb, psi, a1, a0);
+  return tile_pass<T>(c, kTileBwdXYCombine, work, out, nb, psi, a1, a0, pjd);
 }
-template int kinetic_local_tiles<double>(fbse_ctx *, const double2 *, double2 *, double2 *, long, double, double);
-template int kinetic_local_tiles<float>(fbse_ctx *, const float2 *, float2 *, float2 *, long, float, float);
+template int kinetic_local_tiles<double>(fbse_ctx *, const double2 *, double2 *, double2 *, long, double, double,
+                                         const double2 *);
+template int kinetic_local_tiles<float>(fbse_ctx *, const float2 *, float2 *, float2 *, long, float, float,
+                                        const float2 *);
 
 template <typename T>
 int precondition_tiles(fbse_ctx *c, const cplx_t<T> *r, cplx_t<T> *out, cplx_t<T> *work, long nstates, T pa,

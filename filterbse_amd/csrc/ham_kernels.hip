@@ -38,7 +38,7 @@ template <typename C> __device__ __forceinline__ C cjm(C a, C b) {
 constexpr int kPjCols = 16;
 constexpr int kPjLanes = 16;  // kPjCols * kPjLanes = 256 threads
 
-template <typename T, int MAXP>
+template <typename T, int MAXP, bool EXPAND = true>
 __global__ __launch_bounds__(256) void proj_color_kernel(ProjArgs<T> P) {
   using C = cplx_t<T>;
   __shared__ C s_d[kProjRows][kPjCols];
@@ -153,6 +153,13 @@ __global__ __launch_bounds__(256) void proj_color_kernel(ProjArgs<T> P) {
   }
   __syncthreads();
   if (!live) return;
+  if (!EXPAND) {  // projection only: hand the mixed coefficients to the plane pass
+    if (lane < 6 * MAXP) {
+      const int rr = lane < 3 * MAXP ? lane : 15 + lane - 3 * MAXP;
+      P.dout[((long)col * P.n_atoms + atom) * (6 * MAXP) + lane] = s_d[rr][cc];
+    }
+    return;
+  }
   // expansion: this thread's column, every kPjLanes-th point
   C d[NR];
 #pragma unroll
@@ -247,6 +254,54 @@ int proj_apply(fbse_ctx *c, const cplx_t<T> *psi_in, cplx_t<T> *psi_out, long ns
   prof_end(c, 2, ev);
   return FBSE_OK;
 }
+bool proj_fusable(const fbse_ctx *c) {
+  if (!c->has_proj || !c->fuse_proj || !c->d_pt_off) return false;
+  const int nso = c->pj.so ? c->pj.n_so : 0, nnl = c->pj.nl ? c->pj.n_nl : 0;
+  return nso <= 1 && nnl <= 1;
+}
+
+template <typename T>
+int proj_project(fbse_ctx *c, const cplx_t<T> *psi_in, long nstates, double a1, cplx_t<T> *d) {
+  if (!proj_fusable(c)) return set_err(c, FBSE_ERR_UNSUPPORTED, "projection-only pass needs one radial projector");
+  ProjArgs<T> P{};
+  const ProjTables<T> *pt;
+  if constexpr (sizeof(T) == 8) pt = &c->pj;
+  else pt = &c->pj_f;
+  P.n_atoms = pt->n_atoms;
+  P.npts = pt->npts;
+  P.so = pt->so;
+  P.nl = pt->nl;
+  P.n_so = pt->n_so;
+  P.n_nl = pt->n_nl;
+  P.a_off = pt->a_off;
+  P.a_cnt = pt->a_cnt;
+  P.jxyz = pt->jxyz;
+  P.proj = pt->proj;
+  P.nlproj = pt->nlproj;
+  P.y1 = pt->y1;
+  P.sign = pt->sign;
+  P.ls = pt->ls;
+  P.psi_in = psi_in;
+  P.psi_out = nullptr;
+  P.dout = d;
+  P.ngrid = c->ngrid;
+  P.nsg = c->nsg;
+  P.nspin = (int)c->nspin;
+  P.ncols = (int)(nstates * c->nspin);
+  P.scale = (T)(c->dv * a1);
+  P.atoms = pt->color_atoms;  // every atom (any order: nothing is written to the grid)
+  if (P.n_atoms == 0 || P.ncols == 0) return FBSE_OK;
+  hipEvent_t ev = nullptr;
+  prof_begin(c, 2, &ev);
+  const int nchunk = (P.ncols + kPjCols - 1) / kPjCols;
+  hipLaunchKernelGGL((proj_color_kernel<T, 1, false>), dim3(P.n_atoms, nchunk), dim3(256), 0, c->stream, P);
+  FBSE_HIP(c, hipGetLastError());
+  prof_end(c, 2, ev);
+  return FBSE_OK;
+}
+template int proj_project<double>(fbse_ctx *, const double2 *, long, double, double2 *);
+template int proj_project<float>(fbse_ctx *, const float2 *, long, double, float2 *);
+
 template int proj_apply<double>(fbse_ctx *, const double2 *, double2 *, long, double);
 template int proj_apply<float>(fbse_ctx *, const float2 *, float2 *, long, double);
 

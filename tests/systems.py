@@ -99,4 +99,6 @@ VARIANTS = {
     "real_nl": dict(nspin=1, is_complex=0, nl=1),
     "spinor_so_nl": dict(nspin=2, is_complex=1, so=1, nl=1),
     "spinor_vloc": dict(nspin=2, is_complex=1),
+    # one radial projector of each kind: the plane passes add the expansion
+    "spinor_so_nl_1p": dict(nspin=2, is_complex=1, so=1, nl=1, n_so_proj=1, n_nl_proj=1),
 }
