@@ -81,6 +81,13 @@ GPU_SIGNATURES = {
     "fbse_set_fft_kernel": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "fbse_fft_choice": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "fbse_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "fbse_build_local_pot": (C.c_int, [C.c_int, C.c_long, DP, DP, DP, C.c_long, C.c_long, C.c_long, DP,
+                                       C.POINTER(C.c_int), C.c_long, DP, DP, LP, DP, C.c_long, C.c_long, DP]),
+    "fbse_count_nl_gridpts": (C.c_long, [C.c_int, DP, C.c_long, DP, DP, DP, C.c_long, C.c_long, C.c_long,
+                                         C.c_double]),
+    "fbse_init_nl_projectors": (C.c_int, [C.c_int, LP, LP, DP, DP, DP, C.POINTER(C.c_int), DP, C.c_long, DP, DP,
+                                          DP, C.c_long, C.c_long, C.c_long, C.c_double, C.c_long, C.c_long, DP,
+                                          DP, C.POINTER(C.c_int), DP]),
     "fbse_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)]),
     "fbse_mlp_destroy": (None, [C.c_void_p]),
     "fbse_mlp_last_error": (C.c_char_p, [C.c_void_p]),

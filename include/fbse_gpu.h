@@ -219,6 +219,39 @@ int fbse_dev_free(fbse_ctx *ctx, void *d_ptr);
 int fbse_memcpy_h2d(fbse_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
 int fbse_memcpy_d2h(fbse_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
 
+/* ---- input side on the GPU (host arrays in and out; device = -1: current) ----
+ * Results are bit-identical to the reference's arithmetic (same IEEE operations
+ * in the same order, no fused multiply-add). */
+
+/* build_local_pot (init.c:587-669): pot_local[jxyz] = sum over all atoms, in
+ * index order, of interpolate(|r - R_a|) in the table of atom type atype[a]:
+ * pot_r / pot_v [n_types x max_len] rows, file_lens[t] rows used, pot_dr[t]
+ * the row spacing; 0 beyond a table.  nspin = 2 repeats the grid (init.c:666).
+ * Grid point (jx, jy, jz) is (gx[jx], gy[jy], gz[jz]), jxyz = jx + nx (jy + ny jz). */
+int fbse_build_local_pot(int device, long nspin, const double *gx, const double *gy, const double *gz, long nx,
+                         long ny, long nz, const double *R, const int *atype, long natoms, const double *pot_r,
+                         const double *pot_v, const long *file_lens, const double *pot_dr, long n_types,
+                         long max_len, double *pot_local);
+
+/* The per-atom count of grid points with |r - R_a|^2 < R_NLcut2 and its
+ * maximum, n_NL_gridpts (init.c:351-394).  Returns the count or a negative
+ * error code. */
+long fbse_count_nl_gridpts(int device, const double *R, long natoms, const double *gx, const double *gy,
+                           const double *gz, long nx, long ny, long nz, double R_NLcut2);
+
+/* init_NL_projectors (init.c:988-1068) in structure-of-arrays form: for atom a
+ * its nl[a] points in grid order at slots a*stride + k: jxyz, y1 (3 complex,
+ * the Y_1m with the EPSDX offsets of init.c:1028-1038), proj[5 per slot]
+ * (radial table so_tab[p*1024 + i] interpolated at |r|, times sqrt(SO_par[a]),
+ * init.c:1046) and nl_proj[5 per slot] (nl_tab[(a*nproj + p)*1024 + i]); the
+ * radial grid is i*sqrt(R_NLcut2)/1024.  sign[a*5 + p] = nl_sgn[a*5 + p].
+ * Slots beyond nl[a] and projectors beyond nproj are set to 0.  Returns
+ * FBSE_ERR_ARG when an atom has more than stride points. */
+int fbse_init_nl_projectors(int device, long *nl, long *jxyz, double *y1, double *proj, double *nl_proj, int *sign,
+                            const double *R, long natoms, const double *gx, const double *gy, const double *gz,
+                            long nx, long ny, long nz, double R_NLcut2, long stride, long nproj,
+                            const double *so_tab, const double *nl_tab, const int *nl_sgn, const double *SO_par);
+
 #ifdef __cplusplus
 }
 #endif

@@ -150,7 +150,9 @@ def reference_job(directory, n_req=(44, 44, 44), dx=0.5, so=1, nl=1, nproj=5, ke
     O.lib().or_build_local_pot(O._dp(pot), nspin, O._dp(gx), O._dp(gy), O._dp(gz), nx, ny, nz, O._dp(Rc),
                                O._ip(atype), natoms, O._dp(pot_r), O._dp(pot_v), O._lp(lens), O._dp(pdr), mlen)
     job = dict(nx=nx, ny=ny, nz=nz, nspin=nspin, is_complex=1 if so else 0, dv=dx ** 3, ksqr=ksqr, pot_local=pot,
-               proj=None, R=R, syms=syms, mins=mins, dk=dk)
+               proj=None, R=R, syms=syms, mins=mins, dk=dk, gx=gx, gy=gy, gz=gz,
+               pot_tables=dict(R=Rc, atype=atype, pot_r=pot_r, pot_v=pot_v, lens=lens, dr=pdr, n_types=len(types),
+                               max_len=mlen))
     if so or nl:
         R2 = 1.5 + 6.0 * math.log(10.0) + 3.0 * dx
         rcut = math.sqrt(R2)
@@ -177,6 +179,8 @@ def reference_job(directory, n_req=(44, 44, 44), dx=0.5, so=1, nl=1, nproj=5, ke
                                       O._ip(sign_out), O._dp(Rc), natoms, O._dp(gx), O._dp(gy), O._dp(gz), nx, ny,
                                       nz, R2, stride, nproj, O._dp(so_tab), O._dp(nl_tab), O._ip(sgn),
                                       O._dp(so_par))
+        job["nl_inputs"] = dict(R2=R2, stride=stride, nproj=nproj, so_tab=so_tab, nl_tab=nl_tab, sgn=sgn,
+                                so_par=so_par)
         job["proj"] = dict(so=so, nl=nl, n_atoms=natoms, n_gridpts=stride, n_gpt=n_gpt, jxyz=jxyz, y1=y1,
                            proj=prj, nl_proj=nlp, nl_sign=sign_out, n_so_proj=5 if so else 0,
                            n_nl_proj=nproj if nl else 0, LS=O.def_LS())

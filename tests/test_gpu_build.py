@@ -1,0 +1,121 @@
+"""The input side on the GPU (fbse_build_local_pot, fbse_count_nl_gridpts,
+fbse_init_nl_projectors) against the oracle's restatement of build_local_pot
+(init.c:587-669), the n_NL_gridpts count (init.c:351-394) and
+init_NL_projectors (init.c:988-1068): bit-identical, on the CsPbI3 job's own
+geometry and potential tables and on random ones."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from systems import PROJ_LEN, _pot_tables
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def grid(n, dx):
+    n_arr = np.array(n)
+    box = np.where(n_arr % 2 == 0, n_arr * dx, (n_arr - 1) * dx)
+    mins = -box / 2.0
+    _, gx, gy, gz = O.build_grid_ksqr(n_arr, np.array([dx, dx, dx]), mins, 2 * math.pi / box, ke_max=20.0)
+    return gx, gy, gz
+
+
+@pytest.mark.parametrize("nspin,natoms,n", [(1, 7, (20, 22, 18)), (2, 33, (24, 16, 20))])
+def test_local_pot_bit_identical(nspin, natoms, n):
+    from filterbse_amd import api
+    rng = np.random.default_rng(natoms)
+    gx, gy, gz = grid(n, 0.6)
+    R = np.ascontiguousarray(rng.uniform(-4, 4, (natoms, 3)))
+    atype = rng.integers(0, 3, natoms).astype(np.int32)
+    pot_r, pot_v, lens, pdr, mlen = _pot_tables(3, rng)
+    ref = np.zeros(nspin * n[0] * n[1] * n[2])
+    O.lib().or_build_local_pot(O._dp(ref), nspin, O._dp(gx), O._dp(gy), O._dp(gz), *n, O._dp(R), O._ip(atype),
+                               natoms, O._dp(pot_r), O._dp(pot_v), O._lp(lens), O._dp(pdr), mlen)
+    out = np.zeros_like(ref)
+    rc = api.gpu().fbse_build_local_pot(-1, nspin, O._dp(gx), O._dp(gy), O._dp(gz), *n, O._dp(R), O._ip(atype),
+                                        natoms, O._dp(pot_r), O._dp(pot_v), O._lp(lens), O._dp(pdr), 3, mlen,
+                                        O._dp(out))
+    assert rc == 0
+    assert np.array_equal(out, ref)
+
+
+def test_local_pot_cspbi3():
+    """The reference example's own conf.par and pot*.par tables."""
+    from filterbse_amd import api
+    from oracle import inputs as I
+    j = I.reference_job(os.path.join(HERE, "golden", "cspbi3"))
+    t = j["pot_tables"]
+    gx, gy, gz = j["gx"], j["gy"], j["gz"]
+    n = (j["nx"], j["ny"], j["nz"])
+    out = np.zeros(j["nspin"] * n[0] * n[1] * n[2])
+    rc = api.gpu().fbse_build_local_pot(-1, j["nspin"], O._dp(gx), O._dp(gy), O._dp(gz), *n, O._dp(t["R"]),
+                                        O._ip(t["atype"]), len(t["atype"]), O._dp(t["pot_r"]), O._dp(t["pot_v"]),
+                                        O._lp(t["lens"]), O._dp(t["dr"]), t["n_types"], t["max_len"], O._dp(out))
+    assert rc == 0
+    assert np.array_equal(out, j["pot_local"])
+
+
+@pytest.mark.parametrize("natoms,n,dx", [(4, (16, 18, 20), 0.55), (21, (30, 24, 28), 0.5)])
+def test_projector_tables_bit_identical(natoms, n, dx):
+    from filterbse_amd import api
+    rng = np.random.default_rng(natoms)
+    gx, gy, gz = grid(n, dx)
+    R = np.ascontiguousarray(rng.uniform(-3, 3, (natoms, 3)))
+    R2 = 1.5 + 6.0 * math.log(10.0) + 3.0 * dx
+    args = (O._dp(R), natoms, O._dp(gx), O._dp(gy), O._dp(gz), *n, R2)
+    stride = O.lib().or_count_nl_gridpts(*args)
+    assert api.gpu().fbse_count_nl_gridpts(-1, *args) == stride
+    vr = np.arange(PROJ_LEN) * (math.sqrt(R2) / PROJ_LEN)
+    so_tab = np.concatenate([(vr ** (1 + (p % 2))) * np.exp(-((vr - 0.4 * p) / 0.9) ** 2) for p in range(5)])
+    nl_tab = np.concatenate([vr * np.exp(-(vr / (0.8 + 0.2 * p + 0.05 * a)) ** 2)
+                             for a in range(natoms) for p in range(5)])
+    sgn = rng.choice([-1, 1], natoms * 5).astype(np.int32)
+    so_par = rng.uniform(0.1, 0.5, natoms)
+    outs = []
+    for who in ("oracle", "gpu"):
+        o = dict(n=np.zeros(natoms, dtype=np.int64), j=np.zeros(natoms * stride, dtype=np.int64),
+                 y=np.zeros(natoms * stride * 6), p=np.zeros(natoms * stride * 5),
+                 q=np.zeros(natoms * stride * 5), s=np.zeros(natoms * 5, dtype=np.int32))
+        a = (O._lp(o["n"]), O._lp(o["j"]), O._dp(o["y"]), O._dp(o["p"]), O._dp(o["q"]), O._ip(o["s"]), O._dp(R),
+             natoms, O._dp(gx), O._dp(gy), O._dp(gz), *n, R2, stride, 5, O._dp(so_tab), O._dp(nl_tab), O._ip(sgn),
+             O._dp(so_par))
+        if who == "oracle":
+            O.lib().or_init_nl_projectors(*a)
+        else:
+            assert api.gpu().fbse_init_nl_projectors(-1, *a) == 0
+        outs.append(o)
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+    # an atom with more points than the stride is an argument error
+    o = outs[1]
+    assert api.gpu().fbse_init_nl_projectors(-1, O._lp(o["n"]), O._lp(o["j"]), O._dp(o["y"]), O._dp(o["p"]),
+                                             O._dp(o["q"]), O._ip(o["s"]), O._dp(R), natoms, O._dp(gx),
+                                             O._dp(gy), O._dp(gz), *n, R2, stride - 1, 5, O._dp(so_tab),
+                                             O._dp(nl_tab), O._ip(sgn), O._dp(so_par)) == -1
+
+
+def test_projector_tables_cspbi3():
+    """The reference example's geometry, SO_*.par parameters and projectors."""
+    from filterbse_amd import api
+    from oracle import inputs as I
+    j = I.reference_job(os.path.join(HERE, "golden", "cspbi3"))
+    t, q, pr = j["pot_tables"], j["nl_inputs"], j["proj"]
+    natoms, stride = len(t["atype"]), q["stride"]
+    n = (j["nx"], j["ny"], j["nz"])
+    assert api.gpu().fbse_count_nl_gridpts(-1, O._dp(t["R"]), natoms, O._dp(j["gx"]), O._dp(j["gy"]),
+                                           O._dp(j["gz"]), *n, q["R2"]) == stride == 2325
+    o = dict(n=np.zeros(natoms, dtype=np.int64), j=np.zeros(natoms * stride, dtype=np.int64),
+             y=np.zeros(natoms * stride * 6), p=np.zeros(natoms * stride * 5), q=np.zeros(natoms * stride * 5),
+             s=np.zeros(natoms * 5, dtype=np.int32))
+    assert api.gpu().fbse_init_nl_projectors(
+        -1, O._lp(o["n"]), O._lp(o["j"]), O._dp(o["y"]), O._dp(o["p"]), O._dp(o["q"]), O._ip(o["s"]),
+        O._dp(t["R"]), natoms, O._dp(j["gx"]), O._dp(j["gy"]), O._dp(j["gz"]), *n, q["R2"], stride, q["nproj"],
+        O._dp(q["so_tab"]), O._dp(q["nl_tab"]), O._ip(q["sgn"]), O._dp(q["so_par"])) == 0
+    for k, ref in (("n", pr["n_gpt"]), ("j", pr["jxyz"]), ("y", pr["y1"]), ("p", pr["proj"]), ("q", pr["nl_proj"]),
+                   ("s", pr["nl_sign"])):
+        assert np.array_equal(o[k], ref), k
