@@ -111,6 +111,7 @@ GPU_SIGNATURES = {
                                    C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]),
     "fbse_graph_replays": (C.c_long, [C.c_void_p]),
     "fbse_trim": (C.c_int, [C.c_void_p]),
+    "fbse_set_refine_basis": (C.c_int, [C.c_void_p, C.c_int]),
     "fbse_dev_alloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     "fbse_dev_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fbse_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
@@ -418,6 +419,10 @@ class Context:
     def set_graph(self, enable):
         """Newton-window hipGraph replay on (1, default) or off (0)."""
         self._check(gpu().fbse_set_graph(self.ctx, int(enable)))
+
+    def set_refine_basis(self, full):
+        """fbse_refine's basis: 0 selected pairs + corrections, 1 all Ritz vectors + corrections."""
+        self._check(gpu().fbse_set_refine_basis(self.ctx, int(full)))
 
     def trim(self):
         """Release the context's work buffers (allocated again on demand)."""

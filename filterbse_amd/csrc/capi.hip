@@ -462,6 +462,11 @@ extern "C" int fbse_set_graph(fbse_ctx *c, int enable) {
   c->use_graph = enable;
   return FBSE_OK;
 }
+extern "C" int fbse_set_refine_basis(fbse_ctx *c, int full) {
+  if (!c || full < 0 || full > 1) return FBSE_ERR_ARG;
+  c->refine_full = full;
+  return FBSE_OK;
+}
 extern "C" int fbse_trim(fbse_ctx *c) {
   if (!c) return FBSE_ERR_ARG;
   FBSE_HIP(c, hipSetDevice(c->device));
@@ -1418,12 +1423,12 @@ extern "C" int fbse_refine_dev(fbse_ctx *c, double *d_psi, long n_ritz, const do
     if (eval[i] >= e_lo && eval[i] <= e_hi && (!sigma || sigma[i] < sigma_max)) sel.push_back(i);
   *n_out = 0;
   if (sel.empty()) return FBSE_OK;
-  // Full basis when [all n_ritz Ritz vectors, P r of every selected pair] fits
-  // in capacity: the corrections then stay orthogonal to the whole filtered
-  // subspace, which matters when the window holds many close eigenvalues.
-  // Otherwise the selected pairs are compacted to the front and the basis is
-  // [u_sel, P r_sel] (k <= capacity / 2).
-  const bool full = n_ritz + (long)sel.size() <= capacity;
+  // Compact basis (default): the selected pairs are moved to the front and the
+  // basis is [u_sel, P r_sel] (k <= capacity / 2).  Full basis
+  // (fbse_set_refine_basis(ctx, 1)) when [all n_ritz Ritz vectors, P r of every
+  // selected pair] fits in capacity: the corrections stay orthogonal to the
+  // whole filtered subspace, at the price of an ortho + RR over n_ritz + k.
+  const bool full = c->refine_full && n_ritz + (long)sel.size() <= capacity;
   const long k = full ? (long)sel.size() : std::min((long)sel.size(), capacity / 2);
   const long nbase = full ? n_ritz : k;  // corrections go to columns nbase .. nbase+k-1
   std::vector<double> theta(k);

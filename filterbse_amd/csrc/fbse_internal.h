@@ -143,6 +143,7 @@ struct fbse_ctx {
   void *d_pjd = nullptr;        // mixed projection coefficients (column x atom x 6), grown on demand
   size_t pjd_bytes = 0;
   int fuse_proj = 1;            // expansion inside the plane passes when the tables allow it
+  int refine_full = 0;          // fbse_set_refine_basis
 
   // scratch (grown on demand)
   void *d_scratch = nullptr;

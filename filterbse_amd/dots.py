@@ -208,10 +208,85 @@ def friendly_grid(radius_nm, dx, seed=1, ligands="reference"):
     return tuple(fft_friendly(v) for v in n)
 
 
-def build_dot(radius_nm, n_req, dx, so=True, nl=True, seed=1, ke_max=20.0, potential="cdse", ligands="reference"):
+def _gpu_local_pot(syms, R, tables, gaxes, nspin):
+    """V_loc by fbse_build_local_pot (the reference's all-atom interpolation)."""
+    import ctypes as C
+    gx, gy, gz = gaxes
+    types = sorted(set(syms), key=list(syms).index)
+    atype = np.array([types.index(s) for s in syms], dtype=np.int32)
+    mlen = max(len(tables[t][0]) for t in types)
+    pot_r = np.zeros(len(types) * mlen)
+    pot_v = np.zeros(len(types) * mlen)
+    lens = np.zeros(len(types), dtype=np.int64)
+    pdr = np.zeros(len(types))
+    for k, t in enumerate(types):
+        vr, pot = tables[t]
+        pot_r[k * mlen:k * mlen + len(vr)] = vr
+        pot_v[k * mlen:k * mlen + len(pot)] = pot
+        lens[k] = len(vr)
+        pdr[k] = vr[1] - vr[0]
+    n = (len(gx), len(gy), len(gz))
+    out = np.zeros(nspin * n[0] * n[1] * n[2])
+    Rc = np.ascontiguousarray(R, dtype=np.float64)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    rc = api.gpu().fbse_build_local_pot(-1, nspin, dp(gx), dp(gy), dp(gz), *n, dp(Rc),
+                                        atype.ctypes.data_as(C.POINTER(C.c_int)), len(atype), dp(pot_r), dp(pot_v),
+                                        lens.ctypes.data_as(C.POINTER(C.c_long)), dp(pdr), len(types), mlen,
+                                        dp(out))
+    if rc != 0:
+        raise api.FbseError(rc, "fbse_build_local_pot")
+    return out
+
+
+def _gpu_projector_tables(R, syms, gaxes, dx, so, nl):
+    """Projector tables by fbse_count_nl_gridpts / fbse_init_nl_projectors: the
+    radial function r exp(-(r/1.2)^2) tabulated on the reference's 1024-point
+    radial grid, scaled per species as projector_tables does."""
+    import ctypes as C
+    gx, gy, gz = gaxes
+    n = (len(gx), len(gy), len(gz))
+    R2 = 1.5 + 6.0 * math.log(10.0) + 3.0 * dx
+    vr = np.arange(1024) * (math.sqrt(R2) / 1024)
+    beta = vr * np.exp(-(vr / 1.2) ** 2)
+    na = len(R)
+    Rc = np.ascontiguousarray(R, dtype=np.float64)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    lp = lambda a: a.ctypes.data_as(C.POINTER(C.c_long))
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+    stride = api.gpu().fbse_count_nl_gridpts(-1, dp(Rc), na, dp(gx), dp(gy), dp(gz), *n, R2)
+    if stride < 0:
+        raise api.FbseError(int(stride), "fbse_count_nl_gridpts")
+    so_tab = beta.copy()
+    nl_tab = np.concatenate([beta * math.sqrt(abs(NL_LAMBDA[s])) for s in syms])
+    nl_sgn = np.zeros(na * 5, dtype=np.int32)
+    nl_sgn[::5] = [(-1 if NL_LAMBDA[s] < 0 else 1) for s in syms]
+    so_par = np.array([SO_PAR[s] if so else 0.0 for s in syms])
+    n_gpt = np.zeros(na, dtype=np.int64)
+    jxyz = np.zeros(na * stride, dtype=np.int64)
+    y1 = np.zeros(na * stride * 6)
+    prj = np.zeros(na * stride * 5)
+    nlp = np.zeros(na * stride * 5)
+    sign = np.zeros(na * 5, dtype=np.int32)
+    rc = api.gpu().fbse_init_nl_projectors(-1, lp(n_gpt), lp(jxyz), dp(y1), dp(prj), dp(nlp), ip(sign), dp(Rc), na,
+                                           dp(gx), dp(gy), dp(gz), *n, R2, stride, 1, dp(so_tab), dp(nl_tab),
+                                           ip(nl_sgn), dp(so_par))
+    if rc != 0:
+        raise api.FbseError(rc, "fbse_init_nl_projectors")
+    if not nl:
+        nlp[:] = 0.0
+        sign[:] = 0
+    return dict(so=int(so), nl=int(nl), n_atoms=na, n_gridpts=int(stride), n_gpt=n_gpt, jxyz=jxyz, y1=y1, proj=prj,
+                nl_proj=nlp, nl_sign=sign, n_so_proj=1 if so else 0, n_nl_proj=1 if nl else 0, LS=api.def_LS())
+
+
+def build_dot(radius_nm, n_req, dx, so=True, nl=True, seed=1, ke_max=20.0, potential="cdse", ligands="reference",
+              builder="host"):
     """Returns the Context arguments plus metadata for a seeded dot.
     potential = "cdse": the reference's CdSe tables; "vq": the V(q) form.
-    ligands = "reference" or "config" (see LIGANDS)."""
+    ligands = "reference" or "config" (see LIGANDS).  builder = "gpu" builds
+    V_loc and the projector tables with the library's GPU builders (all-atom
+    sums and tabulated radial functions, as the reference computes them);
+    "host" with numpy (per-atom boxes, analytic radial functions)."""
     syms, R, n_core = zincblende_dot(radius_nm, seed=seed, scheme=ligands)
     n, mins, maxs, dk = api.grid_params(R, n_req, (dx, dx, dx))
     ksqr, gx, gy, gz = api.grid_ksqr(n, (dx, dx, dx), mins, dk, ke_max)
@@ -224,8 +299,17 @@ def build_dot(radius_nm, n_req, dx, so=True, nl=True, seed=1, ke_max=20.0, poten
     for lig in ("P1", "P2"):
         a_, b_, _ = LIGANDS[ligands][lig]
         tables[lig] = ligand_table(a_, b_)
-    vloc = np.zeros((nz, ny, nx))
     gaxes = (gx, gy, gz)
+    nspin = 2 if so else 1
+    if builder == "gpu":
+        pot_local = _gpu_local_pot(syms, R, tables, gaxes, nspin)
+        out = dict(nx=nx, ny=ny, nz=nz, nspin=nspin, is_complex=1 if so else 0, dv=dx ** 3, ksqr=ksqr,
+                   pot_local=pot_local, proj=None, syms=syms, R=R, n_core=n_core, mins=mins, dk=dk, dx=dx,
+                   origin=(float(gx[0]), float(gy[0]), float(gz[0])), radius_bohr=radius_nm * 10.0 * ANG)
+        if so or nl:
+            out["proj"] = _gpu_projector_tables(R[:n_core], syms[:n_core], gaxes, dx, so, nl)
+        return out
+    vloc = np.zeros((nz, ny, nx))
     for s, p in zip(syms, R):
         vr, pot = tables[s]
         dr = vr[1] - vr[0]
@@ -238,7 +322,6 @@ def build_dot(radius_nm, n_req, dx, so=True, nl=True, seed=1, ke_max=20.0, poten
         dist = np.sqrt(dxx * dxx + dy * dy + dz * dz)
         vloc[np.ix_(iz, iy, ix)] += _interp(dist, dr, vr, pot)
     vloc = vloc.ravel()
-    nspin = 2 if so else 1
     pot_local = np.tile(vloc, nspin)
     out = dict(nx=nx, ny=ny, nz=nz, nspin=nspin, is_complex=1 if so else 0, dv=dx ** 3, ksqr=ksqr,
                pot_local=pot_local, proj=None, syms=syms, R=R, n_core=n_core, mins=mins, dk=dk, dx=dx,

@@ -147,11 +147,12 @@ int fbse_sigma_dev(fbse_ctx *ctx, const double *d_psitot, long n, double *sigma_
  * counterpart in the reference, whose filter is FP64 throughout).  The k Ritz
  * pairs of a preceding fbse_rayleigh_ritz with eval in [e_lo, e_hi] (and, when
  * sigma is not NULL, sigma[i] < sigma_max) get their residuals r = H u - e u
- * preconditioned in k-space (r / (1 + 0.5|k|^2 / e_p)).  When n_ritz + k <=
- * capacity the k corrections are appended after all n_ritz Ritz vectors and
- * the n_ritz + k vectors are orthogonalised and diagonalised again in FP64;
- * otherwise the selected pairs are compacted to the front and the 2k vectors
- * [u, P r] are used (k limited to capacity / 2).  psitot must hold capacity
+ * preconditioned in k-space (r / (1 + 0.5|k|^2 / e_p)).  By default the
+ * selected pairs are compacted to the front and the 2k vectors [u, P r] are
+ * orthogonalised and diagonalised again in FP64 (k limited to capacity / 2);
+ * after fbse_set_refine_basis(ctx, 1), when n_ritz + k <= capacity, the k
+ * corrections are appended after all n_ritz Ritz vectors and the n_ritz + k
+ * vectors are used instead.  psitot must hold capacity
  * states.  On return psitot holds *n_out Ritz vectors with eval_out /
  * sigma_out (capacity-long host arrays) as from fbse_rayleigh_ritz and
  * fbse_sigma.  e_p <= 0 selects 1 Ha. */
@@ -198,6 +199,10 @@ int fbse_set_fft_kernel(fbse_ctx *ctx, int variant, int lines);
  * window; 0 launches every kernel from the host instead.  Results are
  * identical.  No counterpart in the reference. */
 int fbse_set_graph(fbse_ctx *ctx, int enable);
+/* Refinement basis of fbse_refine: 0 = the selected pairs and their
+ * corrections (default), 1 = every Ritz vector plus the corrections. */
+int fbse_set_refine_basis(fbse_ctx *ctx, int full);
+
 /* Releases the context's work buffers (the filter's ring of Newton terms,
  * FFT work space); they are allocated again on demand.  For callers that keep
  * many contexts alive (one per dot) and filter them in turns. */

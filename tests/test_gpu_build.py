@@ -119,3 +119,22 @@ def test_projector_tables_cspbi3():
     for k, ref in (("n", pr["n_gpt"]), ("j", pr["jxyz"]), ("y", pr["y1"]), ("p", pr["proj"]), ("q", pr["nl_proj"]),
                    ("s", pr["nl_sign"])):
         assert np.array_equal(o[k], ref), k
+
+
+def test_dot_generator_builders_agree():
+    """The synthetic dot generator's numpy build (per-atom boxes, analytic
+    radial projector) and its GPU build (the reference's all-atom sum,
+    tabulated radial projector) describe the same Hamiltonian."""
+    from filterbse_amd import dots
+    a = dots.build_dot(1.0, (48, 48, 48), 1.0, so=True, nl=True, seed=3, builder="host")
+    b = dots.build_dot(1.0, (48, 48, 48), 1.0, so=True, nl=True, seed=3, builder="gpu")
+    assert (a["nx"], a["ny"], a["nz"]) == (b["nx"], b["ny"], b["nz"])
+    scale = np.abs(a["pot_local"]).max()
+    assert np.abs(a["pot_local"] - b["pot_local"]).max() < 1e-9 * scale
+    pa, pb = a["proj"], b["proj"]
+    assert pa["n_gridpts"] == pb["n_gridpts"] and np.array_equal(pa["n_gpt"], pb["n_gpt"])
+    assert np.array_equal(pa["jxyz"], pb["jxyz"])
+    assert np.array_equal(pa["y1"], pb["y1"])
+    assert np.array_equal(pa["nl_sign"], pb["nl_sign"])
+    for k in ("proj", "nl_proj"):
+        assert np.abs(pa[k] - pb[k]).max() < 1e-4 * np.abs(pa[k]).max(), k

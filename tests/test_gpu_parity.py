@@ -176,3 +176,36 @@ def test_energy_range(system):
     out = g.energy_range(p0, p1, max_iter=60)
     assert out[4] == ref[4] and out[5] == ref[5]
     assert np.abs(out[:4] - ref[:4]).max() < 1e-9 * np.abs(ref[:4]).max()
+
+
+def test_refine_bases_agree():
+    """fbse_refine with the compact basis (selected pairs + corrections) and with
+    the full basis (all Ritz vectors + corrections) after a mixed filter: both
+    lower the best residual of the filtered subspace (2e-2 -> 1e-3 here), and
+    their best Ritz pairs agree to within the square of that residual."""
+    import torch
+    s = small_system(**VARIANTS["spinor_so_nl_1p"])
+    g = gpu_context(s)
+    n_starts, ms, M = 4, 6, 600
+    psi_rank, stlen, an, zn, Vmin, dE_1 = filter_inputs(s, n_starts, ms, M)
+    n = n_starts * ms
+    out = {}
+    for full in (0, 1):
+        d = torch.from_numpy(psi_rank.copy()).cuda()
+        p = d.data_ptr()
+        g.filter_dev(p, n_starts, ms, an, zn, Vmin, dE_1, 1)
+        g.normalize_all_dev(p, n)
+        rank, _ = g.ortho_dev(p, n)
+        ev = g.rayleigh_ritz_dev(p, rank)
+        sig, _ = g.sigma_dev(p, rank)
+        g.set_refine_basis(full)
+        ev2, sig2 = g.refine_dev(p, rank, ev, ev.min() - 1.0, np.sort(ev)[rank // 2], n, sigma=sig, sigma_max=1.0)
+        g.set_refine_basis(0)
+        assert np.isfinite(ev2).all() and np.isfinite(sig2).all()
+        assert sig2.min() < sig.min()
+        out[full] = (ev2, sig2)
+        print("full", full, "best sigma before %.2e after %.2e" % (sig.min(), sig2.min()),
+              "n<1e-5: %d" % int((sig2 < 1e-5).sum()))
+    (ea, sa), (eb, sb) = out[0], out[1]
+    for k in np.argsort(sa)[:3]:
+        assert np.min(np.abs(eb - ea[k])) < 10 * sa[k] ** 2 + 1e-10, (ea[k], sa[k])
