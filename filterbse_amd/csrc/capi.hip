@@ -401,7 +401,7 @@ extern "C" void fbse_destroy(fbse_ctx *c) {
                   c->d_ksqr_f, c->d_scratch, c->d_scratch2, c->d_red, c->pj.a_off, c->pj.a_cnt, c->pj.jxyz,
                   c->pj.sign, c->pj.proj, c->pj.nlproj, c->pj.y1, c->pj.ls, c->pj_f.proj, c->pj_f.nlproj,
                   c->pj_f.y1, c->pj_f.ls, c->pj.color_atoms, c->d_step, c->d_pt_off, c->d_pt_atom,
-                  c->d_pt_val, c->d_pt_val_f, c->d_pjd};
+                  c->d_pt_val, c->d_pt_val_f, c->d_pjd, c->d_gram};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (int a = 0; a < 3; ++a) {
@@ -473,8 +473,9 @@ extern "C" int fbse_trim(fbse_ctx *c) {
   FBSE_HIP(c, hipStreamSynchronize(c->stream));
   if (c->d_scratch) FBSE_HIP(c, hipFree(c->d_scratch));
   if (c->d_scratch2) FBSE_HIP(c, hipFree(c->d_scratch2));
-  c->d_scratch = c->d_scratch2 = nullptr;
-  c->scratch_bytes = c->scratch2_bytes = 0;
+  if (c->d_gram) FBSE_HIP(c, hipFree(c->d_gram));
+  c->d_scratch = c->d_scratch2 = c->d_gram = nullptr;
+  c->scratch_bytes = c->scratch2_bytes = c->gram_bytes = 0;
   return FBSE_OK;
 }
 extern "C" long fbse_graph_replays(const fbse_ctx *c) { return c ? c->graph_replays : -1; }
@@ -1029,26 +1030,141 @@ extern "C" int fbse_filter(fbse_ctx *c, double *psi_rank, long n_starts, long ms
 // ---------------------------------------------------------------------------
 // Orthogonalisation and Rayleigh-Ritz (FP64, rocBLAS GEMMs + rocSOLVER)
 // ---------------------------------------------------------------------------
+__global__ void hermitize_kernel(double *H, long n, int cplx) {
+  // H <- (H + H^H)/2, column-major
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n * n; idx += (long)gridDim.x * blockDim.x) {
+    const long i = idx % n, j = idx / n;
+    if (i > j) continue;
+    if (cplx) {
+      double2 *A = (double2 *)H;
+      double2 a = A[i + j * n], b = A[j + i * n];
+      double2 r = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+      A[i + j * n] = r;
+      A[j + i * n] = make_double2(r.x, -r.y);
+    } else {
+      const double r = 0.5 * (H[i + j * n] + H[j + i * n]);
+      H[i + j * n] = r;
+      H[j + i * n] = r;
+    }
+  }
+}
+
+// C[i + j*ldc] = sum_p part[p][i + j*n1] (p ascending), doubles of cw per
+// element; tot = n1 * n2 * cw doubles per partial.
+__global__ void sum_parts_kernel(const double *part, long parts, long n1, long tot, int cw, double *C, long ldc) {
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < tot; idx += (long)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (long p = 0; p < parts; ++p) acc += part[p * tot + idx];
+    const long e = idx / cw, t = idx - e * cw;
+    const long i = e % n1, j = e / n1;
+    C[(i + j * ldc) * cw + t] = acc;
+  }
+}
+
+// Lower triangle of the column-major n x n matrix H from its upper triangle:
+// H[i, j] = conj(H[j, i]) for i > j.
+__global__ void mirror_upper_kernel(double *H, long n, int cplx) {
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n * n; idx += (long)gridDim.x * blockDim.x) {
+    const long i = idx % n, j = idx / n;
+    if (i <= j) continue;
+    if (cplx) {
+      const double2 u = ((const double2 *)H)[j + i * n];
+      ((double2 *)H)[i + j * n] = make_double2(u.x, -u.y);
+    } else {
+      H[i + j * n] = H[j + i * n];
+    }
+  }
+}
+
+// Split-K target of the Gram products: about two work-groups per CU, slabs of
+// at least this many rows.
+static const long kGramGroups = 512;
+static const long kGramMinRows = 4096;
+
 struct LinAlg {
   fbse_ctx *c;
   bool cplx;
   long m;   // rows (complex elements or reals)
   size_t es() const { return cplx ? 16 : 8; }
-  // C(n1 x n2) = A^H B   with A (m x n1), B (m x n2), lda = ldb = m
-  int gram(const double *A, long n1, const double *B, long n2, double *Cm, long ldc) {
+  int cw() const { return cplx ? 2 : 1; }  // doubles per element
+  // C(n1 x n2, ldc) = A^H B over k rows; A, B with leading dimension m
+  int gemm_h(const double *A, long n1, const double *B, long n2, long k, double *Cm, long ldc) {
     if (cplx) {
       const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
       FBSE_BLAS(c, rocblas_zgemm(c->blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, (int)n1,
-                                 (int)n2, (int)m, &one, (const rocblas_double_complex *)A, (int)m,
+                                 (int)n2, (int)k, &one, (const rocblas_double_complex *)A, (int)m,
                                  (const rocblas_double_complex *)B, (int)m, &zero, (rocblas_double_complex *)Cm,
                                  (int)ldc));
     } else {
       const double one = 1.0, zero = 0.0;
       FBSE_BLAS(c, rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, (int)n1, (int)n2,
-                                 (int)m, &one, A, (int)m, B, (int)m, &zero, Cm, (int)ldc));
+                                 (int)k, &one, A, (int)m, B, (int)m, &zero, Cm, (int)ldc));
     }
     return FBSE_OK;
   }
+  // C(n1 x n2) = A^H B   with A (m x n1), B (m x n2), lda = ldb = m.
+  // The library GEMM runs one work-group per 64 x 64 output tile over all m
+  // rows, so a product with few tiles leaves most CUs idle (a 32 x 32 Gram of
+  // 1.8M rows takes as long as a 1024 x 1024 one).  Such products are split
+  // into P slabs of rows (a strided-batched GEMM writes P partial Grams) and
+  // the partials are summed in a fixed order, so results stay deterministic.
+  int gram(const double *A, long n1, const double *B, long n2, double *Cm, long ldc) {
+    const long tiles = ((n1 + 63) / 64) * ((n2 + 63) / 64);
+    long P = 1;
+    if (tiles < kGramGroups) P = std::min((kGramGroups + tiles - 1) / tiles, m / kGramMinRows);
+    if (P <= 1) return gemm_h(A, n1, B, n2, m, Cm, ldc);
+    const long kc = m / P, rem = m - P * kc, parts = P + (rem > 0 ? 1 : 0);
+    const long blk = n1 * n2;  // elements of one partial Gram
+    int rc = grow(c, &c->d_gram, &c->gram_bytes, (size_t)parts * blk * es());
+    if (rc) return rc;
+    double *part = (double *)c->d_gram;
+    if (cplx) {
+      const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
+      FBSE_BLAS(c, rocblas_zgemm_strided_batched(
+                       c->blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, (int)n1, (int)n2,
+                       (int)kc, &one, (const rocblas_double_complex *)A, (int)m, (rocblas_stride)kc,
+                       (const rocblas_double_complex *)B, (int)m, (rocblas_stride)kc, &zero,
+                       (rocblas_double_complex *)part, (int)n1, (rocblas_stride)blk, (int)P));
+    } else {
+      const double one = 1.0, zero = 0.0;
+      FBSE_BLAS(c, rocblas_dgemm_strided_batched(c->blas, rocblas_operation_transpose, rocblas_operation_none, (int)n1,
+                                                 (int)n2, (int)kc, &one, A, (int)m, (rocblas_stride)kc, B, (int)m,
+                                                 (rocblas_stride)kc, &zero, part, (int)n1, (rocblas_stride)blk,
+                                                 (int)P));
+    }
+    if (rem > 0 && (rc = gemm_h(A + (size_t)P * kc * cw(), n1, B + (size_t)P * kc * cw(), n2, rem,
+                                part + (size_t)P * blk * cw(), n1)))
+      return rc;
+    const long tot = blk * cw();
+    hipLaunchKernelGGL(sum_parts_kernel, dim3((unsigned)std::min(2048L, (tot + 255) / 256)), dim3(256), 0, c->stream,
+                       (const double *)part, parts, n1, tot, cw(), Cm, ldc);
+    FBSE_HIP(c, hipGetLastError());
+    return FBSE_OK;
+  }
+  // G (n x n) = A^H B for a product known to be Hermitian (B = A, or B = H A).
+  // Only the blocks on and above the diagonal are formed (column block J of B
+  // against columns 0..end of J of A), then the lower triangle is mirrored:
+  // 9/16 of the full product's flops at n = 1024.
+  int gram_upper(const double *A, const double *B, long n, double *G) {
+    const long w = 128;
+    if (n < 4 * w) {
+      if (A == B) return gram(A, n, B, n, G, n);
+      int rc = gram(A, n, B, n, G, n);
+      if (rc) return rc;
+      hipLaunchKernelGGL(hermitize_kernel, dim3(256), dim3(256), 0, c->stream, G, n, cplx ? 1 : 0);
+      FBSE_HIP(c, hipGetLastError());
+      return FBSE_OK;
+    }
+    int rc;
+    for (long j0 = 0; j0 < n; j0 += w) {
+      const long nw = std::min(w, n - j0);
+      if ((rc = gram(A, j0 + nw, B + (size_t)j0 * m * cw(), nw, G + (size_t)j0 * n * cw(), n))) return rc;
+    }
+    hipLaunchKernelGGL(mirror_upper_kernel, dim3(256), dim3(256), 0, c->stream, G, n, cplx ? 1 : 0);
+    FBSE_HIP(c, hipGetLastError());
+    return FBSE_OK;
+  }
+  int gram_self(const double *A, long n, double *G) { return gram_upper(A, A, n, G); }
   // Out (m x nout) = A[:, 0:nin] * V (nin x nout, ldv)
   int product(const double *A, long nin, const double *V, long ldv, long nout, double *Out) {
     if (cplx) {
@@ -1127,25 +1243,6 @@ struct LinAlg {
   }
 };
 
-__global__ void hermitize_kernel(double *H, long n, int cplx) {
-  // H <- (H + H^H)/2, column-major
-  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n * n; idx += (long)gridDim.x * blockDim.x) {
-    const long i = idx % n, j = idx / n;
-    if (i > j) continue;
-    if (cplx) {
-      double2 *A = (double2 *)H;
-      double2 a = A[i + j * n], b = A[j + i * n];
-      double2 r = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
-      A[i + j * n] = r;
-      A[j + i * n] = make_double2(r.x, -r.y);
-    } else {
-      const double r = 0.5 * (H[i + j * n] + H[j + i * n]);
-      H[i + j * n] = r;
-      H[j + i * n] = r;
-    }
-  }
-}
-
 static int ortho_dev_impl(fbse_ctx *c, double *A, long n, double svd_eps, long *rank, double *sing) {
   LinAlg la{c, c->is_complex != 0, c->nsg};
   const size_t es = la.es();
@@ -1163,7 +1260,7 @@ static int ortho_dev_impl(fbse_ctx *c, double *A, long n, double svd_eps, long *
     hipFree(w);
   };
   // pass 1: G = A^H A -> V (columns of B = A V are orthogonal, |B_i| = s_i)
-  if ((rc = la.gram(A, n, A, n, G, n)) || (rc = la.eigh(G, n, w))) {
+  if ((rc = la.gram_self(A, n, G)) || (rc = la.eigh(G, n, w))) {
     cleanup();
     return rc;
   }
@@ -1188,7 +1285,7 @@ static int ortho_dev_impl(fbse_ctx *c, double *A, long n, double svd_eps, long *
       return set_err(c, FBSE_ERR_NOMEM, "ortho: %ld x %ld block of small singular directions", la.m, ns);
     }
     rc = la.product(A, n, G, n, ns, Bs);
-    if (!rc) rc = la.gram(Bs, ns, Bs, ns, Z, ns);
+    if (!rc) rc = la.gram_self(Bs, ns, Z);
     hipStreamSynchronize(c->stream);
     hipFree(Bs);
     if (rc) {
@@ -1259,7 +1356,7 @@ static int ortho_dev_impl(fbse_ctx *c, double *A, long n, double svd_eps, long *
     }
   }
   // pass 3: Cholesky re-orthogonalisation U <- U R^-1 with U^H U = R^H R
-  if ((rc = la.gram(A, k, A, k, G, k))) {
+  if ((rc = la.gram_self(A, k, G))) {
     cleanup();
     return rc;
   }
@@ -1338,15 +1435,19 @@ static int rr_dev_impl(fbse_ctx *c, double *U, long n, double *eval) {
   FBSE_HIP(c, hipMalloc(&Hs, (size_t)n * n * es));
   FBSE_HIP(c, hipMalloc(&w, (size_t)n * 8));
   FBSE_HIP(c, hipMalloc(&HU, (size_t)ch * sl * 8));
-  for (long c0 = 0; c0 < n && !rc; c0 += ch) {
-    const long nc = std::min(ch, n - c0);
-    rc = apply_h_dev_impl(c, U + c0 * sl, HU, nc);
-    if (!rc) rc = la.gram(U, n, HU, nc, (double *)((char *)Hs + (size_t)c0 * n * es), n);
+  if (ch == n) {
+    // all of H U at once: U^H (H U) is Hermitian, its upper blocks suffice
+    rc = apply_h_dev_impl(c, U, HU, n);
+    if (!rc) rc = la.gram_upper(U, HU, n, Hs);
+  } else {
+    for (long c0 = 0; c0 < n && !rc; c0 += ch) {
+      const long nc = std::min(ch, n - c0);
+      rc = apply_h_dev_impl(c, U + c0 * sl, HU, nc);
+      if (!rc) rc = la.gram(U, n, HU, nc, (double *)((char *)Hs + (size_t)c0 * n * es), n);
+    }
+    if (!rc) hipLaunchKernelGGL(hermitize_kernel, dim3(256), dim3(256), 0, c->stream, Hs, n, la.cplx ? 1 : 0);
   }
-  if (!rc) {
-    hipLaunchKernelGGL(hermitize_kernel, dim3(256), dim3(256), 0, c->stream, Hs, n, la.cplx ? 1 : 0);
-    rc = la.eigh(Hs, n, w);
-  }
+  if (!rc) rc = la.eigh(Hs, n, w);
   if (!rc) rc = la.rotate(U, n, Hs, n, n);
   if (!rc) {
     std::vector<double> hw(n);

@@ -150,6 +150,8 @@ struct fbse_ctx {
   size_t scratch_bytes = 0;
   void *d_scratch2 = nullptr;
   size_t scratch2_bytes = 0;
+  void *d_gram = nullptr;       // partial Grams of the split-K products
+  size_t gram_bytes = 0;
   double *d_red = nullptr;      // reductions
   size_t red_bytes = 0;
   double *h_red = nullptr;      // pinned host mirror
@@ -163,6 +165,7 @@ struct fbse_ctx {
   // 0 = not tuned yet.
   int tune_lines[2][2][3][3] = {};
   int tile_variant[2] = {-1, -1};  // plane/slab kernel instance per precision (-1 = not tuned)
+  int n_cu = 0;                 // compute units (resident-grid launches), 0 until queried
 
   // Newton-window graphs: the Newton shift a0 of step (base + off) is read
   // from a device table while a window is captured (dyn_a0_tab != nullptr)
