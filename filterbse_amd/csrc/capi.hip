@@ -241,35 +241,52 @@ static int build_projectors(fbse_ctx *c, const fbse_proj_desc *pd) {
     F.color_off = P.color_off;
   }
   c->has_proj = (pd->so || pd->nl) && npts > 0;
-  // Projector entries by grid point (counting sort, atoms in index order): the
-  // plane passes add the expansion of every atom whose sphere holds a point
-  // while that point is in LDS.
+  // Projector entries by xy plane and rank: the combine pass adds, in rounds
+  // k = 0, 1, ..., the contribution of every point's k-th atom (atoms in
+  // index order), so no two additions of one round meet at a point and the
+  // summation order per point is fixed.
   if (c->has_proj) {
-    // per entry, in entry order: proj, nl_proj (first radial projector) and
-    // the three Y_1m, so that the plane pass reads each entry contiguously
-    std::vector<int> poff(c->ngrid + 1, 0), pat(npts);
-    std::vector<double> pe((size_t)npts * 8);
-    for (long i = 0; i < npts; ++i) ++poff[jx[i] + 1];
-    for (long g = 0; g < c->ngrid; ++g) poff[g + 1] += poff[g];
-    std::vector<int> fillp(poff.begin(), poff.end() - 1);
+    const long nxy = c->nx * c->ny;
+    std::vector<int> rank(npts), pcnt(c->ngrid, 0);
     for (long a = 0; a < na; ++a)
       for (int k = 0; k < cnt[a]; ++k) {
         const int idx = off[a] + k;
-        const int slot = fillp[jx[idx]]++;
-        pat[slot] = (int)a;
-        double *e = &pe[(size_t)slot * 8];
-        e[0] = (pd->so && pd->n_so_proj > 0) ? proj[idx] : 0.0;
-        e[1] = (pd->nl && pd->n_nl_proj > 0) ? nlp[idx] : 0.0;
-        for (int m = 0; m < 3; ++m) {
-          e[2 + 2 * m] = y[2 * (m * npts + idx)];
-          e[3 + 2 * m] = y[2 * (m * npts + idx) + 1];
-        }
+        rank[idx] = pcnt[jx[idx]]++;
       }
+    int nr = 0;
+    for (long g = 0; g < c->ngrid; ++g) nr = std::max(nr, pcnt[g]);
+    c->pt_ranks = nr;
+    // bucket (z, rank), then point order inside a bucket
+    std::vector<int> roff((size_t)c->nz * nr + 1, 0);
+    for (long i = 0; i < npts; ++i) ++roff[(size_t)(jx[i] / nxy) * nr + rank[i] + 1];
+    for (size_t b = 0; b + 1 < roff.size(); ++b) roff[b + 1] += roff[b];
+    std::vector<std::pair<long, int>> keyed(npts);  // (bucket * ngrid + point, entry)
+    for (long i = 0; i < npts; ++i)
+      keyed[i] = {((long)(jx[i] / nxy) * nr + rank[i]) * c->ngrid + jx[i], (int)i};
+    std::sort(keyed.begin(), keyed.end());
+    std::vector<int> atom_of(npts);
+    for (long a = 0; a < na; ++a)
+      for (int k = 0; k < cnt[a]; ++k) atom_of[off[a] + k] = (int)a;
+    std::vector<int> pa((size_t)npts * 2);
+    std::vector<double> pe((size_t)npts * 8);
+    for (long e = 0; e < npts; ++e) {
+      const int idx = keyed[e].second;
+      const long g = jx[idx], px = g % c->nx, py = (g / c->nx) % c->ny;
+      pa[2 * e] = (int)((py << 16) | px);
+      pa[2 * e + 1] = atom_of[idx];
+      double *v = &pe[(size_t)e * 8];
+      v[0] = (pd->so && pd->n_so_proj > 0) ? proj[idx] : 0.0;
+      v[1] = (pd->nl && pd->n_nl_proj > 0) ? nlp[idx] : 0.0;
+      for (int m = 0; m < 3; ++m) {
+        v[2 + 2 * m] = y[2 * (m * npts + idx)];
+        v[3 + 2 * m] = y[2 * (m * npts + idx) + 1];
+      }
+    }
     auto pef = cast_vec<float>(pe.data(), (long)pe.size());
-    if ((rc = upload(c, (void **)&c->d_pt_off, poff.data(), poff.size() * sizeof(int))) ||
-        (rc = upload(c, (void **)&c->d_pt_atom, pat.data(), npts * sizeof(int))) ||
-        (rc = upload(c, (void **)&c->d_pt_val, pe.data(), pe.size() * 8)) ||
-        (rc = upload(c, (void **)&c->d_pt_val_f, pef.data(), pef.size() * 4)))
+    if ((rc = upload(c, (void **)&c->d_rec_off, roff.data(), roff.size() * sizeof(int))) ||
+        (rc = upload(c, (void **)&c->d_rec_pa, pa.data(), pa.size() * sizeof(int))) ||
+        (rc = upload(c, (void **)&c->d_rec_val, pe.data(), pe.size() * 8)) ||
+        (rc = upload(c, (void **)&c->d_rec_val_f, pef.data(), pef.size() * 4)))
       return rc;
   }
   return FBSE_OK;
@@ -400,8 +417,8 @@ extern "C" void fbse_destroy(fbse_ctx *c) {
   void *ptrs[] = {c->d_vloc, c->d_vloc_f, c->d_kx, c->d_ky, c->d_kz, c->d_ksqr, c->d_kx_f, c->d_ky_f, c->d_kz_f,
                   c->d_ksqr_f, c->d_scratch, c->d_scratch2, c->d_red, c->pj.a_off, c->pj.a_cnt, c->pj.jxyz,
                   c->pj.sign, c->pj.proj, c->pj.nlproj, c->pj.y1, c->pj.ls, c->pj_f.proj, c->pj_f.nlproj,
-                  c->pj_f.y1, c->pj_f.ls, c->pj.color_atoms, c->d_step, c->d_pt_off, c->d_pt_atom,
-                  c->d_pt_val, c->d_pt_val_f, c->d_pjd, c->d_gram};
+                  c->pj_f.y1, c->pj_f.ls, c->pj.color_atoms, c->d_step, c->d_rec_off, c->d_rec_pa,
+                  c->d_rec_val, c->d_rec_val_f, c->d_pjd, c->d_gram};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (int a = 0; a < 3; ++a) {

@@ -134,12 +134,17 @@ struct fbse_ctx {
   fbse::ProjTables<double> pj;
   fbse::ProjTables<float> pj_f;
   int has_proj = 0;
-  // projector entries by grid point (spin-0 index): entries of point g are
-  // pt_off[g] .. pt_off[g+1]-1 with atom pt_atom[e] and values pt_val[8e ..]:
-  // proj, nl_proj, Re/Im Y_1m (m = 0..2), FP64 and FP32 copies
-  int *d_pt_off = nullptr, *d_pt_atom = nullptr;
-  double *d_pt_val = nullptr;
-  float *d_pt_val_f = nullptr;
+  // projector entries of the xy planes for the plane passes: the entries of
+  // plane z with rank k (the k-th atom, in atom order, whose sphere holds the
+  // point) are records rec_off[z*pt_ranks + k] .. rec_off[z*pt_ranks + k + 1]-1,
+  // ordered by point; record e holds rec_pa[2e] = (y << 16) | x and
+  // rec_pa[2e+1] = atom, values rec_val[8e ..] = proj, nl_proj, Re/Im Y_1m
+  // (m = 0..2), FP64 and FP32 copies.  Within one rank no two records share a
+  // point.
+  int *d_rec_off = nullptr, *d_rec_pa = nullptr;
+  double *d_rec_val = nullptr;
+  float *d_rec_val_f = nullptr;
+  int pt_ranks = 0;
   void *d_pjd = nullptr;        // mixed projection coefficients (column x atom x 6), grown on demand
   size_t pjd_bytes = 0;
   int fuse_proj = 1;            // expansion inside the plane passes when the tables allow it

@@ -30,7 +30,6 @@ enum TileKind { kTileFwdXY = 0, kTileBwdXY = 1, kTileBwdXYCombine = 2, kTileKinZ
 // A thread stages ceil(E / R) butterflies of radix R per pass: a work-group
 // of NT threads covers tiles of up to NT * E elements.  E is a compile-time
 // kernel parameter; the autotuner picks among the instantiated values.
-constexpr int kTileLoadBatch = 8;  // global loads in flight per thread
 
 template <typename T> struct TileArgs {
   const cplx_t<T> *src;
@@ -58,34 +57,30 @@ template <typename T> struct TileArgs {
   const double *a0_tab;
   const long *step_base;
   int step_off;
-  // projector expansion in the combine pass (pt_off != nullptr): the entries
-  // of spin-grid point g are pt_off[g] .. pt_off[g+1]-1, atom pt_atom[e],
-  // values pt_val[8e ..] = proj, nl_proj, Re/Im Y_1m; coefficients of this
-  // work-group's column pjd[(item*pj_natoms + atom)*6 + r]
-  const int *pt_off, *pt_atom;
-  const T *pt_val;
+  // projector expansion in the combine pass (rec_off != nullptr): in rounds
+  // k < pt_ranks, records rec_off[z*pt_ranks + k] .. rec_off[z*pt_ranks + k + 1]-1
+  // of plane z, record e at point (y, x) = (rec_pa[2e] >> 16, rec_pa[2e] & 0xffff)
+  // of atom rec_pa[2e+1] with values rec_val[8e ..] = proj, nl_proj, Re/Im Y_1m;
+  // coefficients of this work-group's column pjd[(item*pj_natoms + atom)*6 + r]
+  const int *rec_off, *rec_pa;
+  const T *rec_val;
   const cplx_t<T> *pjd;
-  int pj_natoms;
+  int pj_natoms, pt_ranks;
 };
 
-// sum over the atoms whose projector sphere holds spin-grid point g of
-// Y_m(g) (proj(g) d_so[m] + nl_proj(g) d_nl[m]) (phamiltonian.c:140-444)
+// contribution of one projector record to its point:
+// Y_m (proj d_so[m] + nl_proj d_nl[m]) summed over m (phamiltonian.c:140-444)
 template <typename T>
-__device__ __forceinline__ cplx_t<T> proj_expansion(const TileArgs<T> &A, long g, const cplx_t<T> *dcol) {
+__device__ __forceinline__ cplx_t<T> record_term(const T *v, const cplx_t<T> *dd) {
   cplx_t<T> acc{(T)0, (T)0};
-  const int e1 = A.pt_off[g + 1];
-  for (int e = A.pt_off[g]; e < e1; ++e) {
-    const cplx_t<T> *dd = dcol + (long)A.pt_atom[e] * 6;
-    const T *v = A.pt_val + (long)e * 8;
-    const T wso = v[0], wnl = v[1];
+  const T wso = v[0], wnl = v[1];
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      const cplx_t<T> a = dd[m], b = dd[3 + m];
-      const T sx = wso * a.x + wnl * b.x, sy = wso * a.y + wnl * b.y;
-      const T yx = v[2 + 2 * m], yy = v[3 + 2 * m];
-      acc.x += yx * sx - yy * sy;
-      acc.y += yx * sy + yy * sx;
-    }
+  for (int m = 0; m < 3; ++m) {
+    const cplx_t<T> a = dd[m], b = dd[3 + m];
+    const T sx = wso * a.x + wnl * b.x, sy = wso * a.y + wnl * b.y;
+    const T yx = v[2 + 2 * m], yy = v[3 + 2 * m];
+    acc.x += yx * sx - yy * sy;
+    acc.y += yx * sy + yy * sx;
   }
   return acc;
 }
@@ -171,7 +166,6 @@ __device__ __forceinline__ void tile_fft(cplx_t<T> *buf, const cplx_t<T> *tw, in
 template <typename T, int KIND, int E>
 __device__ __forceinline__ void tile_body(const TileArgs<T> &A) {
   using C = cplx_t<T>;
-  constexpr int NB = kTileLoadBatch;
   extern __shared__ unsigned char smem_raw[];
   const int nc = A.nc, nr = A.nr, ls = A.ls;
   const int NT = blockDim.x;
@@ -192,11 +186,13 @@ __device__ __forceinline__ void tile_body(const TileArgs<T> &A) {
   const int c0 = threadIdx.x % nc, r0 = threadIdx.x / nc, dr = NT / nc;
   const cplx_t<T> *src = A.src + base + c0;
 
-  for (int rb = r0; rb < nr; rb += dr * NB) {
-    T vr[NB], vi[NB];
+  // all of this thread's rows in flight at once (at most E: the launch has
+  // blockDim.x * E >= nr * nc)
+  {
+    T vr[E], vi[E];
 #pragma unroll
-    for (int u = 0; u < NB; ++u) {
-      const int r = rb + u * dr;
+    for (int u = 0; u < E; ++u) {
+      const int r = r0 + u * dr;
       if (r < nr) {
         const C t = src[r * A.row_stride];
         vr[u] = t.x;
@@ -204,10 +200,19 @@ __device__ __forceinline__ void tile_body(const TileArgs<T> &A) {
       }
     }
 #pragma unroll
-    for (int u = 0; u < NB; ++u) {
-      const int r = rb + u * dr;
+    for (int u = 0; u < E; ++u) {
+      const int r = r0 + u * dr;
       if (r < nr) buf[r * ls + c0] = C{vr[u], vi[u]};
     }
+  }
+  // z passes with a separable k^2: the x and z tables sit in LDS after the
+  // twiddles, so the scaling step needs no global load inside the transform
+  T *kxl = reinterpret_cast<T *>(twr + nr);
+  T *kzl = kxl + nc;
+  const bool ksep = (KIND == kTileKinZ || KIND == kTilePrecZ) && !A.ksqr;
+  if (ksep) {
+    for (int t = threadIdx.x; t < nc; t += NT) kxl[t] = A.kx[t];
+    for (int t = threadIdx.x; t < nr; t += NT) kzl[t] = A.kz[t];
   }
   __syncthreads();
 
@@ -219,19 +224,15 @@ __device__ __forceinline__ void tile_body(const TileArgs<T> &A) {
     tile_fft<T, E>(buf, twc, nc, A.nfc, A.fc, 1, ls, nr, true);
   } else {  // z columns: forward, scale by k^2 (or the preconditioner), backward
     tile_fft<T, E>(buf, twr, nr, A.nfr, A.fr, ls, 1, nc, false);
-    T kxy = 0;
-    const T *ksq = nullptr;
-    if (A.ksqr)
-      ksq = A.ksqr + c0 + (long)w * nc;
-    else
-      kxy = A.kx[c0] + A.ky[w];
+    const T kxy = ksep ? kxl[c0] + A.ky[w] : (T)0;
+    const T *ksq = ksep ? nullptr : A.ksqr + c0 + (long)w * nc;
     for (int r = r0; r < nr; r += dr) {
       T k2;
-      if (ksq) {
-        k2 = ksq[(long)r * A.nxny];
-      } else {
-        k2 = kxy + A.kz[r];
+      if (ksep) {
+        k2 = kxy + kzl[r];
         if (k2 > A.kcap) k2 = A.kcap;
+      } else {
+        k2 = ksq[(long)r * A.nxny];
       }
       if (KIND == kTilePrecZ) k2 = A.pa / ((T)1 + k2 * A.pb);
       C v = buf[r * ls + c0];
@@ -250,7 +251,8 @@ __device__ __forceinline__ void tile_body(const TileArgs<T> &A) {
     const long vofs = (item % A.nspin) * A.item_stride + (long)w * A.tile_stride + c0;
     const T *vl = A.vloc + vofs;
     const cplx_t<T> *pin = A.psi_in + base + c0;
-    const long gsp0 = (long)w * A.tile_stride + c0;  // spin-grid index of (row 0, c0)
+    const bool expand = A.rec_off != nullptr;
+    constexpr int NB = 8;  // rows of psi_in and V in flight per thread
     for (int rb = r0; rb < nr; rb += dr * NB) {
       T pr[NB], pim[NB], vv[NB];
 #pragma unroll
@@ -271,14 +273,46 @@ __device__ __forceinline__ void tile_body(const TileArgs<T> &A) {
           C o;
           o.x = A.a1 * (t.x + vv[u] * pr[u]) + a0 * pr[u];
           o.y = A.a1 * (t.y + vv[u] * pim[u]) + a0 * pim[u];
-          if (A.pt_off) {
-            const C pe = proj_expansion<T>(A, gsp0 + r * A.row_stride, A.pjd + item * A.pj_natoms * 6);
-            o.x += pe.x;
-            o.y += pe.y;
-          }
-          dst[r * A.row_stride] = o;
+          if (expand)
+            buf[r * ls + c0] = o;
+          else
+            dst[r * A.row_stride] = o;
         }
       }
+    }
+    if (expand) {
+      // projector expansion, one round per rank: the records of a round
+      // touch distinct points, so they add in place without atomics, and
+      // every point sums its atoms in index order
+      __syncthreads();
+      const cplx_t<T> *dcol = A.pjd + item * A.pj_natoms * 6;
+      const int2 *pa = reinterpret_cast<const int2 *>(A.rec_pa);
+      for (int k = 0; k < A.pt_ranks; ++k) {
+        const int e0 = A.rec_off[w * A.pt_ranks + k], e1 = A.rec_off[w * A.pt_ranks + k + 1];
+        for (int e = e0 + threadIdx.x; e < e1; e += 2 * NT) {
+          // two records per iteration, loads of both in flight together
+          const int eb = e + NT;
+          const bool two = eb < e1;
+          const int2 qa = pa[e];
+          const int2 qb = two ? pa[eb] : qa;
+          const C ta = record_term<T>(A.rec_val + (long)e * 8, dcol + (long)qa.y * 6);
+          const C tb = record_term<T>(A.rec_val + (long)(two ? eb : e) * 8, dcol + (long)qb.y * 6);
+          const int pos_a = (qa.x >> 16) * ls + (qa.x & 0xffff);
+          C o = buf[pos_a];
+          o.x += ta.x;
+          o.y += ta.y;
+          buf[pos_a] = o;
+          if (two) {
+            const int pos_b = (qb.x >> 16) * ls + (qb.x & 0xffff);
+            C q = buf[pos_b];
+            q.x += tb.x;
+            q.y += tb.y;
+            buf[pos_b] = q;
+          }
+        }
+        __syncthreads();
+      }
+      for (int r = r0; r < nr; r += dr) dst[r * A.row_stride] = buf[r * ls + c0];
     }
   } else {
     for (int r = r0; r < nr; r += dr) dst[r * A.row_stride] = buf[r * ls + c0];
@@ -356,8 +390,9 @@ static bool radices_ok(const AxisPlan &pl) {
 // hit distinct banks.
 static int tile_pitch(int nc) { return nc | 1; }
 
+// tile, twiddles of both axes, then the real k^2 tables of x and z
 template <typename T> static size_t tile_lds_bytes(int nr, int nc) {
-  return ((size_t)nr * tile_pitch(nc) + nc + nr) * sizeof(cplx_t<T>);
+  return ((size_t)nr * tile_pitch(nc) + nc + nr) * sizeof(cplx_t<T>) + (size_t)(nc + nr) * sizeof(T);
 }
 
 static int gcd_int(int a, int b) { return b ? gcd_int(b, a % b) : a; }
@@ -461,10 +496,11 @@ static int tile_pass_v(fbse_ctx *c, int v, int kind, const cplx_t<T> *src, cplx_
     A.step_off = c->dyn_step_off;
   }
   if (kind == kTileBwdXYCombine && pjd) {
-    A.pt_off = c->d_pt_off;
-    A.pt_atom = c->d_pt_atom;
-    if constexpr (sizeof(T) == 8) A.pt_val = c->d_pt_val;
-    else A.pt_val = c->d_pt_val_f;
+    A.rec_off = c->d_rec_off;
+    A.rec_pa = c->d_rec_pa;
+    if constexpr (sizeof(T) == 8) A.rec_val = c->d_rec_val;
+    else A.rec_val = c->d_rec_val_f;
+    A.pt_ranks = c->pt_ranks;
     A.pjd = pjd;
     A.pj_natoms = c->pj.n_atoms;
   }

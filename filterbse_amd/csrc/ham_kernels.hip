@@ -255,7 +255,7 @@ int proj_apply(fbse_ctx *c, const cplx_t<T> *psi_in, cplx_t<T> *psi_out, long ns
   return FBSE_OK;
 }
 bool proj_fusable(const fbse_ctx *c) {
-  if (!c->has_proj || !c->fuse_proj || !c->d_pt_off) return false;
+  if (!c->has_proj || !c->fuse_proj || !c->d_rec_off) return false;
   const int nso = c->pj.so ? c->pj.n_so : 0, nnl = c->pj.nl ? c->pj.n_nl : 0;
   return nso <= 1 && nnl <= 1;
 }
