@@ -36,9 +36,13 @@ template <typename C, typename T> __device__ __forceinline__ C cmul_i(C a, T s) 
 }
 
 // Butterfly of radix R on values x[0..R-1] (already twiddled); sgn = -1 forward.
+// Output q is left in x[perm(q)] (the identity for the prime-power radices).
 template <typename T, int R> struct Bfly;
+struct NaturalOrder {
+  __host__ __device__ static constexpr int perm(int q) { return q; }
+};
 
-template <typename T> struct Bfly<T, 2> {
+template <typename T> struct Bfly<T, 2> : NaturalOrder {
   using C = cplx_t<T>;
   __device__ __forceinline__ static void run(C *x, T) {
     C a = x[0], b = x[1];
@@ -46,7 +50,7 @@ template <typename T> struct Bfly<T, 2> {
     x[1] = csub(a, b);
   }
 };
-template <typename T> struct Bfly<T, 3> {
+template <typename T> struct Bfly<T, 3> : NaturalOrder {
   using C = cplx_t<T>;
   __device__ __forceinline__ static void run(C *x, T sgn) {
     const T h = (T)0.86602540378443864676;  // sqrt(3)/2
@@ -61,7 +65,7 @@ template <typename T> struct Bfly<T, 3> {
     x[2] = csub(t2, t3);
   }
 };
-template <typename T> struct Bfly<T, 4> {
+template <typename T> struct Bfly<T, 4> : NaturalOrder {
   using C = cplx_t<T>;
   __device__ __forceinline__ static void run(C *x, T sgn) {
     C a = cadd(x[0], x[2]), b = csub(x[0], x[2]);
@@ -73,7 +77,7 @@ template <typename T> struct Bfly<T, 4> {
     x[3] = csub(b, di);
   }
 };
-template <typename T> struct Bfly<T, 5> {
+template <typename T> struct Bfly<T, 5> : NaturalOrder {
   using C = cplx_t<T>;
   __device__ __forceinline__ static void run(C *x, T sgn) {
     const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
@@ -97,7 +101,7 @@ template <typename T> struct Bfly<T, 5> {
     x[3] = csub(a2, b2);
   }
 };
-template <typename T> struct Bfly<T, 8> {
+template <typename T> struct Bfly<T, 8> : NaturalOrder {
   using C = cplx_t<T>;
   __device__ __forceinline__ static void run(C *x, T sgn) {
     C e[4] = {x[0], x[2], x[4], x[6]};
@@ -122,5 +126,71 @@ template <typename T> struct Bfly<T, 8> {
     x[7] = csub(e[3], w3o);
   }
 };
+
+// Composite butterflies R = P * Q (one register pass instead of two LDS
+// passes): x[Q n1 + n2] -> DFT_P over n1 -> twiddle w_R^(n2 k1) -> DFT_Q over
+// n2 -> X[k1 + P k2].  RootTab<N> holds cos and sin of 2 pi j / N, j < N.
+template <int N> struct RootTab;
+template <> struct RootTab<6> {
+  static constexpr double c[6] = {1.0, 0.5, -0.5, -1.0, -0.5, 0.5};
+  static constexpr double s[6] = {0.0, 0.866025403784438646764, 0.866025403784438646764, 0.0, -0.866025403784438646764, -0.866025403784438646764};
+};
+template <> struct RootTab<9> {
+  static constexpr double c[9] = {1.0, 0.766044443118978035202, 0.173648177666930348852, -0.5, -0.939692620785908384054, -0.939692620785908384054, -0.5, 0.173648177666930348852, 0.766044443118978035202};
+  static constexpr double s[9] = {0.0, 0.642787609686539326323, 0.984807753012208059367, 0.866025403784438646764, 0.342020143325668733044, -0.342020143325668733044, -0.866025403784438646764, -0.984807753012208059367, -0.642787609686539326323};
+};
+template <> struct RootTab<10> {
+  static constexpr double c[10] = {1.0, 0.809016994374947424102, 0.309016994374947424102, -0.309016994374947424102, -0.809016994374947424102, -1.0, -0.809016994374947424102, -0.309016994374947424102, 0.309016994374947424102, 0.809016994374947424102};
+  static constexpr double s[10] = {0.0, 0.587785252292473129169, 0.951056516295153572116, 0.951056516295153572116, 0.587785252292473129169, 0.0, -0.587785252292473129169, -0.951056516295153572116, -0.951056516295153572116, -0.587785252292473129169};
+};
+template <> struct RootTab<12> {
+  static constexpr double c[12] = {1.0, 0.866025403784438646764, 0.5, 0.0, -0.5, -0.866025403784438646764, -1.0, -0.866025403784438646764, -0.5, 0.0, 0.5, 0.866025403784438646764};
+  static constexpr double s[12] = {0.0, 0.5, 0.866025403784438646764, 1.0, 0.866025403784438646764, 0.5, 0.0, -0.5, -0.866025403784438646764, -1.0, -0.866025403784438646764, -0.5};
+};
+template <> struct RootTab<15> {
+  static constexpr double c[15] = {1.0, 0.913545457642600895502, 0.669130606358858213826, 0.309016994374947424102, -0.1045284632676534714, -0.5, -0.809016994374947424102, -0.978147600733805637929, -0.978147600733805637929, -0.809016994374947424102, -0.5, -0.1045284632676534714, 0.309016994374947424102, 0.669130606358858213826, 0.913545457642600895502};
+  static constexpr double s[15] = {0.0, 0.406736643075800207754, 0.743144825477394235015, 0.951056516295153572116, 0.994521895368273336923, 0.866025403784438646764, 0.587785252292473129169, 0.207911690817759337102, -0.207911690817759337102, -0.587785252292473129169, -0.866025403784438646764, -0.994521895368273336923, -0.951056516295153572116, -0.743144825477394235015, -0.406736643075800207754};
+};
+template <> struct RootTab<16> {
+  static constexpr double c[16] = {1.0, 0.923879532511286756128, 0.707106781186547524401, 0.382683432365089771728, 0.0, -0.382683432365089771728, -0.707106781186547524401, -0.923879532511286756128, -1.0, -0.923879532511286756128, -0.707106781186547524401, -0.382683432365089771728, 0.0, 0.382683432365089771728, 0.707106781186547524401, 0.923879532511286756128};
+  static constexpr double s[16] = {0.0, 0.382683432365089771728, 0.707106781186547524401, 0.923879532511286756128, 1.0, 0.923879532511286756128, 0.707106781186547524401, 0.382683432365089771728, 0.0, -0.382683432365089771728, -0.707106781186547524401, -0.923879532511286756128, -1.0, -0.923879532511286756128, -0.707106781186547524401, -0.382683432365089771728};
+};
+
+template <typename T, int P, int Q> struct BflyPQ {
+  using C = cplx_t<T>;
+  // In place, output permuted: X[k1 + P k2] ends in x[perm(k1 + P k2)] =
+  // x[Q k1 + k2] (compile-time indices, so the permutation costs nothing).
+  __host__ __device__ static constexpr int perm(int q) { return Q * (q % P) + q / P; }
+  __device__ __forceinline__ static void run(C *x, T sgn) {
+    constexpr int N = P * Q;
+#pragma unroll
+    for (int n2 = 0; n2 < Q; ++n2) {
+      C t[P];
+#pragma unroll
+      for (int n1 = 0; n1 < P; ++n1) t[n1] = x[Q * n1 + n2];
+      Bfly<T, P>::run(t, sgn);
+#pragma unroll
+      for (int k1 = 0; k1 < P; ++k1) {
+        C v = t[k1];
+        if (n2 > 0 && k1 > 0) {
+          const int j = (n2 * k1) % N;
+          C w;
+          w.x = (T)RootTab<N>::c[j];
+          w.y = sgn * (T)RootTab<N>::s[j];
+          v = cmul(v, w);
+        }
+        x[Q * k1 + n2] = v;  // y[n2][k1]
+      }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) Bfly<T, Q>::run(x + Q * k1, sgn);
+  }
+};
+template <typename T> struct Bfly<T, 6> : BflyPQ<T, 2, 3> {};
+template <typename T> struct Bfly<T, 9> : BflyPQ<T, 3, 3> {};
+template <typename T> struct Bfly<T, 10> : BflyPQ<T, 2, 5> {};
+template <typename T> struct Bfly<T, 12> : BflyPQ<T, 4, 3> {};
+template <typename T> struct Bfly<T, 15> : BflyPQ<T, 3, 5> {};
+template <typename T> struct Bfly<T, 16> : BflyPQ<T, 4, 4> {};
 
 }  // namespace fbse

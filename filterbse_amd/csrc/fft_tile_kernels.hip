@@ -22,6 +22,7 @@
 #include "fft_common.h"
 
 #include <algorithm>
+#include <vector>
 
 namespace fbse {
 
@@ -29,7 +30,10 @@ enum TileKind { kTileFwdXY = 0, kTileBwdXY = 1, kTileBwdXYCombine = 2, kTileKinZ
 
 // A thread stages ceil(E / R) butterflies of radix R per pass: a work-group
 // of NT threads covers tiles of up to NT * E elements.  E is a compile-time
-// kernel parameter; the autotuner picks among the instantiated values.
+// kernel parameter; the autotuner picks among the instantiated values.  An
+// instance runs radix R only when its staged values stay within E + 4, which
+// bounds the registers a pass holds across its barrier.
+constexpr bool tile_radix_fits(int e, int r) { return (e + r - 1) / r * r <= e + 4; }
 
 template <typename T> struct TileArgs {
   const cplx_t<T> *src;
@@ -37,7 +41,7 @@ template <typename T> struct TileArgs {
   const cplx_t<T> *twc, *twr;  // twiddles of the row (length nc) and column (length nr) transforms
   int nc, nr, ls;              // tile columns, rows, LDS row stride (elements)
   int nfc, nfr;
-  unsigned long long fc, fr;   // radix schedules, 4 bits per factor, first factor lowest
+  unsigned long long fc, fr;   // radix schedules, 4 bits (radix - 1) per factor, first factor lowest
   long row_stride;             // global elements between tile rows
   long tile_stride;            // global elements between tiles of one batch item
   long item_stride;            // global elements between batch items (ngrid)
@@ -101,8 +105,11 @@ __device__ __forceinline__ int mod_small(int i, int p, float inv_p) {
 // threads are on consecutive lines: for column transforms (lstride 1) a
 // wavefront touches consecutive LDS words, for row transforms (lstride = odd
 // row pitch) words 2*pitch apart, which spread over all banks.
+// Each radix pass is a separate (not inlined) function: one kernel holding
+// every radix's inlined pass spills registers, while separate functions keep
+// their own allocation at the cost of a call per pass.
 template <typename T, int R, int KB>
-__device__ __forceinline__ void tile_radix_pass(cplx_t<T> *buf, const cplx_t<T> *tw, int n, int p, int estride,
+__device__ __attribute__((noinline)) void tile_radix_pass(cplx_t<T> *buf, const cplx_t<T> *tw, int n, int p, int estride,
                                                 int lstride, int l0, int i0, int di, bool inv) {
   using C = cplx_t<T>;
   const int m = n / R;
@@ -120,14 +127,29 @@ __device__ __forceinline__ void tile_radix_pass(cplx_t<T> *buf, const cplx_t<T> 
       const int kk = mod_small(i, p, inv_p);
       const C *src = buf + lbase + i * estride;
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        C v = src[r * m * estride];
-        if (r > 0 && kk > 0) {
-          C w = tw[r * kk * tstep];
-          if (inv) w = cconj(w);
-          v = cmul(v, w);
+      for (int r = 0; r < R; ++r) x[k][r] = src[r * m * estride];
+      if (kk > 0) {
+        // twiddles w^r, w = tw[kk*tstep]: w and w^4 are read, w^(4q + j) =
+        // w^(4q) w^j with w^(4q) a product of reads (at most three roundings)
+        C w1 = tw[kk * tstep];
+        if (inv) w1 = cconj(w1);
+        const C w2 = cmul(w1, w1), w3 = cmul(w2, w1);
+        x[k][1] = cmul(x[k][1], w1);
+        if (R > 2) x[k][2] = cmul(x[k][2], w2);
+        if (R > 3) x[k][3] = cmul(x[k][3], w3);
+        if (R > 4) {
+          C w4 = tw[4 * kk * tstep];
+          if (inv) w4 = cconj(w4);
+          C base = w4;
+#pragma unroll
+          for (int q = 1; 4 * q < R; ++q) {
+            if (q > 1) base = cmul(base, w4);
+            x[k][4 * q] = cmul(x[k][4 * q], base);
+            if (4 * q + 1 < R) x[k][4 * q + 1] = cmul(x[k][4 * q + 1], cmul(base, w1));
+            if (4 * q + 2 < R) x[k][4 * q + 2] = cmul(x[k][4 * q + 2], cmul(base, w2));
+            if (4 * q + 3 < R) x[k][4 * q + 3] = cmul(x[k][4 * q + 3], cmul(base, w3));
+          }
         }
-        x[k][r] = v;
       }
       dsti[k] = lbase + ((i - kk) * R + kk) * estride;
     }
@@ -138,7 +160,7 @@ __device__ __forceinline__ void tile_radix_pass(cplx_t<T> *buf, const cplx_t<T> 
     if (dsti[k] >= 0) {
       Bfly<T, R>::run(x[k], sgn);
 #pragma unroll
-      for (int q = 0; q < R; ++q) buf[dsti[k] + q * p * estride] = x[k][q];
+      for (int q = 0; q < R; ++q) buf[dsti[k] + q * p * estride] = x[k][Bfly<T, R>::perm(q)];
     }
   }
   __syncthreads();
@@ -151,14 +173,24 @@ __device__ __forceinline__ void tile_fft(cplx_t<T> *buf, const cplx_t<T> *tw, in
   const int l0 = threadIdx.x % nlines, i0 = threadIdx.x / nlines, di = blockDim.x / nlines;
   int p = 1;
   for (int s = 0; s < nf; ++s) {
-    const int R = (int)((f >> (4 * s)) & 15u);
+    const int R = (int)((f >> (4 * s)) & 15u) + 1;
+#define FBSE_PASS(RR)                                                                               \
+  if constexpr (tile_radix_fits(E, RR))                                                             \
+    tile_radix_pass<T, RR, (E + RR - 1) / RR>(buf, tw, n, p, estride, lstride, l0, i0, di, inv)
     switch (R) {
-      case 2: tile_radix_pass<T, 2, (E + 1) / 2>(buf, tw, n, p, estride, lstride, l0, i0, di, inv); break;
-      case 3: tile_radix_pass<T, 3, (E + 2) / 3>(buf, tw, n, p, estride, lstride, l0, i0, di, inv); break;
-      case 4: tile_radix_pass<T, 4, (E + 3) / 4>(buf, tw, n, p, estride, lstride, l0, i0, di, inv); break;
-      case 5: tile_radix_pass<T, 5, (E + 4) / 5>(buf, tw, n, p, estride, lstride, l0, i0, di, inv); break;
-      default: tile_radix_pass<T, 8, (E + 7) / 8>(buf, tw, n, p, estride, lstride, l0, i0, di, inv); break;
+      case 2: FBSE_PASS(2); break;
+      case 3: FBSE_PASS(3); break;
+      case 4: FBSE_PASS(4); break;
+      case 5: FBSE_PASS(5); break;
+      case 6: FBSE_PASS(6); break;
+      case 9: FBSE_PASS(9); break;
+      case 10: FBSE_PASS(10); break;
+      case 12: FBSE_PASS(12); break;
+      case 15: FBSE_PASS(15); break;
+      case 16: FBSE_PASS(16); break;
+      default: FBSE_PASS(8); break;
     }
+#undef FBSE_PASS
     p *= R;
   }
 }
@@ -386,6 +418,36 @@ static bool radices_ok(const AxisPlan &pl) {
   return pl.nf > 0 && pl.nf <= 16;
 }
 
+// Radix schedule of the plane/slab passes for a line of length n (a product
+// of 2, 3 and 5) in an instance staging e values per thread: the fewest
+// passes over the radices below that fit e (the composite ones run as one
+// register butterfly), ties to the smaller largest radix,
+// larger radices first.  Packed as 4 bits (radix - 1) per pass into *f;
+// returns the number of passes.
+static int tile_schedule(int n, int e, unsigned long long *f) {
+  static const int rad[] = {16, 15, 12, 10, 9, 8, 6, 5, 4, 3, 2};
+  std::vector<int> passes(n + 1, 1 << 20), maxr(n + 1, 0), pick(n + 1, 0);
+  passes[1] = 0;
+  for (int m = 2; m <= n; ++m) {
+    if (n % m) continue;
+    for (int r : rad) {
+      if (m % r || passes[m / r] >= (1 << 20) || !tile_radix_fits(e, r)) continue;
+      const int pp = passes[m / r] + 1, mr = std::max(maxr[m / r], r);
+      if (pp < passes[m] || (pp == passes[m] && mr < maxr[m])) {
+        passes[m] = pp;
+        maxr[m] = mr;
+        pick[m] = r;
+      }
+    }
+  }
+  std::vector<int> fs;
+  for (int m = n; m > 1 && pick[m]; m /= pick[m]) fs.push_back(pick[m]);
+  std::sort(fs.begin(), fs.end(), [](int a, int b) { return a > b; });
+  *f = 0;
+  for (size_t i = 0; i < fs.size() && i < 16; ++i) *f |= (unsigned long long)(fs[i] - 1) << (4 * i);
+  return (int)fs.size();
+}
+
 // LDS row pitch: odd, so that row transforms (threads on consecutive rows)
 // hit distinct banks.
 static int tile_pitch(int nc) { return nc | 1; }
@@ -455,12 +517,6 @@ static int tile_pass_v(fbse_ctx *c, int v, int kind, const cplx_t<T> *src, cplx_
   const bool xy = kind == kTileFwdXY || kind == kTileBwdXY || kind == kTileBwdXYCombine;
   const AxisPlan &pc = c->plan[0], &pr = c->plan[xy ? 1 : 2];
   A.nr = pr.n;
-  A.nfc = pc.nf;
-  A.nfr = pr.nf;
-  A.fc = 0;
-  A.fr = 0;
-  for (int i = 0; i < pc.nf; ++i) A.fc |= (unsigned long long)pc.f[i] << (4 * i);
-  for (int i = 0; i < pr.nf; ++i) A.fr |= (unsigned long long)pr.f[i] << (4 * i);
   A.twc = twiddles<T>(c, 0);
   A.twr = twiddles<T>(c, xy ? 1 : 2);
   if (xy) {
@@ -509,6 +565,8 @@ static int tile_pass_v(fbse_ctx *c, int v, int kind, const cplx_t<T> *src, cplx_
   if (ntiles > 0x7fffffffL) return set_err(c, FBSE_ERR_UNSUPPORTED, "%ld FFT tiles in one launch", ntiles);
   int nv;
   const TileVariant *tv = tile_variants<T>(&nv);
+  A.nfc = tile_schedule(pc.n, tv[v].e, &A.fc);
+  A.nfr = tile_schedule(pr.n, tv[v].e, &A.fr);
   const int nt = tile_threads(A.nr, A.nc, tv[v].e, tv[v].max_nt);
   if (nt == 0) return set_err(c, FBSE_ERR_UNSUPPORTED, "tile %d x %d does not fit FFT instance %d", A.nr, A.nc, v);
   const size_t lds = tile_lds_bytes<T>(A.nr, A.nc);
