@@ -842,7 +842,7 @@ template <typename T> struct FilterRun {
     const C *in = ring + (size_t)((j - 1) % R) * slot_elems;
     C *out = ring + (size_t)(j % R) * slot_elems;
     const double a0 = -(2.0 + zn[j - 1] + c2);
-    int r = apply_h_cplx<T>(c, in, out, work, n_items, (T)dE_1, (T)a0);
+    int r = apply_h_chain<T>(c, in, out, work, n_items, (T)dE_1, (T)a0, j == 1);
     if (r) return r;
     if (j + 1 - acc_lo == R || j == M - 1) {
       if ((r = accumulate_window<T>(c, ring, R, slot_elems, acc_lo, j + 1, d_an, ms, M, d_psi_rank, n_starts,
@@ -869,7 +869,7 @@ template <typename T> struct FilterRun {
       c->dyn_step_off = i;
       const C *in = ring + (size_t)((i + R - 1) % R) * slot_elems;
       C *out = ring + (size_t)i * slot_elems;
-      crc = apply_h_cplx<T>(c, in, out, work, n_items, (T)dE_1, (T)0);
+      crc = apply_h_chain<T>(c, in, out, work, n_items, (T)dE_1, (T)0, false);
     }
     c->dyn_a0_tab = nullptr;
     c->dyn_step_base = nullptr;

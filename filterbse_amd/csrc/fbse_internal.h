@@ -230,9 +230,12 @@ int fft_pass(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src, cplx_t<T> *d
 template <typename T> bool tiles_supported(const fbse_ctx *c);
 // pjd != nullptr: the combine pass also adds the projector expansion with the
 // coefficients of proj_project
+// fwd_next: the combine pass also leaves the forward xy transform of out in
+// work (the first pass of the next Newton step); fwd_ready: work already
+// holds the forward xy transform of psi, so the first pass is skipped.
 template <typename T>
 int kinetic_local_tiles(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *work, long nstates, T a1,
-                        T a0, const cplx_t<T> *pjd = nullptr);
+                        T a0, const cplx_t<T> *pjd = nullptr, bool fwd_ready = false, bool fwd_next = false);
 template <typename T>
 int precondition_tiles(fbse_ctx *c, const cplx_t<T> *r, cplx_t<T> *out, cplx_t<T> *work, long nstates, T pa, T pb);
 
@@ -246,6 +249,13 @@ int fft_path(const fbse_ctx *c, int f32, int *instance);
 template <typename T>
 int apply_h_cplx(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *work, long nstates, T a1,
                  T a0);
+// One step of the Newton recurrence, out = a1*H psi + a0*psi, for a chain of
+// steps out_j = step(out_{j-1}) that keep `work` between calls: with the
+// plane/slab passes each step also leaves the forward xy transform of out in
+// work for the next one.  first: psi does not come from the previous step.
+template <typename T>
+int apply_h_chain(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *work, long nstates, T a1, T a0,
+                  bool first);
 
 // n contiguous states (reference layout, FP64): out = F^-1 [ F[r] / (N (1 + 0.5|k|^2 / e_p)) ]
 // (capi.hip; out may equal r)

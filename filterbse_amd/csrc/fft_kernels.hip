@@ -465,6 +465,19 @@ int fft_path(const fbse_ctx *c, int f32, int *instance) {
   return tiles ? 1 : 0;
 }
 
+// projection coefficients (column x atom x 6) for nstates states
+template <typename T> static int ensure_pjd(fbse_ctx *c, long nstates) {
+  const size_t need = (size_t)c->pj.n_atoms * nstates * c->nspin * 6 * sizeof(cplx_t<T>);
+  if (need > c->pjd_bytes) {
+    if (c->d_pjd) (void)hipFree(c->d_pjd);
+    c->d_pjd = nullptr;
+    c->pjd_bytes = 0;
+    FBSE_HIP(c, hipMalloc(&c->d_pjd, need));
+    c->pjd_bytes = need;
+  }
+  return FBSE_OK;
+}
+
 template <typename T>
 int apply_h_cplx(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *work, long nstates, T a1, T a0) {
   const long nb = nstates * c->nspin;
@@ -472,14 +485,7 @@ int apply_h_cplx(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *w
   if (use_tiles<T>(c)) {
     if (proj_fusable(c)) {
       // projections first (they read psi), the expansion rides on the combine pass
-      const size_t need = (size_t)c->pj.n_atoms * nb * 6 * sizeof(cplx_t<T>);
-      if (need > c->pjd_bytes) {
-        if (c->d_pjd) (void)hipFree(c->d_pjd);
-        c->d_pjd = nullptr;
-        c->pjd_bytes = 0;
-        FBSE_HIP(c, hipMalloc(&c->d_pjd, need));
-        c->pjd_bytes = need;
-      }
+      if ((rc = ensure_pjd<T>(c, nstates))) return rc;
       cplx_t<T> *d = (cplx_t<T> *)c->d_pjd;
       if ((rc = proj_project<T>(c, psi, nstates, (double)a1, d))) return rc;
       return kinetic_local_tiles<T>(c, psi, out, work, nstates, a1, a0, d);
@@ -497,6 +503,27 @@ int apply_h_cplx(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *w
   }
   return FBSE_OK;
 }
+
+template <typename T>
+int apply_h_chain(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *work, long nstates, T a1, T a0,
+                  bool first) {
+  if (!use_tiles<T>(c)) return apply_h_cplx<T>(c, psi, out, work, nstates, a1, a0);
+  const cplx_t<T> *d = nullptr;
+  int rc;
+  if (proj_fusable(c)) {
+    if ((rc = ensure_pjd<T>(c, nstates))) return rc;
+    d = (cplx_t<T> *)c->d_pjd;
+    if ((rc = proj_project<T>(c, psi, nstates, (double)a1, (cplx_t<T> *)d))) return rc;
+  }
+  // the expansion of non-fusable projectors is added to out after the
+  // combine pass, so then out's forward transform cannot ride on it
+  const bool chain = !c->has_proj || d;
+  if ((rc = kinetic_local_tiles<T>(c, psi, out, work, nstates, a1, a0, d, chain && !first, chain))) return rc;
+  if (!chain) return proj_apply<T>(c, psi, out, nstates, (double)a1);
+  return FBSE_OK;
+}
+template int apply_h_chain<double>(fbse_ctx *, const double2 *, double2 *, double2 *, long, double, double, bool);
+template int apply_h_chain<float>(fbse_ctx *, const float2 *, float2 *, float2 *, long, float, float, bool);
 
 template <typename T>
 int precondition_cplx(fbse_ctx *c, const cplx_t<T> *r, cplx_t<T> *out, cplx_t<T> *work, long nstates, double e_p) {

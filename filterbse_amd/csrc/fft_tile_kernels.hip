@@ -70,6 +70,9 @@ template <typename T> struct TileArgs {
   const T *rec_val;
   const cplx_t<T> *pjd;
   int pj_natoms, pt_ranks;
+  // combine: when set, the forward x and y transforms of the output plane are
+  // also written here (same layout as dst)
+  cplx_t<T> *fwd_dst;
 };
 
 // contribution of one projector record to its point:
@@ -305,10 +308,8 @@ __device__ __forceinline__ void tile_body(const TileArgs<T> &A) {
           C o;
           o.x = A.a1 * (t.x + vv[u] * pr[u]) + a0 * pr[u];
           o.y = A.a1 * (t.y + vv[u] * pim[u]) + a0 * pim[u];
-          if (expand)
-            buf[r * ls + c0] = o;
-          else
-            dst[r * A.row_stride] = o;
+          if (expand || A.fwd_dst) buf[r * ls + c0] = o;
+          if (!expand) dst[r * A.row_stride] = o;
         }
       }
     }
@@ -345,6 +346,14 @@ __device__ __forceinline__ void tile_body(const TileArgs<T> &A) {
         __syncthreads();
       }
       for (int r = r0; r < nr; r += dr) dst[r * A.row_stride] = buf[r * ls + c0];
+    }
+    if (A.fwd_dst) {
+      // the next Newton step's first pass on the plane just written
+      __syncthreads();
+      tile_fft<T, E>(buf, twc, nc, A.nfc, A.fc, 1, ls, nr, false);  // x rows
+      tile_fft<T, E>(buf, twr, nr, A.nfr, A.fr, ls, 1, nc, false);  // y columns
+      cplx_t<T> *fd = A.fwd_dst + base + c0;
+      for (int r = r0; r < nr; r += dr) fd[r * A.row_stride] = buf[r * ls + c0];
     }
   } else {
     for (int r = r0; r < nr; r += dr) dst[r * A.row_stride] = buf[r * ls + c0];
@@ -507,7 +516,8 @@ template <typename T, int KIND> static void set_lds_limit(int device, int v) {
 
 template <typename T>
 static int tile_pass_v(fbse_ctx *c, int v, int kind, const cplx_t<T> *src, cplx_t<T> *dst, long nbatch,
-                       const cplx_t<T> *psi_in, T a1, T a0, const cplx_t<T> *pjd = nullptr) {
+                       const cplx_t<T> *psi_in, T a1, T a0, const cplx_t<T> *pjd = nullptr,
+                       cplx_t<T> *fwd_dst = nullptr) {
   TileArgs<T> A{};
   A.src = src;
   A.dst = dst;
@@ -560,6 +570,7 @@ static int tile_pass_v(fbse_ctx *c, int v, int kind, const cplx_t<T> *src, cplx_
     A.pjd = pjd;
     A.pj_natoms = c->pj.n_atoms;
   }
+  if (kind == kTileBwdXYCombine) A.fwd_dst = fwd_dst;
   const long ntiles = nbatch * A.tiles_per_item;
   A.nbatch = nbatch;
   if (ntiles > 0x7fffffffL) return set_err(c, FBSE_ERR_UNSUPPORTED, "%ld FFT tiles in one launch", ntiles);
@@ -607,20 +618,21 @@ static int tile_pass_v(fbse_ctx *c, int v, int kind, const cplx_t<T> *src, cplx_
 // before that use the first instance that fits.
 template <typename T>
 static int tile_pass(fbse_ctx *c, int kind, const cplx_t<T> *src, cplx_t<T> *dst, long nbatch,
-                     const cplx_t<T> *psi_in, T a1, T a0, const cplx_t<T> *pjd = nullptr) {
+                     const cplx_t<T> *psi_in, T a1, T a0, const cplx_t<T> *pjd = nullptr,
+                     cplx_t<T> *fwd_dst = nullptr) {
   const int pi = sizeof(T) == 8 ? 0 : 1;
   if (c->force_variant == 1 && c->force_lines > 0) {  // forced instance (1-based)
     int nv;
     tile_variants<T>(&nv);
     if (c->force_lines > nv) return set_err(c, FBSE_ERR_ARG, "FFT tile instance %d of %d", c->force_lines, nv);
-    return tile_pass_v<T>(c, c->force_lines - 1, kind, src, dst, nbatch, psi_in, a1, a0, pjd);
+    return tile_pass_v<T>(c, c->force_lines - 1, kind, src, dst, nbatch, psi_in, a1, a0, pjd, fwd_dst);
   }
   if (c->tile_variant[pi] < 0) {
     int nv;
     const TileVariant *tv = tile_variants<T>(&nv);
     int v0 = first_tile_variant<T>(c);
     if (!(c->autotune && nbatch * c->ngrid >= (1L << 20)))
-      return tile_pass_v<T>(c, v0, kind, src, dst, nbatch, psi_in, a1, a0, pjd);
+      return tile_pass_v<T>(c, v0, kind, src, dst, nbatch, psi_in, a1, a0, pjd, fwd_dst);
     cplx_t<T> *scratch = nullptr;
     const size_t bytes = (size_t)nbatch * c->ngrid * sizeof(cplx_t<T>);
     int best_v = v0;
@@ -665,24 +677,24 @@ static int tile_pass(fbse_ctx *c, int kind, const cplx_t<T> *src, cplx_t<T> *dst
     }
     c->tile_variant[pi] = best_v;
   }
-  return tile_pass_v<T>(c, c->tile_variant[pi], kind, src, dst, nbatch, psi_in, a1, a0, pjd);
+  return tile_pass_v<T>(c, c->tile_variant[pi], kind, src, dst, nbatch, psi_in, a1, a0, pjd, fwd_dst);
 }
 
 // out = a1*(T + V) psi + a0*psi over nstates*nspin grids (projectors are
 // added by the caller); work holds one batch of grids.
 template <typename T>
 int kinetic_local_tiles(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *work, long nstates, T a1,
-                        T a0, const cplx_t<T> *pjd) {
+                        T a0, const cplx_t<T> *pjd, bool fwd_ready, bool fwd_next) {
   const long nb = nstates * c->nspin;
   int rc;
-  if ((rc = tile_pass<T>(c, kTileFwdXY, psi, work, nb, nullptr, 0, 0))) return rc;
+  if (!fwd_ready && (rc = tile_pass<T>(c, kTileFwdXY, psi, work, nb, nullptr, 0, 0))) return rc;
   if ((rc = tile_pass<T>(c, kTileKinZ, work, work, nb, nullptr, 0, 0))) return rc;
-  return tile_pass<T>(c, kTileBwdXYCombine, work, out, nb, psi, a1, a0, pjd);
+  return tile_pass<T>(c, kTileBwdXYCombine, work, out, nb, psi, a1, a0, pjd, fwd_next ? work : nullptr);
 }
 template int kinetic_local_tiles<double>(fbse_ctx *, const double2 *, double2 *, double2 *, long, double, double,
-                                         const double2 *);
+                                         const double2 *, bool, bool);
 template int kinetic_local_tiles<float>(fbse_ctx *, const float2 *, float2 *, float2 *, long, float, float,
-                                        const float2 *);
+                                        const float2 *, bool, bool);
 
 template <typename T>
 int precondition_tiles(fbse_ctx *c, const cplx_t<T> *r, cplx_t<T> *out, cplx_t<T> *work, long nstates, T pa,
