@@ -137,18 +137,16 @@ def cpu_baseline(S, budget_s=15.0):
 
 
 def measured_traffic(cfg_name, kname):
-    """HBM bytes per launch of the roofline kernel from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload, when it covers
-    the kernel instance in use (else None)."""
-    path = os.path.join(ROOT, "profiles", "%s_fft_traffic.json" % cfg_name)
+    """HBM bytes per launch of a kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload, when it covers that
+    kernel instance (else None)."""
+    path = os.path.join(ROOT, "profiles", "%s_traffic.json" % cfg_name)
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if not kname.startswith(t.get("kernel", "?")):
-        return None
-    return t.get("traffic_bytes_per_launch")
+    return t.get("traffic_bytes_per_launch", {}).get(kname)
 
 
 def traffic_sample(cfg_name, dev, M):
@@ -377,24 +375,42 @@ def main():
             % (t64, rank64, len(ev64_c), fp["max_eig_err_mixed_vs_fp64"]))
 
     job = S["job"]
-    nsg = job["nspin"] * job["nx"] * job["ny"] * job["nz"]
+    ngrid = job["nx"] * job["ny"] * job["nz"]
+    nsg = job["nspin"] * ngrid
     n_items = S["n_starts"] if job["is_complex"] else (S["n_starts"] + 1) // 2
     elem = 8  # complex float
-    fft_launches, fft_ms = prof[0]
-    avg_ms = fft_ms / max(fft_launches, 1)
-    # algorithmic bytes of one FFT launch: the batch of n_items spinor grids
-    # (nsg complex FP32 values each) read once and written once
-    bytes_per_launch = 2.0 * n_items * nsg * elem
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    grid_bytes = n_items * nsg * elem  # one pass over the batch of spinor grids
+    pj = job["proj"]
+    chained = pj is None or ((pj["n_so_proj"] if pj["so"] else 0) <= 1 and (pj["n_nl_proj"] if pj["nl"] else 0) <= 1)
     if fft_path == 1:
-        kname = "fft_tile_f32 instance %d (xy-plane forward and xz-slab kinetic passes)" % fft_inst
+        e = {1: 12, 2: 16, 3: 32}.get(fft_inst, 12)
+        # dominant kernel: the combine pass.  Algorithmic bytes per launch:
+        # the z-slab output and psi_{j-1} read, psi_j written, V_loc read
+        # once, and (chained) the next step's forward xy transform written
+        launches, ms_tot = prof[1]
+        bytes_per_launch = (4.0 if chained else 3.0) * grid_bytes + job["nspin"] * ngrid * 4
+        kname = "fft_tile_f32_e%d<2>" % e
+        kdesc = ("xy-plane backward FFT, V_loc, Newton combine, projector expansion%s"
+                 % (", next step's forward xy FFT" if chained else ""))
+        zl, zms = prof[0]
+        zslab = dict(kernel="fft_tile_f32_e%d<3>" % e, desc="xz-slab forward z FFT, k^2, backward z FFT",
+                     avg_launch_ms=round(zms / max(zl, 1), 4), bytes_per_launch=2.0 * grid_bytes,
+                     achieved=round(2.0 * grid_bytes / (zms / max(zl, 1) * 1e-3) / 1e9, 1))
+        zslab["frac"] = round(zslab["achieved"] / HBM_PEAK_GBS, 4)
+        zslab["traffic"] = measured_traffic(args.config, zslab["kernel"])
     else:
-        kname = "fft_axis_kernel<float> (x/y forward, z kinetic, y backward passes)"
+        launches, ms_tot = prof[0]
+        bytes_per_launch = 2.0 * grid_bytes
+        kname = "fft_axis_kernel<float>"
+        kdesc = "x/y forward, z kinetic, y backward passes"
+        zslab = None
+    avg_ms = ms_tot / max(launches, 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(args.config, kname),
-                    kernel=kname,
-                    avg_launch_ms=round(avg_ms, 4), bytes_per_launch=bytes_per_launch,
-                    launches=fft_launches, measured="HIP events on the library stream, filter of the last timed pass")
+                    kernel=kname, desc=kdesc, avg_launch_ms=round(avg_ms, 4), bytes_per_launch=bytes_per_launch,
+                    launches=launches, measured="HIP events on the library stream, filter of the last timed pass",
+                    zslab=zslab)
     kernels = {name: dict(launches=prof[k][0], total_ms=round(prof[k][1], 3))
                for k, name in enumerate(["fft_passes", "x_backward_newton_combine", "projectors",
                                          "window_accumulate", "all"])}
