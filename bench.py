@@ -186,6 +186,56 @@ def traffic_sample(cfg_name, dev, M):
         % (M, nbytes, nbytes))
 
 
+def io_sample(args, dev):
+    """State-file streaming: one start block of the c1 workload (64 start
+    states x 16 targets would be 29 GB; here the first n FP64 spinor states of
+    96^3, 28.3 MB each) written from HBM in the reference's psi.dat layout
+    (fbse_write_states: pinned double buffering) and read back as start states
+    (fbse_read_start_states), against the same write through torch's
+    device-to-host copy and numpy's tofile.  Files go to $TMPDIR (page cache:
+    no fsync, as the reference's fwrite/fclose)."""
+    import tempfile
+    import torch
+    from filterbse_amd import api
+    cfg = dict(__import__("filterbse_amd.dots", fromlist=["CONFIGS"]).CONFIGS["c1_so_96"])
+    n = args.io_sample
+    nx, ny, nz = cfg["n_req"]
+    nspin = 2
+    ngrid = nx * ny * nz
+    ksqr = np.zeros(ngrid)
+    ctx = api.Context(nx, ny, nz, nspin, 1, cfg["dx"] ** 3, ksqr, np.zeros(nspin * ngrid), None)
+    stlen = 2 * nspin * ngrid
+    g = torch.Generator(device=dev)
+    g.manual_seed(1)
+    d = torch.randn(n * stlen, dtype=torch.float64, device=dev, generator=g)
+    torch.cuda.synchronize()
+    nbytes = n * stlen * 8
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "psi.dat")
+        for rep in range(2):  # the second repetition is reported (pinned buffers allocated)
+            t0 = time.perf_counter()
+            ctx.write_states(d.data_ptr(), n, path)
+            out["write_gbs"] = nbytes / (time.perf_counter() - t0) / 1e9
+        t0 = time.perf_counter()
+        d.cpu().numpy().tofile(path + ".ref")
+        out["write_torch_numpy_gbs"] = nbytes / (time.perf_counter() - t0) / 1e9
+        same = open(path, "rb").read() == open(path + ".ref", "rb").read()
+        r = torch.empty(n * stlen, dtype=torch.float64, device=dev)
+        t0 = time.perf_counter()
+        k = ctx.read_start_states(path, 0, n, 1, r.data_ptr(), n, 1)
+        out["read_gbs"] = nbytes / (time.perf_counter() - t0) / 1e9
+        ok = k == n and bool(torch.equal(r, d))
+    line = dict(metric="state-file streaming GB/s (HBM <-> file, reference psi.dat layout)",
+                value=round(out["write_gbs"], 3), unit="GB/s", n_gpus=1, higher_is_better=True,
+                data="synthetic (seeded normal FP64 states)",
+                config=dict(workload="%d FP64 spinor states of a 96^3 grid (%.2f GB), 64 MiB pinned chunks" % (
+                    n, nbytes / 1e9)),
+                write_gbs=round(out["write_gbs"], 3), write_torch_numpy_gbs=round(out["write_torch_numpy_gbs"], 3),
+                read_gbs=round(out["read_gbs"], 3), bytes_identical_to_numpy=same, read_back_exact=ok)
+    print(json.dumps(line), flush=True)
+
+
 # Batched uneven dots: 32 spin-orbit dots with seeded radii uniform in
 # 1.0-3.0 nm at dx = 1.0 bohr (the reference grid rule needs ~52^3 ... ~120^3;
 # each axis is rounded up to a size with prime factors 2, 3, 5), 32 starts x 8
@@ -321,6 +371,8 @@ def main():
     ap.add_argument("--traffic-sample", type=int, default=0, metavar="M",
                     help="only run an M-step mixed filter of the workload and one device copy of the bytes of "
                          "one FFT launch (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs), then exit")
+    ap.add_argument("--io-sample", type=int, default=0, metavar="N",
+                    help="only measure state-file streaming of N FP64 spinor states of 96^3, then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the filter on a single MI355X; "
@@ -332,6 +384,9 @@ def main():
     dev = torch.device("cuda:0")
     if args.traffic_sample:
         traffic_sample(args.config, dev, args.traffic_sample)
+        return
+    if args.io_sample:
+        io_sample(args, dev)
         return
     if args.config == "c2_uneven":
         bench_uneven(args, dev)

@@ -17,14 +17,15 @@ HOST_LIB = os.path.join(HERE, "libfbse_host.so")
 FBSE_PREC_FP64 = 0
 FBSE_PREC_MIXED = 1
 FBSE_OK, FBSE_ERR_ARG, FBSE_ERR_HIP, FBSE_ERR_NORM_ZERO, FBSE_ERR_NORM_NAN, FBSE_ERR_SOLVER, FBSE_ERR_NOMEM, \
-    FBSE_ERR_UNSUPPORTED = 0, -1, -2, -3, -4, -5, -6, -7
+    FBSE_ERR_UNSUPPORTED, FBSE_ERR_IO = 0, -1, -2, -3, -4, -5, -6, -7, -8
 
 DP = C.POINTER(C.c_double)
 LP = C.POINTER(C.c_long)
 IP = C.POINTER(C.c_int)
 
 ERRORS = {0: "FBSE_OK", -1: "FBSE_ERR_ARG", -2: "FBSE_ERR_HIP", -3: "FBSE_ERR_NORM_ZERO",
-          -4: "FBSE_ERR_NORM_NAN", -5: "FBSE_ERR_SOLVER", -6: "FBSE_ERR_NOMEM", -7: "FBSE_ERR_UNSUPPORTED"}
+          -4: "FBSE_ERR_NORM_NAN", -5: "FBSE_ERR_SOLVER", -6: "FBSE_ERR_NOMEM", -7: "FBSE_ERR_UNSUPPORTED",
+          -8: "FBSE_ERR_IO"}
 
 
 class FbseError(RuntimeError):
@@ -111,6 +112,11 @@ GPU_SIGNATURES = {
                                    C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]),
     "fbse_graph_replays": (C.c_long, [C.c_void_p]),
     "fbse_trim": (C.c_int, [C.c_void_p]),
+    "fbse_fwrite_states": (C.c_int, [C.c_void_p, C.c_void_p, LP, C.c_long, C.c_int, C.c_void_p]),
+    "fbse_write_states": (C.c_int, [C.c_void_p, C.c_void_p, LP, C.c_long, C.c_int, C.c_char_p]),
+    "fbse_read_start_states": (C.c_int, [C.c_void_p, C.c_char_p, C.c_long, C.c_long, C.c_int, C.c_void_p,
+                                         C.c_long, C.c_long, LP]),
+    "fbse_set_io_chunk": (C.c_int, [C.c_void_p, C.c_size_t]),
     "fbse_set_refine_basis": (C.c_int, [C.c_void_p, C.c_int]),
     "fbse_dev_alloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     "fbse_dev_free": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -427,6 +433,37 @@ class Context:
     def trim(self):
         """Release the context's work buffers (allocated again on demand)."""
         self._check(gpu().fbse_trim(self.ctx))
+
+    @staticmethod
+    def _idx(idx):
+        if idx is None:
+            return None, None
+        ix = np.ascontiguousarray(idx, dtype=np.int64)
+        return ix, ix.ctypes.data_as(LP)
+
+    def write_states(self, d_psi, n, path, idx=None, n_every=1):
+        """States of the device block d_psi (idx order, or 0..n-1) to a new
+        file at path in the reference's raw layout (fbse_write_states)."""
+        ix, p = self._idx(idx)
+        self._check(gpu().fbse_write_states(self.ctx, C.c_void_p(d_psi), p, len(ix) if ix is not None else n,
+                                            n_every, os.fsencode(path)))
+
+    def fwrite_states(self, d_psi, n, stream, idx=None, n_every=1):
+        """fbse_fwrite_states to an open C stream (FILE *, as an integer)."""
+        ix, p = self._idx(idx)
+        self._check(gpu().fbse_fwrite_states(self.ctx, C.c_void_p(d_psi), p, len(ix) if ix is not None else n,
+                                             n_every, C.c_void_p(stream)))
+
+    def read_start_states(self, path, start, end, n_every, d_psi_rank, n_starts, ms):
+        """Start states from a file into the filter's start blocks
+        (fbse_read_start_states); returns the number placed."""
+        n = C.c_long(0)
+        self._check(gpu().fbse_read_start_states(self.ctx, os.fsencode(path), start, end, n_every,
+                                                 C.c_void_p(d_psi_rank), n_starts, ms, C.byref(n)))
+        return n.value
+
+    def set_io_chunk(self, nbytes):
+        self._check(gpu().fbse_set_io_chunk(self.ctx, nbytes))
 
     def graph_replays(self):
         """hipGraph replays of the last filter call."""

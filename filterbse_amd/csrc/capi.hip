@@ -426,6 +426,7 @@ extern "C" void fbse_destroy(fbse_ctx *c) {
     if (c->d_tw_f[a]) hipFree(c->d_tw_f[a]);
   }
   if (c->h_red) hipHostFree(c->h_red);
+  io_release(c);
   for (auto e : c->ev_pool) hipEventDestroy(e);
   for (auto &p : c->ev_pending) {
     hipEventDestroy(p.second.first);
@@ -493,6 +494,7 @@ extern "C" int fbse_trim(fbse_ctx *c) {
   if (c->d_gram) FBSE_HIP(c, hipFree(c->d_gram));
   c->d_scratch = c->d_scratch2 = c->d_gram = nullptr;
   c->scratch_bytes = c->scratch2_bytes = c->gram_bytes = 0;
+  io_release(c);
   return FBSE_OK;
 }
 extern "C" long fbse_graph_replays(const fbse_ctx *c) { return c ? c->graph_replays : -1; }

@@ -161,6 +161,11 @@ struct fbse_ctx {
   size_t red_bytes = 0;
   double *h_red = nullptr;      // pinned host mirror
   size_t h_red_bytes = 0;
+  // state-file streaming (io.hip): two pinned staging buffers and their events
+  void *h_io[2] = {nullptr, nullptr};
+  size_t io_bytes = 0;
+  size_t io_chunk = (size_t)64 << 20;
+  hipEvent_t io_ev[2] = {nullptr, nullptr};
 
   int window = 0;               // forced window depth (0 = auto)
 
@@ -196,6 +201,8 @@ namespace fbse {
 
 // error helpers
 int set_err(fbse_ctx *c, int code, const char *fmt, ...);
+// releases the pinned staging buffers of io.hip
+void io_release(fbse_ctx *c);
 #define FBSE_HIP(c, call)                                                                  \
   do {                                                                                     \
     hipError_t e_ = (call);                                                                \

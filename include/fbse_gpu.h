@@ -36,6 +36,7 @@ extern "C" {
 #define FBSE_ERR_SOLVER (-5)      /* LAPACK info != 0 analogue (ortho.c:56-62, Hmat.c:202-216) */
 #define FBSE_ERR_NOMEM (-6)
 #define FBSE_ERR_UNSUPPORTED (-7)
+#define FBSE_ERR_IO (-8)         /* a state file could not be opened, read or written (fbse_io.h) */
 
 #define FBSE_PREC_FP64 0          /* FFT, recurrence and accumulation in FP64 */
 #define FBSE_PREC_MIXED 1         /* FP32 complex FFT/recurrence, FP64 coefficients,

@@ -20,7 +20,7 @@ def declared(header):
     return sorted(set(re.findall(r"\b(fbse_[A-Za-z0-9_]+)\s*\(", txt)))
 
 
-@pytest.mark.parametrize("headers,path", [(("fbse_gpu.h", "fbse_mlp.h"), api.GPU_LIB),
+@pytest.mark.parametrize("headers,path", [(("fbse_gpu.h", "fbse_mlp.h", "fbse_io.h"), api.GPU_LIB),
                                           (("fbse_host.h",), api.HOST_LIB)])
 def test_library_exports_every_declared_symbol(headers, path):
     names = sorted(set(n for h in headers for n in declared(h)))
