@@ -36,7 +36,8 @@ def setup(cfg_name, dev):
 
     cfg = dict(dots.CONFIGS[cfg_name])
     t0 = time.perf_counter()
-    job = dots.build_dot(cfg["radius_nm"], cfg["n_req"], cfg["dx"], so=cfg["so"], nl=cfg["nl"])
+    job = dots.build_dot(cfg["radius_nm"], cfg["n_req"], cfg["dx"], so=cfg["so"], nl=cfg["nl"],
+                         builder=cfg.get("builder", "host"))
     log("dot: %d core atoms, %d ligands, grid %dx%dx%d, %.1fs" % (
         job["n_core"], len(job["syms"]) - job["n_core"], job["nx"], job["ny"], job["nz"], time.perf_counter() - t0))
     ctx = api.Context(job["nx"], job["ny"], job["nz"], job["nspin"], job["is_complex"], job["dv"], job["ksqr"],

@@ -260,6 +260,12 @@ static int build_projectors(fbse_ctx *c, const fbse_proj_desc *pd) {
     std::vector<int> roff((size_t)c->nz * nr + 1, 0);
     for (long i = 0; i < npts; ++i) ++roff[(size_t)(jx[i] / nxy) * nr + rank[i] + 1];
     for (size_t b = 0; b + 1 < roff.size(); ++b) roff[b + 1] += roff[b];
+    c->rec_seg_max.assign(nr, 0);
+    for (long z = 0; z < c->nz; ++z)
+      for (int k = 0; k < nr; ++k) {
+        const size_t b = (size_t)z * nr + k;
+        c->rec_seg_max[k] = std::max(c->rec_seg_max[k], roff[b + 1] - roff[b]);
+      }
     std::vector<std::pair<long, int>> keyed(npts);  // (bucket * ngrid + point, entry)
     for (long i = 0; i < npts; ++i)
       keyed[i] = {((long)(jx[i] / nxy) * nr + rank[i]) * c->ngrid + jx[i], (int)i};

@@ -145,6 +145,7 @@ struct fbse_ctx {
   double *d_rec_val = nullptr;
   float *d_rec_val_f = nullptr;
   int pt_ranks = 0;
+  std::vector<int> rec_seg_max;  // per rank: the most records one plane holds
   void *d_pjd = nullptr;        // mixed projection coefficients (column x atom x 6), grown on demand
   size_t pjd_bytes = 0;
   int fuse_proj = 1;            // expansion inside the plane passes when the tables allow it
@@ -271,6 +272,11 @@ int precondition_dev(fbse_ctx *c, const double *d_r, double *d_out, long n, doub
 // kinetic-energy preconditioner out = F^-1 [ F[r] / (N (1 + 0.5|k|^2 / e_p)) ]
 template <typename T>
 int precondition_cplx(fbse_ctx *c, const cplx_t<T> *r, cplx_t<T> *out, cplx_t<T> *work, long nstates, double e_p);
+
+// expansion from the rank records (ham_kernels.hip, one launch per rank):
+// out += the expansion of the coefficients d of proj_project
+template <typename T>
+int proj_expand(fbse_ctx *c, const cplx_t<T> *d, cplx_t<T> *out, long nstates);
 
 // projector launcher (ham_kernels.hip): psi_out += scale * Vproj psi_in
 template <typename T>

@@ -35,6 +35,23 @@ template <typename C, typename T> __device__ __forceinline__ C cmul_i(C a, T s) 
   return r;
 }
 
+// contribution of one projector record to its point:
+// Y_m (proj d_so[m] + nl_proj d_nl[m]) summed over m (phamiltonian.c:140-444)
+template <typename T>
+__device__ __forceinline__ cplx_t<T> record_term(const T *v, const cplx_t<T> *dd) {
+  cplx_t<T> acc{(T)0, (T)0};
+  const T wso = v[0], wnl = v[1];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const cplx_t<T> a = dd[m], b = dd[3 + m];
+    const T sx = wso * a.x + wnl * b.x, sy = wso * a.y + wnl * b.y;
+    const T yx = v[2 + 2 * m], yy = v[3 + 2 * m];
+    acc.x += yx * sx - yy * sy;
+    acc.y += yx * sy + yy * sx;
+  }
+  return acc;
+}
+
 // Butterfly of radix R on values x[0..R-1] (already twiddled); sgn = -1 forward.
 // Output q is left in x[perm(q)] (the identity for the prime-power radices).
 template <typename T, int R> struct Bfly;

@@ -492,11 +492,20 @@ int apply_h_cplx(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *w
     }
     if ((rc = kinetic_local_tiles<T>(c, psi, out, work, nstates, a1, a0))) return rc;
   } else {
+    // with one radial projector: projections before the passes (they read
+    // psi), the expansion from the rank records after the combine
+    cplx_t<T> *d = nullptr;
+    if (proj_fusable(c)) {
+      if ((rc = ensure_pjd<T>(c, nstates))) return rc;
+      d = (cplx_t<T> *)c->d_pjd;
+      if ((rc = proj_project<T>(c, psi, nstates, (double)a1, d))) return rc;
+    }
     if ((rc = fft_pass<T>(c, 0, kPassFwd, psi, work, nb, nullptr, 0, 0))) return rc;
     if ((rc = fft_pass<T>(c, 1, kPassFwd, work, work, nb, nullptr, 0, 0))) return rc;
     if ((rc = fft_pass<T>(c, 2, kPassKin, work, work, nb, nullptr, 0, 0))) return rc;
     if ((rc = fft_pass<T>(c, 1, kPassBwd, work, work, nb, nullptr, 0, 0))) return rc;
     if ((rc = fft_pass<T>(c, 0, kPassBwdCombine, work, out, nb, psi, a1, a0))) return rc;
+    if (d) return proj_expand<T>(c, d, out, nstates);
   }
   if (c->has_proj) {
     if ((rc = proj_apply<T>(c, psi, out, nstates, (double)a1))) return rc;

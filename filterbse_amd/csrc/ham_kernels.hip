@@ -1,6 +1,7 @@
 // ham_kernels.hip -- projector (SO / NL) application, windowed Newton
 // accumulation and per-state reductions for gfx950.
 #include "fbse_internal.h"
+#include "fft_common.h"
 
 namespace fbse {
 
@@ -299,6 +300,51 @@ int proj_project(fbse_ctx *c, const cplx_t<T> *psi_in, long nstates, double a1, 
   prof_end(c, 2, ev);
   return FBSE_OK;
 }
+// Expansion of the rank-k records of every plane into out (one batch item
+// per blockIdx.z, one plane per blockIdx.y): the arithmetic of the plane
+// pass's rounds; the records of one rank touch distinct points, and ranks run
+// as separate launches in order, so every point sums its atoms in index order.
+template <typename T>
+__global__ void proj_expand_rank_kernel(const int *rec_off, const int *rec_pa, const T *rec_val, int ranks, int k,
+                                        const cplx_t<T> *d, int natoms, cplx_t<T> *out, long item_stride, long nx,
+                                        long nxny) {
+  const int z = blockIdx.y;
+  const long item = blockIdx.z;
+  const int e = rec_off[z * ranks + k] + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (e >= rec_off[z * ranks + k + 1]) return;
+  const int2 q = reinterpret_cast<const int2 *>(rec_pa)[e];
+  const cplx_t<T> t = record_term<T>(rec_val + (long)e * 8, d + (item * natoms + q.y) * 6);
+  cplx_t<T> *o = out + item * item_stride + z * nxny + (long)(q.x >> 16) * nx + (q.x & 0xffff);
+  cplx_t<T> v = *o;
+  v.x += t.x;
+  v.y += t.y;
+  *o = v;
+}
+
+template <typename T>
+int proj_expand(fbse_ctx *c, const cplx_t<T> *d, cplx_t<T> *out, long nstates) {
+  if (!proj_fusable(c)) return set_err(c, FBSE_ERR_UNSUPPORTED, "expansion pass needs one radial projector");
+  const long nb = nstates * c->nspin;
+  if (nb > 65535) return set_err(c, FBSE_ERR_UNSUPPORTED, "%ld grids in one expansion launch", nb);
+  const T *val;
+  if constexpr (sizeof(T) == 8) val = c->d_rec_val;
+  else val = c->d_rec_val_f;
+  hipEvent_t ev = nullptr;
+  prof_begin(c, 2, &ev);
+  for (int k = 0; k < c->pt_ranks; ++k) {
+    const int seg = c->rec_seg_max[k];
+    if (seg == 0) continue;
+    const dim3 g((unsigned)((seg + 255) / 256), (unsigned)c->nz, (unsigned)nb);
+    hipLaunchKernelGGL((proj_expand_rank_kernel<T>), g, dim3(256), 0, c->stream, c->d_rec_off, c->d_rec_pa, val,
+                       c->pt_ranks, k, d, c->pj.n_atoms, out, c->ngrid, c->nx, c->nx * c->ny);
+  }
+  FBSE_HIP(c, hipGetLastError());
+  prof_end(c, 2, ev);
+  return FBSE_OK;
+}
+template int proj_expand<double>(fbse_ctx *, const double2 *, double2 *, long);
+template int proj_expand<float>(fbse_ctx *, const float2 *, float2 *, long);
+
 template int proj_project<double>(fbse_ctx *, const double2 *, long, double, double2 *);
 template int proj_project<float>(fbse_ctx *, const float2 *, long, double, float2 *);
 

@@ -393,4 +393,11 @@ CONFIGS = {
                        ncheby=512, targets=(4, 4, -0.31, -0.24, -0.15, -0.06)),
     "c1_so_96": dict(radius_nm=2.0, n_req=(96, 96, 96), dx=0.9, so=True, nl=True, n_starts=64, ms=16,
                      ncheby=1024, targets=(10, 6, -0.31, -0.24, -0.15, -0.06)),
+    # 3.5 nm dot, 160^3, 128 filtered states (8 starts x 16 targets), Newton length 2048
+    "c3_so_160": dict(radius_nm=3.5, n_req=(160, 160, 160), dx=0.89, so=True, nl=True, n_starts=8, ms=16,
+                      ncheby=2048, targets=(10, 6, -0.31, -0.24, -0.15, -0.06), builder="gpu"),
+    # 4.5 nm dot, 256^3 (dx 0.71: the grid rule keeps 256), 256 filtered states
+    # (16 starts x 16 targets), Newton length 2048
+    "c4_so_256": dict(radius_nm=4.5, n_req=(256, 256, 256), dx=0.71, so=True, nl=True, n_starts=16, ms=16,
+                      ncheby=2048, targets=(10, 6, -0.31, -0.24, -0.15, -0.06), builder="gpu"),
 }
