@@ -397,7 +397,9 @@ def main():
     mixed, fp64 = api.FBSE_PREC_MIXED, api.FBSE_PREC_FP64
 
     for _ in range(args.warmup):
+        log("warmup pass (mixed)")
         one_pass(S, mixed, refine=True)
+    log("timed mixed pass%s" % ("es" if args.steps > 1 else ""))
     stages = []
     ctx = S["ctx"]
     torch.cuda.synchronize()
@@ -417,6 +419,7 @@ def main():
     fp = None
     if not args.no_fp64:
         fst = []
+        log("fp64 pass")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         ev64, sig64, rank64 = one_pass(S, fp64, fst)

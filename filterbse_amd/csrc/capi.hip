@@ -47,6 +47,28 @@ int ensure_scratch(fbse_ctx *c, size_t b) { return grow(c, &c->d_scratch, &c->sc
 int ensure_scratch2(fbse_ctx *c, size_t b) { return grow(c, &c->d_scratch2, &c->scratch2_bytes, b); }
 int ensure_red(fbse_ctx *c, size_t b) { return grow(c, (void **)&c->d_red, &c->red_bytes, b); }
 
+// Releases the context's work buffers (the filter's ring, FFT work space,
+// partial Grams): they are allocated again on demand.
+static void release_work(fbse_ctx *c) {
+  (void)hipStreamSynchronize(c->stream);
+  void **bufs[] = {&c->d_scratch, &c->d_scratch2, &c->d_gram};
+  size_t *sizes[] = {&c->scratch_bytes, &c->scratch2_bytes, &c->gram_bytes};
+  for (int i = 0; i < 3; ++i) {
+    if (*bufs[i]) (void)hipFree(*bufs[i]);
+    *bufs[i] = nullptr;
+    *sizes[i] = 0;
+  }
+}
+
+// hipMalloc of a large block: when the device is full, the work buffers are
+// released (after a filter they hold its ring) and the allocation is retried.
+static hipError_t big_malloc(fbse_ctx *c, void **p, size_t bytes) {
+  if (hipMalloc(p, bytes) == hipSuccess) return hipSuccess;
+  (void)hipGetLastError();
+  release_work(c);
+  return hipMalloc(p, bytes);
+}
+
 static int ensure_host_red(fbse_ctx *c, size_t b) {
   if (b <= c->h_red_bytes) return FBSE_OK;
   if (c->h_red) hipHostFree(c->h_red);
@@ -471,7 +493,7 @@ extern "C" int fbse_profile_read(fbse_ctx *c, int kind, long *launches, double *
   return FBSE_OK;
 }
 extern "C" int fbse_set_window(fbse_ctx *c, int w) {
-  if (!c || w < 0 || w > 63) return FBSE_ERR_ARG;
+  if (!c || w < 0 || w > kMaxWindow) return FBSE_ERR_ARG;
   c->window = w;
   return FBSE_OK;
 }
@@ -495,11 +517,7 @@ extern "C" int fbse_trim(fbse_ctx *c) {
   if (!c) return FBSE_ERR_ARG;
   FBSE_HIP(c, hipSetDevice(c->device));
   FBSE_HIP(c, hipStreamSynchronize(c->stream));
-  if (c->d_scratch) FBSE_HIP(c, hipFree(c->d_scratch));
-  if (c->d_scratch2) FBSE_HIP(c, hipFree(c->d_scratch2));
-  if (c->d_gram) FBSE_HIP(c, hipFree(c->d_gram));
-  c->d_scratch = c->d_scratch2 = c->d_gram = nullptr;
-  c->scratch_bytes = c->scratch2_bytes = c->gram_bytes = 0;
+  release_work(c);
   io_release(c);
   return FBSE_OK;
 }
@@ -921,20 +939,22 @@ template <typename T> struct FilterRun {
 };
 
 // window depth: forced, or as deep as ~40% of free memory allows for all the
-// runs' rings together (<= 63)
-static int window_depth(int forced, size_t ring_bytes_per_slot) {
-  if (forced > 0) return std::min(forced, 63);
+// runs' rings together (<= kMaxWindow)
+static int window_depth(int forced, size_t ring_bytes_per_slot, size_t held) {
+  if (forced > 0) return std::min(forced, kMaxWindow);
   size_t fr = 0, tot = 0;
   (void)hipMemGetInfo(&fr, &tot);
+  // rings reuse the contexts' scratch buffers: what they hold counts as free
+  fr += held;
   long kk = (long)((0.4 * (double)fr) / (double)ring_bytes_per_slot) - 2;
-  return (int)std::max(1L, std::min(63L, kk));
+  return (int)std::max(1L, std::min((long)kMaxWindow, kk));
 }
 
 template <typename T>
 static int filter_impl(fbse_ctx *c, double *d_psi_rank, long n_starts, long ms, long M, const double *an,
                        const double *zn, double Vmin, double dE_1) {
   FilterRun<T> r(c, d_psi_rank, n_starts, ms, M, an, zn, Vmin, dE_1);
-  int rc = r.prepare(window_depth(c->window, r.slot_bytes()));
+  int rc = r.prepare(window_depth(c->window, r.slot_bytes(), c->scratch_bytes));
   while (!rc && r.in_window0()) rc = r.step();  // eager (runs the autotuners)
   if (!rc) rc = r.capture();
   while (!rc && r.replays_left()) rc = r.replay();
@@ -951,14 +971,15 @@ static int filter_many_impl(int n, fbse_ctx *const *ctx, double *const *psi, con
                             const double *const *an, const double *const *zn, const double *Vmin,
                             const double *dE_1) {
   std::vector<std::unique_ptr<FilterRun<T>>> runs;
-  size_t ring_slot = 0;
+  size_t ring_slot = 0, held = 0;
   int forced = 0;
   for (int d = 0; d < n; ++d) {
     runs.emplace_back(new FilterRun<T>(ctx[d], psi[d], n_starts[d], ms, M, an[d], zn[d], Vmin[d], dE_1[d]));
     ring_slot += runs.back()->slot_bytes();
+    held += ctx[d]->scratch_bytes;
     if (ctx[d]->window > 0) forced = forced ? std::min(forced, ctx[d]->window) : ctx[d]->window;
   }
-  const int K = window_depth(forced, ring_slot);
+  const int K = window_depth(forced, ring_slot, held);
   int rc = FBSE_OK, bad = -1;
   auto fail = [&](int d, int r) {
     if (r && !rc) {
@@ -1305,7 +1326,7 @@ static int ortho_dev_impl(fbse_ctx *c, double *A, long n, double svd_eps, long *
   std::vector<double> mu;
   if (ns > 0) {
     double *Bs = nullptr;
-    if (hipMalloc(&Bs, (size_t)la.m * ns * es) != hipSuccess) {
+    if (big_malloc(c, (void **)&Bs, (size_t)la.m * ns * es) != hipSuccess) {
       cleanup();
       return set_err(c, FBSE_ERR_NOMEM, "ortho: %ld x %ld block of small singular directions", la.m, ns);
     }
@@ -1456,10 +1477,14 @@ static int rr_dev_impl(fbse_ctx *c, double *U, long n, double *eval) {
   // H U is held for as many columns as memory allows: U^H (H U) over a narrow
   // block of columns runs far below the GEMM peak, so one wide product is used
   // whenever it fits (1024 states of a 96^3 spinor grid take 29 GB).
-  const long ch = std::min(n, states_that_fit(c, (size_t)sl * 8, 0.5));
+  long ch = std::min(n, states_that_fit(c, (size_t)sl * 8, 0.5));
+  if (ch < n && c->scratch_bytes) {  // the filter's ring is no longer needed: make room for one wide product
+    release_work(c);
+    ch = std::min(n, states_that_fit(c, (size_t)sl * 8, 0.5));
+  }
   FBSE_HIP(c, hipMalloc(&Hs, (size_t)n * n * es));
   FBSE_HIP(c, hipMalloc(&w, (size_t)n * 8));
-  FBSE_HIP(c, hipMalloc(&HU, (size_t)ch * sl * 8));
+  FBSE_HIP(c, big_malloc(c, (void **)&HU, (size_t)ch * sl * 8));
   if (ch == n) {
     // all of H U at once: U^H (H U) is Hermitian, its upper blocks suffice
     rc = apply_h_dev_impl(c, U, HU, n);
@@ -1560,12 +1585,13 @@ extern "C" int fbse_refine_dev(fbse_ctx *c, double *d_psi, long n_ritz, const do
   std::vector<double> theta(k);
   for (long j = 0; j < k; ++j) theta[j] = eval[sel[j]];
   double *d_u = nullptr, *d_hu = nullptr, *d_th = nullptr;
-  FBSE_HIP(c, hipMalloc(&d_hu, (size_t)k * sl * 8));
+  FBSE_HIP(c, big_malloc(c, (void **)&d_hu, (size_t)k * sl * 8));
   FBSE_HIP(c, hipMalloc(&d_th, (size_t)k * 8));
   int rc = FBSE_OK;
   if (full) {
     // gather the selected vectors into a work block (they stay in place too)
-    if (hipMalloc(&d_u, (size_t)k * sl * 8) != hipSuccess) rc = set_err(c, FBSE_ERR_NOMEM, "refine work block");
+    if (big_malloc(c, (void **)&d_u, (size_t)k * sl * 8) != hipSuccess)
+      rc = set_err(c, FBSE_ERR_NOMEM, "refine work block");
     for (long j = 0; j < k && !rc; ++j) {
       hipError_t e = hipMemcpyAsync(d_u + j * sl, d_psi + sel[j] * sl, sl * 8, hipMemcpyDeviceToDevice, c->stream);
       if (e != hipSuccess) rc = set_err(c, FBSE_ERR_HIP, "copy: %s", hipGetErrorString(e));

@@ -14,6 +14,9 @@
 
 namespace fbse {
 
+// deepest Newton accumulation window K (ring of K+1 slots)
+constexpr int kMaxWindow = 63;
+
 template <typename T> struct cplx_of;
 template <> struct cplx_of<double> { using type = double2; };
 template <> struct cplx_of<float> { using type = float2; };

@@ -533,7 +533,7 @@ int axpby_states(fbse_ctx *c, const double *x, double *y, long n, double a, doub
 // packed = 1: real states packed in pairs, only an.re is used (an.im is
 // identically 0 for the Gaussian filter, coeff.c:89-105).
 // ---------------------------------------------------------------------------
-constexpr int kMaxWin = 64;
+constexpr int kMaxWin = kMaxWindow + 1;  // ring slots one accumulation reads
 constexpr int kMaxMs = 16;
 
 constexpr int kAccGroup = 4;  // ring slots loaded per group in accumulate_kernel
