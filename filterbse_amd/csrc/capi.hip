@@ -260,6 +260,15 @@ static int build_projectors(fbse_ctx *c, const fbse_proj_desc *pd) {
     P.color_off[ncolors] = (int)order.size();
     if ((rc = upload(c, (void **)&P.color_atoms, order.data(), order.size() * sizeof(int)))) return rc;
     F.color_atoms = P.color_atoms;
+    // z-y-x order of the middle sphere point (points are stored in the scan
+    // order z, y, x, so the middle one sits near the atom)
+    std::vector<std::pair<int, int>> key(na);
+    for (long a = 0; a < na; ++a) key[a] = {cnt[a] ? jx[off[a] + cnt[a] / 2] : 0, (int)a};
+    std::sort(key.begin(), key.end());
+    std::vector<int> sp(na);
+    for (long a = 0; a < na; ++a) sp[a] = key[a].second;
+    if ((rc = upload(c, (void **)&P.spatial_atoms, sp.data(), sp.size() * sizeof(int)))) return rc;
+    F.spatial_atoms = P.spatial_atoms;
     F.color_off = P.color_off;
   }
   c->has_proj = (pd->so || pd->nl) && npts > 0;
@@ -445,7 +454,7 @@ extern "C" void fbse_destroy(fbse_ctx *c) {
   void *ptrs[] = {c->d_vloc, c->d_vloc_f, c->d_kx, c->d_ky, c->d_kz, c->d_ksqr, c->d_kx_f, c->d_ky_f, c->d_kz_f,
                   c->d_ksqr_f, c->d_scratch, c->d_scratch2, c->d_red, c->pj.a_off, c->pj.a_cnt, c->pj.jxyz,
                   c->pj.sign, c->pj.proj, c->pj.nlproj, c->pj.y1, c->pj.ls, c->pj_f.proj, c->pj_f.nlproj,
-                  c->pj_f.y1, c->pj_f.ls, c->pj.color_atoms, c->d_step, c->d_rec_off, c->d_rec_pa,
+                  c->pj_f.y1, c->pj_f.ls, c->pj.color_atoms, c->pj.spatial_atoms, c->d_step, c->d_rec_off, c->d_rec_pa,
                   c->d_rec_val, c->d_rec_val_f, c->d_pjd, c->d_gram};
   for (void *p : ptrs)
     if (p) hipFree(p);

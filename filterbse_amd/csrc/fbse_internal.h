@@ -84,6 +84,9 @@ template <typename T> struct ProjTables {
   // atoms grouped by colour: atoms of one colour share no grid point
   int *color_atoms = nullptr;          // atom indices, colour by colour
   std::vector<int> color_off;          // host: start of each colour in color_atoms
+  // every atom in the z-y-x order of its sphere's middle point: work-groups
+  // that run together then gather overlapping spheres (shared cache lines)
+  int *spatial_atoms = nullptr;
 };
 
 template <typename T> struct ProjArgs {

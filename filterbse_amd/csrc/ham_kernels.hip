@@ -43,8 +43,12 @@ template <typename T, int MAXP, bool EXPAND = true>
 __global__ __launch_bounds__(256) void proj_color_kernel(ProjArgs<T> P) {
   using C = cplx_t<T>;
   __shared__ C s_d[kProjRows][kPjCols];
-  const int atom = P.atoms[blockIdx.x];
-  const int col0 = blockIdx.y * kPjCols;
+  // expansion launches: atoms along x; projection-only launch: column chunks
+  // along x, so that the work-groups one XCD runs together (the dispatcher
+  // deals them round-robin over the 8 XCDs) take neighbouring atoms of the
+  // same columns and share the cache lines of overlapping spheres
+  const int atom = P.atoms[EXPAND ? blockIdx.x : blockIdx.y];
+  const int col0 = (EXPAND ? blockIdx.y : blockIdx.x) * kPjCols;
   const int lane = threadIdx.x % kPjLanes;   // point lane (fastest: consecutive points)
   const int cc = threadIdx.x / kPjLanes;     // column within the chunk
   const int col = col0 + cc;
@@ -60,9 +64,21 @@ __global__ __launch_bounds__(256) void proj_color_kernel(ProjArgs<T> P) {
 #pragma unroll
   for (int r = 0; r < NR; ++r) acc[r].x = acc[r].y = 0;
   if (live) {
-    for (int g = lane; g < cnt; g += kPjLanes) {
-      const int idx = off + g;
-      const C v = psi[P.jxyz[idx]];
+    // four points per lane in flight: the index loads, then the gathers
+    constexpr int U = 4;
+    for (int g0 = lane; g0 < cnt; g0 += U * kPjLanes) {
+      int gi[U];
+      C vv[U];
+#pragma unroll
+      for (int q = 0; q < U; ++q) gi[q] = g0 + q * kPjLanes < cnt ? P.jxyz[off + g0 + q * kPjLanes] : -1;
+#pragma unroll
+      for (int q = 0; q < U; ++q)
+        if (gi[q] >= 0) vv[q] = psi[gi[q]];
+#pragma unroll
+      for (int q = 0; q < U; ++q) {
+      if (gi[q] < 0) continue;
+      const int idx = off + g0 + q * kPjLanes;
+      const C v = vv[q];
       C u[3];
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
@@ -88,6 +104,7 @@ __global__ __launch_bounds__(256) void proj_color_kernel(ProjArgs<T> P) {
             acc[3 * MAXP + p * 3 + m].y += w * u[m].y;
           }
         }
+      }
       }
     }
   }
@@ -258,7 +275,7 @@ int proj_apply(fbse_ctx *c, const cplx_t<T> *psi_in, cplx_t<T> *psi_out, long ns
 bool proj_fusable(const fbse_ctx *c) {
   if (!c->has_proj || !c->fuse_proj || !c->d_rec_off) return false;
   const int nso = c->pj.so ? c->pj.n_so : 0, nnl = c->pj.nl ? c->pj.n_nl : 0;
-  return nso <= 1 && nnl <= 1;
+  return nso <= 1 && nnl <= 1 && c->pj.n_atoms <= 65535;  // the projection launch puts atoms along grid y
 }
 
 template <typename T>
@@ -290,12 +307,13 @@ int proj_project(fbse_ctx *c, const cplx_t<T> *psi_in, long nstates, double a1, 
   P.nspin = (int)c->nspin;
   P.ncols = (int)(nstates * c->nspin);
   P.scale = (T)(c->dv * a1);
-  P.atoms = pt->color_atoms;  // every atom (any order: nothing is written to the grid)
+  P.atoms = pt->spatial_atoms;  // every atom (any order works: nothing is written to the grid)
   if (P.n_atoms == 0 || P.ncols == 0) return FBSE_OK;
   hipEvent_t ev = nullptr;
   prof_begin(c, 2, &ev);
   const int nchunk = (P.ncols + kPjCols - 1) / kPjCols;
-  hipLaunchKernelGGL((proj_color_kernel<T, 1, false>), dim3(P.n_atoms, nchunk), dim3(256), 0, c->stream, P);
+  if (P.n_atoms > 65535) return set_err(c, FBSE_ERR_UNSUPPORTED, "%d atoms in one projection launch", P.n_atoms);
+  hipLaunchKernelGGL((proj_color_kernel<T, 1, false>), dim3(nchunk, P.n_atoms), dim3(256), 0, c->stream, P);
   FBSE_HIP(c, hipGetLastError());
   prof_end(c, 2, ev);
   return FBSE_OK;
