@@ -121,6 +121,38 @@ void prof_collect(fbse_ctx *c) {
   c->ev_pending.clear();
 }
 
+// Per-axis passes: for a length with prime factors 2, 3, 5 only, the fewest
+// passes over the register butterflies (composite radices included), larger
+// radices first; otherwise the schedule below (other primes by direct DFT).
+static void factorize(int n, AxisPlan &pl);
+static void axis_schedule(int n, AxisPlan &pl) {
+  static const int rad[] = {16, 15, 12, 10, 9, 8, 6, 5, 4, 3, 2};
+  std::vector<int> passes(n + 1, 1 << 20), maxr(n + 1, 0), pick(n + 1, 0);
+  passes[1] = 0;
+  for (int m = 2; m <= n; ++m) {
+    if (n % m) continue;
+    for (int r : rad) {
+      if (m % r || passes[m / r] >= (1 << 20)) continue;
+      const int pp = passes[m / r] + 1, mr = std::max(maxr[m / r], r);
+      if (pp < passes[m] || (pp == passes[m] && mr < maxr[m])) {
+        passes[m] = pp;
+        maxr[m] = mr;
+        pick[m] = r;
+      }
+    }
+  }
+  if (n > 1 && passes[n] >= (1 << 20)) {
+    factorize(n, pl);
+    return;
+  }
+  std::vector<int> fs;
+  for (int m = n; m > 1; m /= pick[m]) fs.push_back(pick[m]);
+  std::sort(fs.begin(), fs.end(), [](int a, int b) { return a > b; });
+  pl.n = n;
+  pl.nf = (int)std::min<size_t>(fs.size(), kMaxFactors);
+  for (int i = 0; i < pl.nf; ++i) pl.f[i] = fs[i];
+}
+
 // Radix schedule: 8s first, then 4, 2, 3, 5, then any remaining primes.
 static void factorize(int n, AxisPlan &pl) {
   pl.n = n;
@@ -418,6 +450,7 @@ extern "C" int fbse_create(const fbse_grid_desc *g, const fbse_proj_desc *pd, in
   for (int ax = 0; ax < 3; ++ax) {
     const int n = (int)dims[ax];
     factorize(n, c->plan[ax]);
+    axis_schedule(n, c->aplan[ax]);
     // tile shape: x rows fill ~2048 elements; y/z take 16 consecutive inner
     // indices (256-byte rows for FP64)
     if (ax == 0) c->plan[ax].lines = std::max(1, std::min(64, 2048 / n));

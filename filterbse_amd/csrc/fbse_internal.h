@@ -133,6 +133,7 @@ struct fbse_ctx {
 
   // twiddles per axis
   fbse::AxisPlan plan[3];
+  fbse::AxisPlan aplan[3];      // per-axis passes' radix schedule (composite radices allowed)
   double2 *d_tw[3] = {nullptr, nullptr, nullptr};
   float2 *d_tw_f[3] = {nullptr, nullptr, nullptr};
 

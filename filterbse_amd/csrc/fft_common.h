@@ -210,4 +210,31 @@ template <typename T> struct Bfly<T, 12> : BflyPQ<T, 4, 3> {};
 template <typename T> struct Bfly<T, 15> : BflyPQ<T, 3, 5> {};
 template <typename T> struct Bfly<T, 16> : BflyPQ<T, 4, 4> {};
 
+// x[r] *= w^r (r = 1 .. R-1) with w = tw[kk*tstep] (conjugated when inv): w
+// and w^4 are read from the length-n table, w^(4q + j) = w^(4q) w^j with
+// w^(4q) a product of reads (at most three roundings per power).
+template <typename T, int R>
+__device__ __forceinline__ void twiddle_powers(cplx_t<T> *x, const cplx_t<T> *tw, int kk, int tstep, bool inv) {
+  using C = cplx_t<T>;
+  C w1 = tw[kk * tstep];
+  if (inv) w1 = cconj(w1);
+  const C w2 = cmul(w1, w1), w3 = cmul(w2, w1);
+  x[1] = cmul(x[1], w1);
+  if (R > 2) x[2] = cmul(x[2], w2);
+  if (R > 3) x[3] = cmul(x[3], w3);
+  if (R > 4) {
+    C w4 = tw[4 * kk * tstep];
+    if (inv) w4 = cconj(w4);
+    C base = w4;
+#pragma unroll
+    for (int q = 1; 4 * q < R; ++q) {
+      if (q > 1) base = cmul(base, w4);
+      x[4 * q] = cmul(x[4 * q], base);
+      if (4 * q + 1 < R) x[4 * q + 1] = cmul(x[4 * q + 1], cmul(base, w1));
+      if (4 * q + 2 < R) x[4 * q + 2] = cmul(x[4 * q + 2], cmul(base, w2));
+      if (4 * q + 3 < R) x[4 * q + 3] = cmul(x[4 * q + 3], cmul(base, w3));
+    }
+  }
+}
+
 }  // namespace fbse

@@ -22,36 +22,45 @@
 
 namespace fbse {
 
-// One Stockham pass of compile-time radix R over all lines of the tile.
+// q = i / d, r = i % d for 0 <= i < 2^22, 1 <= d: float reciprocal, one correction
+__device__ __forceinline__ void divmod_small(int i, int d, float inv_d, int &q, int &r) {
+  q = (int)((float)i * inv_d);
+  r = i - q * d;
+  if (r >= d) {
+    r -= d;
+    ++q;
+  }
+  if (r < 0) {
+    r += d;
+    --q;
+  }
+}
+
+// One Stockham pass of compile-time radix R over all lines of the tile (one
+// butterfly per loop iteration: nothing is held across a barrier).
 template <typename T, int R>
-__device__ __forceinline__ void pass_fixed(const cplx_t<T> *in, cplx_t<T> *out, const cplx_t<T> *tw, int n,
-                                           int p, int LS, int nlines, bool inv) {
+__device__ __forceinline__ void pass_fixed(const cplx_t<T> *in, cplx_t<T> *out, const cplx_t<T> *tw,
+                                                     int n, int p, int LS, int nlines, bool inv) {
   using C = cplx_t<T>;
   const int m = n / R;
   const int tstep = n / (p * R);
   const int nb = m * nlines;
+  const float inv_m = 1.0f / (float)m, inv_p = 1.0f / (float)p;
   const T sgn = inv ? (T)1 : (T)-1;
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-    const int l = b / m;
-    const int i = b - l * m;
-    const int k = i % p;
+    int l, i, q0, k;
+    divmod_small(b, m, inv_m, l, i);
+    divmod_small(i, p, inv_p, q0, k);
     const C *src = in + l * LS;
     C *dst = out + l * LS;
     C x[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      C v = src[i + r * m];
-      if (r > 0 && k > 0) {
-        C w = tw[r * k * tstep];
-        if (inv) w = cconj(w);
-        v = cmul(v, w);
-      }
-      x[r] = v;
-    }
+    for (int r = 0; r < R; ++r) x[r] = src[i + r * m];
+    if (k > 0) twiddle_powers<T, R>(x, tw, k, tstep, inv);
     Bfly<T, R>::run(x, sgn);
     const int j = (i - k) * R + k;
 #pragma unroll
-    for (int q = 0; q < R; ++q) dst[j + q * p] = x[q];
+    for (int q = 0; q < R; ++q) dst[j + q * p] = x[Bfly<T, R>::perm(q)];
   }
 }
 
@@ -103,7 +112,13 @@ __device__ cplx_t<T> *run_passes(cplx_t<T> *a, cplx_t<T> *b, const cplx_t<T> *tw
       case 3: pass_fixed<T, 3>(a, b, tw, A.n, p, LS, nlines, inv); break;
       case 4: pass_fixed<T, 4>(a, b, tw, A.n, p, LS, nlines, inv); break;
       case 5: pass_fixed<T, 5>(a, b, tw, A.n, p, LS, nlines, inv); break;
+      case 6: pass_fixed<T, 6>(a, b, tw, A.n, p, LS, nlines, inv); break;
       case 8: pass_fixed<T, 8>(a, b, tw, A.n, p, LS, nlines, inv); break;
+      case 9: pass_fixed<T, 9>(a, b, tw, A.n, p, LS, nlines, inv); break;
+      case 10: pass_fixed<T, 10>(a, b, tw, A.n, p, LS, nlines, inv); break;
+      case 12: pass_fixed<T, 12>(a, b, tw, A.n, p, LS, nlines, inv); break;
+      case 15: pass_fixed<T, 15>(a, b, tw, A.n, p, LS, nlines, inv); break;
+      case 16: pass_fixed<T, 16>(a, b, tw, A.n, p, LS, nlines, inv); break;
       default: pass_generic<T>(a, b, tw, A.n, R, p, LS, nlines, inv); break;
     }
     __syncthreads();
@@ -133,6 +148,8 @@ __global__ __launch_bounds__(kFftThreads) void fft_axis_kernel(AxisArgs<T> A) {
   C *buf0 = reinterpret_cast<C *>(smem_raw);
   C *buf1 = buf0 + L * LS;
   C *tw = buf1 + L * LS;
+  T *kzl = reinterpret_cast<T *>(tw + n);  // kinetic pass: k^2 of z, then x+y per line
+  T *kxyl = kzl + n;
 
   for (int t = threadIdx.x; t < n; t += blockDim.x) tw[t] = A.tw[t];
 
@@ -153,6 +170,15 @@ __global__ __launch_bounds__(kFftThreads) void fft_axis_kernel(AxisArgs<T> A) {
   }
 
   const int tot = L * n;
+  if ((KIND == kPassKin || KIND == kPassPrec) && !A.ksqr) {
+    // separable k^2 into LDS now, so no global load sits between the transforms
+    for (int t = threadIdx.x; t < n; t += blockDim.x) kzl[t] = A.kz[t];
+    for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+      const long inner = i0 + l;
+      const long jy = inner / A.nx, jx = inner - jy * A.nx;
+      kxyl[l] = A.kx[jx] + A.ky[jy];
+    }
+  }
   if (A.xaxis) {
     const int totv = nl * n;
     for (int e0 = threadIdx.x; e0 < totv; e0 += kFftThreads * NB) {
@@ -205,15 +231,15 @@ __global__ __launch_bounds__(kFftThreads) void fft_axis_kernel(AxisArgs<T> A) {
     res = run_passes<T>(buf0, buf1, tw, A, LS, nl, true);
   } else {  // kinetic (x k^2) or preconditioner (x g(k^2)): forward, scale, backward
     res = run_passes<T>(buf0, buf1, tw, A, LS, nl, false);
+    const float inv_n = 1.0f / (float)n;
     for (int e = threadIdx.x; e < nl * n; e += blockDim.x) {
-      const int l = e / n, p = e - l * n;
-      const long inner = i0 + l;
+      int l, p;
+      divmod_small(e, n, inv_n, l, p);
       T k2;
       if (A.ksqr) {
-        k2 = A.ksqr[(long)p * A.inner + inner];
+        k2 = A.ksqr[(long)p * A.inner + i0 + l];
       } else {
-        const long jy = inner / A.nx, jx = inner - jy * A.nx;
-        k2 = A.kx[jx] + A.ky[jy] + A.kz[p];
+        k2 = kxyl[l] + kzl[p];
         if (k2 > A.kcap) k2 = A.kcap;
       }
       if (KIND == kPassPrec) k2 = A.pa / ((T)1 + k2 * A.pb);
@@ -271,10 +297,23 @@ __global__ __launch_bounds__(kFftThreads) void fft_axis_kernel(AxisArgs<T> A) {
   }
 }
 
-// LDS of a two-buffer tile: 2 x lines x (n | 1) elements plus n twiddles
+// LDS of a two-buffer tile: 2 x lines x (n | 1) elements, n twiddles, and the
+// kinetic pass's real k^2 tables (n + lines)
 static size_t axis_lds_bytes(int n, int lines, size_t elem) {
   const int LS = n | 1;
-  return (size_t)(2 * lines * LS + n) * elem;
+  return (size_t)(2 * lines * LS + n) * elem + (size_t)(n + lines) * (elem / 2);
+}
+
+// LDS a per-axis tile may take (dynamic LDS above 64 KiB is enabled per kernel)
+constexpr size_t kAxisLdsCap = 160 * 1024;
+
+template <typename T, int KIND> static void axis_lds_limit(int device) {
+  static bool done[64] = {};
+  if (device < 0 || device >= 64 || !done[device]) {
+    (void)hipFuncSetAttribute((const void *)fft_axis_kernel<T, KIND>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kAxisLdsCap);
+    if (device >= 0 && device < 64) done[device] = true;
+  }
 }
 
 template <typename T>
@@ -301,9 +340,10 @@ static int fft_pass_lines(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src,
   }
   A.kcap = (T)c->kcap;
   A.nx = (int)c->nx;
+  const AxisPlan &sch = c->aplan[axis];  // the per-axis radix schedule
   A.n = pl.n;
-  A.nf = pl.nf;
-  for (int i = 0; i < pl.nf; ++i) A.f[i] = pl.f[i];
+  A.nf = sch.nf;
+  for (int i = 0; i < sch.nf; ++i) A.f[i] = sch.f[i];
   // lines per tile: the requested value, clamped so the tile never exceeds the data
   int lines = lines_req > 0 ? lines_req : pl.lines;
   A.psi_in = psi_in;
@@ -334,8 +374,8 @@ static int fft_pass_lines(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src,
     const long nouter = axis == 1 ? nbatch * c->nz : nbatch;
     ntiles = nouter * A.tiles_per_outer;
   }
-  // keep the tile within 64 KiB of LDS
-  while (lines > 1 && axis_lds_bytes(pl.n, lines, sizeof(cplx_t<T>)) > 65536) --lines;
+  // keep the tile within the LDS a work-group may take
+  while (lines > 1 && axis_lds_bytes(pl.n, lines, sizeof(cplx_t<T>)) > kAxisLdsCap) --lines;
   if (axis == 0) {
     ntiles = (A.total_lines + lines - 1) / lines;
   } else {
@@ -344,24 +384,29 @@ static int fft_pass_lines(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src,
   }
   A.lines = lines;
   const size_t lds = axis_lds_bytes(pl.n, lines, sizeof(cplx_t<T>));
-  if (lds > 65536) return set_err(c, FBSE_ERR_UNSUPPORTED, "axis length %d needs %zu bytes of LDS", pl.n, lds);
+  if (lds > kAxisLdsCap) return set_err(c, FBSE_ERR_UNSUPPORTED, "axis length %d needs %zu bytes of LDS", pl.n, lds);
   hipEvent_t ev = nullptr;
   const int pk = (kind == kPassBwdCombine) ? 1 : 0;
   prof_begin(c, pk, &ev);
   switch (kind) {
     case kPassFwd:
+      axis_lds_limit<T, kPassFwd>(c->device);
       hipLaunchKernelGGL((fft_axis_kernel<T, kPassFwd>), dim3(ntiles), dim3(kFftThreads), lds, c->stream, A);
       break;
     case kPassBwd:
+      axis_lds_limit<T, kPassBwd>(c->device);
       hipLaunchKernelGGL((fft_axis_kernel<T, kPassBwd>), dim3(ntiles), dim3(kFftThreads), lds, c->stream, A);
       break;
     case kPassKin:
+      axis_lds_limit<T, kPassKin>(c->device);
       hipLaunchKernelGGL((fft_axis_kernel<T, kPassKin>), dim3(ntiles), dim3(kFftThreads), lds, c->stream, A);
       break;
     case kPassPrec:
+      axis_lds_limit<T, kPassPrec>(c->device);
       hipLaunchKernelGGL((fft_axis_kernel<T, kPassPrec>), dim3(ntiles), dim3(kFftThreads), lds, c->stream, A);
       break;
     default:
+      axis_lds_limit<T, kPassBwdCombine>(c->device);
       hipLaunchKernelGGL((fft_axis_kernel<T, kPassBwdCombine>), dim3(ntiles), dim3(kFftThreads), lds, c->stream,
                          A);
       break;

@@ -114,29 +114,7 @@ __device__ __attribute__((noinline)) void tile_radix_pass(cplx_t<T> *buf, const 
       const C *src = buf + lbase + i * estride;
 #pragma unroll
       for (int r = 0; r < R; ++r) x[k][r] = src[r * m * estride];
-      if (kk > 0) {
-        // twiddles w^r, w = tw[kk*tstep]: w and w^4 are read, w^(4q + j) =
-        // w^(4q) w^j with w^(4q) a product of reads (at most three roundings)
-        C w1 = tw[kk * tstep];
-        if (inv) w1 = cconj(w1);
-        const C w2 = cmul(w1, w1), w3 = cmul(w2, w1);
-        x[k][1] = cmul(x[k][1], w1);
-        if (R > 2) x[k][2] = cmul(x[k][2], w2);
-        if (R > 3) x[k][3] = cmul(x[k][3], w3);
-        if (R > 4) {
-          C w4 = tw[4 * kk * tstep];
-          if (inv) w4 = cconj(w4);
-          C base = w4;
-#pragma unroll
-          for (int q = 1; 4 * q < R; ++q) {
-            if (q > 1) base = cmul(base, w4);
-            x[k][4 * q] = cmul(x[k][4 * q], base);
-            if (4 * q + 1 < R) x[k][4 * q + 1] = cmul(x[k][4 * q + 1], cmul(base, w1));
-            if (4 * q + 2 < R) x[k][4 * q + 2] = cmul(x[k][4 * q + 2], cmul(base, w2));
-            if (4 * q + 3 < R) x[k][4 * q + 3] = cmul(x[k][4 * q + 3], cmul(base, w3));
-          }
-        }
-      }
+      if (kk > 0) twiddle_powers<T, R>(x[k], tw, kk, tstep, inv);
       dsti[k] = lbase + ((i - kk) * R + kk) * estride;
     }
   }
