@@ -117,6 +117,7 @@ GPU_SIGNATURES = {
     "fbse_read_start_states": (C.c_int, [C.c_void_p, C.c_char_p, C.c_long, C.c_long, C.c_int, C.c_void_p,
                                          C.c_long, C.c_long, LP]),
     "fbse_set_io_chunk": (C.c_int, [C.c_void_p, C.c_size_t]),
+    "fbse_set_slab_split": (C.c_int, [C.c_void_p, C.c_int]),
     "fbse_set_refine_basis": (C.c_int, [C.c_void_p, C.c_int]),
     "fbse_dev_alloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     "fbse_dev_free": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -461,6 +462,10 @@ class Context:
         self._check(gpu().fbse_read_start_states(self.ctx, os.fsencode(path), start, end, n_every,
                                                  C.c_void_p(d_psi_rank), n_starts, ms, C.byref(n)))
         return n.value
+
+    def set_slab_split(self, split):
+        """z-slab tiles per slab along x (0 = autotune)."""
+        self._check(gpu().fbse_set_slab_split(self.ctx, split))
 
     def set_io_chunk(self, nbytes):
         self._check(gpu().fbse_set_io_chunk(self.ctx, nbytes))

@@ -545,6 +545,11 @@ extern "C" int fbse_set_fft_kernel(fbse_ctx *c, int variant, int lines) {
   c->force_lines = lines;
   return FBSE_OK;
 }
+extern "C" int fbse_set_slab_split(fbse_ctx *c, int split) {
+  if (!c || split < 0 || split > 16) return FBSE_ERR_ARG;
+  c->force_zsplit = split;
+  return FBSE_OK;
+}
 extern "C" int fbse_set_graph(fbse_ctx *c, int enable) {
   if (!c || enable < 0 || enable > 1) return FBSE_ERR_ARG;
   c->use_graph = enable;

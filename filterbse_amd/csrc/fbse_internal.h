@@ -183,7 +183,8 @@ struct fbse_ctx {
   // 0 = not tuned yet.
   int tune_lines[2][2][3][3] = {};
   int tile_variant[2] = {-1, -1};  // plane/slab kernel instance per precision (-1 = not tuned)
-  int n_cu = 0;                 // compute units (resident-grid launches), 0 until queried
+  int zsplit[2] = {0, 0};       // z-slab tiles per slab along x, per precision (0 = not tuned)
+  int force_zsplit = 0;         // fbse_set_slab_split (0 = autotune)
 
   // Newton-window graphs: the Newton shift a0 of step (base + off) is read
   // from a device table while a window is captured (dyn_a0_tab != nullptr)

@@ -46,6 +46,7 @@ template <typename T> struct TileArgs {
   long tile_stride;            // global elements between tiles of one batch item
   long item_stride;            // global elements between batch items (ngrid)
   int tiles_per_item;
+  int col_split;               // tiles per slab along x (z passes; 1 elsewhere)
   long nbatch;                 // batch items (state x spin grids)
   // z passes: k2 = min(kx[x] + ky[y] + kz[z], kcap) when separable, else ksqr[x + nx*y + nx*ny*z]
   const T *kx, *ky, *kz, *ksqr;
@@ -175,9 +176,12 @@ __device__ __forceinline__ void tile_body(const TileArgs<T> &A) {
   // XCD's L2: the dispatcher deals work-groups round-robin over the 8 XCDs)
   // transform the same plane of different states and read the same plane of
   // V_loc (or k^2) once per XCD
-  const int w = (int)(blockIdx.x / A.nbatch);
-  const long item = blockIdx.x - (long)w * A.nbatch;
-  const long base = item * A.item_stride + (long)w * A.tile_stride;
+  const int wt = (int)(blockIdx.x / A.nbatch);
+  const long item = blockIdx.x - (long)wt * A.nbatch;
+  // z passes may split a slab into col_split tiles of nc columns (x ranges)
+  const int w = A.col_split > 1 ? wt / A.col_split : wt;
+  const int xs = wt - w * A.col_split;
+  const long base = item * A.item_stride + (long)w * A.tile_stride + (long)xs * nc;
   // blockDim.x is a multiple of nc: thread -> column c0, rows r0, r0 + dr, ...
   const int c0 = threadIdx.x % nc, r0 = threadIdx.x / nc, dr = NT / nc;
   const cplx_t<T> *src = A.src + base + c0;
@@ -207,7 +211,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T> &A) {
   T *kzl = kxl + nc;
   const bool ksep = (KIND == kTileKinZ || KIND == kTilePrecZ) && !A.ksqr;
   if (ksep) {
-    for (int t = threadIdx.x; t < nc; t += NT) kxl[t] = A.kx[t];
+    for (int t = threadIdx.x; t < nc; t += NT) kxl[t] = A.kx[xs * nc + t];
     for (int t = threadIdx.x; t < nr; t += NT) kzl[t] = A.kz[t];
   }
   __syncthreads();
@@ -221,7 +225,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T> &A) {
   } else {  // z columns: forward, scale by k^2 (or the preconditioner), backward
     tile_fft<T, E>(buf, twr, nr, A.nfr, A.fr, ls, 1, nc, false);
     const T kxy = ksep ? kxl[c0] + A.ky[w] : (T)0;
-    const T *ksq = ksep ? nullptr : A.ksqr + c0 + (long)w * nc;
+    const T *ksq = ksep ? nullptr : A.ksqr + xs * nc + c0 + (long)w * nc * A.col_split;
     for (int r = r0; r < nr; r += dr) {
       T k2;
       if (ksep) {
@@ -478,14 +482,18 @@ template <typename T, int KIND> static void set_lds_limit(int device, int v) {
 template <typename T>
 static int tile_pass_v(fbse_ctx *c, int v, int kind, const cplx_t<T> *src, cplx_t<T> *dst, long nbatch,
                        const cplx_t<T> *psi_in, T a1, T a0, const cplx_t<T> *pjd = nullptr,
-                       cplx_t<T> *fwd_dst = nullptr) {
+                       cplx_t<T> *fwd_dst = nullptr, int zsplit = 1) {
   TileArgs<T> A{};
   A.src = src;
   A.dst = dst;
-  A.nc = (int)c->nx;
+  const bool xy = kind == kTileFwdXY || kind == kTileBwdXY || kind == kTileBwdXYCombine;
+  // z passes transform columns only: a slab may be cut into zsplit x ranges
+  A.col_split = xy ? 1 : zsplit;
+  if (A.col_split < 1 || c->nx % A.col_split)
+    return set_err(c, FBSE_ERR_ARG, "slab split %d of %ld columns", A.col_split, c->nx);
+  A.nc = (int)(c->nx / A.col_split);
   A.ls = tile_pitch(A.nc);
   A.item_stride = c->ngrid;
-  const bool xy = kind == kTileFwdXY || kind == kTileBwdXY || kind == kTileBwdXYCombine;
   const AxisPlan &pc = c->plan[0], &pr = c->plan[xy ? 1 : 2];
   A.nr = pr.n;
   A.twc = twiddles<T>(c, 0);
@@ -497,7 +505,7 @@ static int tile_pass_v(fbse_ctx *c, int v, int kind, const cplx_t<T> *src, cplx_
   } else {
     A.row_stride = c->nx * c->ny;
     A.tile_stride = c->nx;
-    A.tiles_per_item = (int)c->ny;
+    A.tiles_per_item = (int)c->ny * A.col_split;
   }
   A.nxny = c->nx * c->ny;
   const bool f32 = sizeof(T) == 4;
@@ -573,6 +581,66 @@ static int tile_pass_v(fbse_ctx *c, int v, int kind, const cplx_t<T> *src, cplx_
   return FBSE_OK;
 }
 
+// Columns per z-slab tile: a slab cut into S x ranges needs 1/S of the LDS,
+// so more work-groups share a CU and hide each other's loads.  Forced by
+// fbse_set_slab_split, else timed once per precision on the first batch of at
+// least 1M grid points (z pass into a scratch buffer, best of 3) over the
+// splits that keep at least 16 columns and fit the instance; 1 before that.
+template <typename T> static bool slab_split_ok(const fbse_ctx *c, int v, int s) {
+  if (s < 1 || c->nx % s || c->nx / s < 16) return s == 1;
+  int nv;
+  const TileVariant *tv = tile_variants<T>(&nv);
+  return v >= 0 && v < nv && tile_threads((int)c->nz, (int)(c->nx / s), tv[v].e, tv[v].max_nt) > 0;
+}
+template <typename T> static int slab_split(fbse_ctx *c, int v, const cplx_t<T> *src, long nbatch) {
+  const int pi = sizeof(T) == 8 ? 0 : 1;
+  if (c->force_zsplit > 0) return slab_split_ok<T>(c, v, c->force_zsplit) ? c->force_zsplit : 1;
+  if (c->zsplit[pi] > 0) return c->zsplit[pi];
+  if (!(c->autotune && nbatch * c->ngrid >= (1L << 20))) return 1;
+  int best_s = 1;
+  cplx_t<T> *scratch = nullptr;
+  const size_t bytes = (size_t)nbatch * c->ngrid * sizeof(cplx_t<T>);
+  if (hipMalloc(&scratch, bytes) == hipSuccess) {
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const int saved_profile = c->profile;
+    c->profile = 0;
+    double best = 1e30;
+    for (int s = 1; s <= 4; ++s) {
+      if (!slab_split_ok<T>(c, v, s)) continue;
+      double tmin = 1e30;
+      for (int rep = 0; rep < 3; ++rep) {
+        (void)hipEventRecord(e0, c->stream);
+        int rc = tile_pass_v<T>(c, v, kTileKinZ, src, scratch, nbatch, nullptr, 0, 0, nullptr, nullptr, s);
+        (void)hipEventRecord(e1, c->stream);
+        if (rc) {
+          c->err.clear();
+          tmin = 1e30;
+          break;
+        }
+        (void)hipEventSynchronize(e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        tmin = std::min(tmin, (double)ms);
+      }
+      if (tmin < best) {
+        best = tmin;
+        best_s = s;
+      }
+    }
+    c->profile = saved_profile;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(scratch);
+  } else {
+    (void)hipGetLastError();
+  }
+  c->zsplit[pi] = best_s;
+  return best_s;
+}
+
 // Instance choice: the autotuner times every instance that fits on the
 // first batch of at least 1M grid points (forward xy pass into a scratch
 // buffer, best of 3) and keeps the fastest per precision; smaller batches
@@ -582,6 +650,15 @@ static int tile_pass(fbse_ctx *c, int kind, const cplx_t<T> *src, cplx_t<T> *dst
                      const cplx_t<T> *psi_in, T a1, T a0, const cplx_t<T> *pjd = nullptr,
                      cplx_t<T> *fwd_dst = nullptr) {
   const int pi = sizeof(T) == 8 ? 0 : 1;
+  if (kind == kTileKinZ || kind == kTilePrecZ) {
+    int nv;
+    tile_variants<T>(&nv);
+    const int v = c->tile_variant[pi] >= 0 ? c->tile_variant[pi] : first_tile_variant<T>(c);
+    const int vv = (c->force_variant == 1 && c->force_lines > 0) ? c->force_lines - 1 : v;
+    if (vv < 0 || vv >= nv) return set_err(c, FBSE_ERR_ARG, "FFT tile instance %d of %d", vv + 1, nv);
+    const int zs = slab_split<T>(c, vv, src, nbatch);
+    return tile_pass_v<T>(c, vv, kind, src, dst, nbatch, psi_in, a1, a0, pjd, fwd_dst, zs);
+  }
   if (c->force_variant == 1 && c->force_lines > 0) {  // forced instance (1-based)
     int nv;
     tile_variants<T>(&nv);

@@ -195,6 +195,12 @@ int fbse_set_window(fbse_ctx *ctx, int window);
  * 1..3 stage 12, 16, 32 elements per thread, FP64 instances 1..2 stage 12, 16). */
 int fbse_set_fft_kernel(fbse_ctx *ctx, int variant, int lines);
 
+/* Plane/slab passes: the z-slab pass cuts each xz slab into `split` tiles of
+ * nx/split columns (smaller LDS tiles, more work-groups per CU).  0 = autotune
+ * (the default), otherwise forced when it divides nx and leaves at least 16
+ * columns (else 1).  Results do not depend on it. */
+int fbse_set_slab_split(fbse_ctx *ctx, int split);
+
 /* Newton-window graphs (default on): the filter captures one full
  * accumulation window (R Newton steps, their accumulation and a device step
  * counter update) into a hipGraph and replays it for every further full

@@ -1,7 +1,8 @@
 """Plane/slab FFT passes on grids whose radix schedules use every composite
 butterfly (6, 9, 10, 12, 15, 16 as well as 2, 3, 4, 5, 8): H from each FP64
-plane/slab kernel instance must equal H from the per-axis passes (which the
-parity tests hold to the oracle) to FFT rounding, and the oracle itself.
+plane/slab kernel instance, with z slabs whole or cut into 2 or 3 column
+tiles, must equal H from the per-axis passes (which the parity tests hold to
+the oracle) to FFT rounding, and the oracle itself.
 
 Schedules (fewest passes, ties to the smaller largest radix):
   12 staged values: 60 -> 12*5, 45 -> 15*3, 80 -> 16*5, 96 -> 12*8,
@@ -32,5 +33,8 @@ def test_tile_radices(grid):
     assert rel(ref, oracle_ham(s).apply(psi)) < 1e-11
     for inst in (1, 2):
         g.set_fft_kernel(1, inst)
-        out = g.apply_h(psi)
-        assert rel(out, ref) < 1e-12, inst
+        for split in (1, 2, 3):  # z-slab tiles of 60, 30, 20 (96: 96, 48, 32) columns
+            g.set_slab_split(split)
+            out = g.apply_h(psi)
+            assert rel(out, ref) < 1e-12, (inst, split)
+    g.set_slab_split(0)
