@@ -30,11 +30,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def setup(cfg_name, dev):
+def setup(cfg_name, dev, ncheby=0):
     import torch
     from filterbse_amd import api, dots
 
     cfg = dict(dots.CONFIGS[cfg_name])
+    if ncheby:
+        cfg["ncheby"] = ncheby
     t0 = time.perf_counter()
     job = dots.build_dot(cfg["radius_nm"], cfg["n_req"], cfg["dx"], so=cfg["so"], nl=cfg["nl"],
                          builder=cfg.get("builder", "host"))
@@ -237,6 +239,56 @@ def io_sample(args, dev):
     print(json.dumps(line), flush=True)
 
 
+def mlp_sample(args, dev):
+    """The warm-start network's own BF16 MFMA kernels (include/fbse_mlp.h) on
+    n grid points of synthetic features: forward passes, then training steps
+    (loss + gradient + Adam).  FLOPs are those of the MFMA tiles the kernels
+    issue (input width padded to a multiple of 16): 2 n (k_in w + 2 w^2 + w
+    n_out) per forward pass, three times that per training step (forward, data
+    gradient, weight gradient); against the dense BF16 peak of 2.5 PFLOP/s."""
+    import torch
+    from filterbse_amd import api
+    from filterbse_amd import warmstart as WS
+    n = args.mlp_sample
+    width = 128
+    mlp = api.Mlp(WS.N_IN, width, WS.N_OUT, seed=1)
+    k = mlp.k_in
+    g = torch.Generator(device=dev)
+    g.manual_seed(3)
+    x = (torch.randn((n, k), generator=g, device=dev) * (torch.arange(k, device=dev) < WS.N_IN)).to(
+        torch.bfloat16).view(torch.int16).contiguous()
+    t = torch.randn((n, WS.N_OUT), generator=g, device=dev)
+    y = torch.empty((n, WS.N_OUT), dtype=torch.float32, device=dev)
+    flops_fwd = 2.0 * n * (k * width + 2 * width * width + width * WS.N_OUT)
+    torch.cuda.synchronize()
+    mlp.forward_dev(x.data_ptr(), n, y.data_ptr())  # warm-up
+    torch.cuda.synchronize()
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        mlp.forward_dev(x.data_ptr(), n, y.data_ptr())
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / reps
+    mlp.loss_grad_dev(x.data_ptr(), t.data_ptr(), None, n)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        mlp.loss_grad_dev(x.data_ptr(), t.data_ptr(), None, n)
+        mlp.adam_step(1e-3)
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / reps
+    peak = 2500.0
+    fwd = flops_fwd / t_fwd / 1e12
+    trn = 3.0 * flops_fwd / t_step / 1e12
+    line = dict(metric="warm-start network BF16 MFMA throughput", value=round(trn, 2), unit="TFLOP/s", n_gpus=1,
+                higher_is_better=True, dtype="bf16 (fp32 accumulate)", data="synthetic features",
+                config=dict(workload="%d grid points, %d -> %d -> %d -> %d -> %d MLP" % (
+                    n, WS.N_IN, width, width, width, WS.N_OUT)),
+                forward_ms=round(1e3 * t_fwd, 3), forward_tflops=round(fwd, 2),
+                train_step_ms=round(1e3 * t_step, 3), train_tflops=round(trn, 2),
+                mfma=dict(bound="mfma", achieved=round(trn, 2), peak=peak, unit="TFLOP/s", frac=round(trn / peak, 4)))
+    print(json.dumps(line), flush=True)
+
+
 # Batched uneven dots: 32 spin-orbit dots with seeded radii uniform in
 # 1.0-3.0 nm at dx = 1.0 bohr (the reference grid rule needs ~52^3 ... ~120^3;
 # each axis is rounded up to a size with prime factors 2, 3, 5), 32 starts x 8
@@ -372,6 +424,10 @@ def main():
     ap.add_argument("--traffic-sample", type=int, default=0, metavar="M",
                     help="only run an M-step mixed filter of the workload and one device copy of the bytes of "
                          "one FFT launch (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs), then exit")
+    ap.add_argument("--ncheby", type=int, default=0, metavar="M",
+                    help="Newton length override (short filters for counter runs of the other stages)")
+    ap.add_argument("--mlp-sample", type=int, default=0, metavar="N",
+                    help="only measure the warm-start network's MFMA kernels on N grid points, then exit")
     ap.add_argument("--io-sample", type=int, default=0, metavar="N",
                     help="only measure state-file streaming of N FP64 spinor states of 96^3, then exit")
     args = ap.parse_args()
@@ -389,10 +445,13 @@ def main():
     if args.io_sample:
         io_sample(args, dev)
         return
+    if args.mlp_sample:
+        mlp_sample(args, dev)
+        return
     if args.config == "c2_uneven":
         bench_uneven(args, dev)
         return
-    S = setup(args.config, dev)
+    S = setup(args.config, dev, args.ncheby)
     S["cfg_name"] = args.config
     mixed, fp64 = api.FBSE_PREC_MIXED, api.FBSE_PREC_FP64
 

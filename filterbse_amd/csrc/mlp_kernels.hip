@@ -79,7 +79,8 @@ __device__ __forceinline__ float silu_grad(float z) {
 
 constexpr int kMlpThreads = 256;  // 4 wavefronts, 16 points each
 constexpr int kMlpPoints = 64;    // points per work-group in the point GEMMs
-constexpr int kWgChunk = 128;     // points per work-group in the weight-gradient reduction
+constexpr int kWgChunk = 128;     // points per chunk of the weight-gradient reduction
+constexpr int kWgBlocks = 1024;   // work-groups of the weight-gradient kernel (at most)
 
 // Y = act(X W + b) for n points.  X: n x K BF16 (ld K), W: K x N BF16 in LDS,
 // NT = N / 16.  OUT_F32: write FP32 Y (the linear output layer), else BF16
@@ -177,9 +178,12 @@ __global__ __launch_bounds__(kMlpThreads) void dense_bwd_data_kernel(const unsig
   }
 }
 
-// Partial weight and bias gradients of one chunk of kWgChunk points:
+// Partial weight and bias gradients of the chunks of kWgChunk points a
+// work-group takes (chunks blockIdx.x, blockIdx.x + gridDim.x, ... in order):
 // part[blk][k*N + j] = sum_p X[p][k] dZ[p][j], part[blk][K*N + j] = sum_p dZ[p][j].
-// Every 16x16 tile of dW is an MFMA accumulation with the points as K.
+// Every 16x16 tile of dW is an MFMA accumulation with the points as K; the
+// accumulators stay in registers across the work-group's chunks, so the
+// partial buffer has one tile per work-group, not one per chunk.
 template <int KT, int NT>
 __global__ __launch_bounds__(kMlpThreads) void dense_wgrad_kernel(const unsigned short *__restrict__ X,
                                                                   const unsigned short *__restrict__ dZ, long n,
@@ -188,28 +192,38 @@ __global__ __launch_bounds__(kMlpThreads) void dense_wgrad_kernel(const unsigned
   constexpr int TPW = (TILES + 3) / 4;  // tiles per wavefront
   __shared__ unsigned short s_x[kWgChunk * K];
   __shared__ unsigned short s_d[kWgChunk * N];
-  const long base = (long)blockIdx.x * kWgChunk;
-  const int np = (int)((n - base) < kWgChunk ? (n - base) : kWgChunk);
-  for (int i = threadIdx.x; i < kWgChunk * K; i += kMlpThreads) s_x[i] = (i / K) < np ? X[base * K + i] : 0;
-  for (int i = threadIdx.x; i < kWgChunk * N; i += kMlpThreads) s_d[i] = (i / N) < np ? dZ[base * N + i] : 0;
-  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = lane & 15, kq = 4 * (lane >> 4);
   f32x4_t acc[TPW];
 #pragma unroll
   for (int u = 0; u < TPW; ++u) acc[u] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-  for (int p0 = 0; p0 < kWgChunk; p0 += 16) {
+  float bsum = 0.f;  // bias gradient of column threadIdx.x (< N)
+  const long nblk = (n + kWgChunk - 1) / kWgChunk;
+  for (long blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const long base = blk * kWgChunk;
+    const int np = (int)((n - base) < kWgChunk ? (n - base) : kWgChunk);
+    __syncthreads();  // the previous chunk's tiles are consumed
+    for (int i = threadIdx.x; i < kWgChunk * K; i += kMlpThreads) s_x[i] = (i / K) < np ? X[base * K + i] : 0;
+    for (int i = threadIdx.x; i < kWgChunk * N; i += kMlpThreads) s_d[i] = (i / N) < np ? dZ[base * N + i] : 0;
+    __syncthreads();
+    for (int p0 = 0; p0 < kWgChunk; p0 += 16) {
 #pragma unroll
-    for (int u = 0; u < TPW; ++u) {
-      const int tile = wave * TPW + u;
-      if (tile >= TILES) continue;
-      const int kt = tile / NT, nt = tile - kt * NT;
-      // A = X^T tile: A[i][pp] = X[p0+pp][kt*16+i]; B = dZ tile: B[pp][j] = dZ[p0+pp][nt*16+j]
-      const unsigned short *xp = s_x + (p0 + kq) * K + kt * 16 + r;
-      const unsigned short *dp = s_d + (p0 + kq) * N + nt * 16 + r;
-      bf16x4_t a = bf16x4_t{(short)xp[0], (short)xp[K], (short)xp[2 * K], (short)xp[3 * K]};
-      bf16x4_t b = bf16x4_t{(short)dp[0], (short)dp[N], (short)dp[2 * N], (short)dp[3 * N]};
-      acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[u], 0, 0, 0);
+      for (int u = 0; u < TPW; ++u) {
+        const int tile = wave * TPW + u;
+        if (tile >= TILES) continue;
+        const int kt = tile / NT, nt = tile - kt * NT;
+        // A = X^T tile: A[i][pp] = X[p0+pp][kt*16+i]; B = dZ tile: B[pp][j] = dZ[p0+pp][nt*16+j]
+        const unsigned short *xp = s_x + (p0 + kq) * K + kt * 16 + r;
+        const unsigned short *dp = s_d + (p0 + kq) * N + nt * 16 + r;
+        bf16x4_t a = bf16x4_t{(short)xp[0], (short)xp[K], (short)xp[2 * K], (short)xp[3 * K]};
+        bf16x4_t b = bf16x4_t{(short)dp[0], (short)dp[N], (short)dp[2 * N], (short)dp[3 * N]};
+        acc[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[u], 0, 0, 0);
+      }
+    }
+    if (threadIdx.x < N) {
+      float s = 0.f;
+      for (int p = 0; p < kWgChunk; ++p) s += bf2f(s_d[p * N + threadIdx.x]);
+      bsum += s;
     }
   }
   float *out = part + (long)blockIdx.x * (K * N + N);
@@ -221,11 +235,7 @@ __global__ __launch_bounds__(kMlpThreads) void dense_wgrad_kernel(const unsigned
 #pragma unroll
     for (int q = 0; q < 4; ++q) out[(kt * 16 + kq + q) * N + nt * 16 + r] = acc[u][q];
   }
-  for (int j = threadIdx.x; j < N; j += kMlpThreads) {
-    float s = 0.f;
-    for (int p = 0; p < kWgChunk; ++p) s += bf2f(s_d[p * N + j]);
-    out[K * N + j] = s;
-  }
+  if (threadIdx.x < N) out[K * N + threadIdx.x] = bsum;
 }
 
 // grad (packed layout) = sum over blocks of the partials (fixed order); the
@@ -640,7 +650,9 @@ static void launch_wgrad_k(fbse_mlp *m, int nt, dim3 g, const unsigned short *x,
   }
 }
 static int launch_wgrad(fbse_mlp *m, int l, const unsigned short *x, const unsigned short *dz, long n) {
-  const long nblk = (n + kWgChunk - 1) / kWgChunk;
+  // a fixed number of work-groups (so the summation order does not depend on
+  // the device), each over many chunks
+  const long nblk = std::min((n + kWgChunk - 1) / kWgChunk, (long)kWgBlocks);
   const long need = nblk * ((long)m->lk[l] * m->ln[l] + m->ln[l]);
   if (need > m->part_cap) {
     if (m->d_part) (void)hipFree(m->d_part);
@@ -660,8 +672,9 @@ static int launch_wgrad(fbse_mlp *m, int l, const unsigned short *x, const unsig
   }
   int kr, nr;
   layer_dims(m, l, &kr, &nr);
-  hipLaunchKernelGGL(reduce_part_kernel, dim3(64), dim3(256), 0, m->stream, m->d_part, (int)nblk, m->lk[l], m->ln[l],
-                     kr, nr, m->d_grads + m->off_w[l], m->d_grads + m->off_b[l]);
+  const int stride = m->lk[l] * m->ln[l] + m->ln[l];
+  hipLaunchKernelGGL(reduce_part_kernel, dim3((stride + 255) / 256), dim3(256), 0, m->stream, m->d_part, (int)nblk,
+                     m->lk[l], m->ln[l], kr, nr, m->d_grads + m->off_w[l], m->d_grads + m->off_b[l]);
   MLP_HIP(m, hipGetLastError());
   return FBSE_OK;
 }
