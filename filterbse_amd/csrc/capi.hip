@@ -384,6 +384,12 @@ extern "C" int fbse_create(const fbse_grid_desc *g, const fbse_proj_desc *pd, in
     return FBSE_ERR_HIP;
   }
   c->stream = c->own_stream;
+  if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+    fbse_destroy(c);
+    return FBSE_ERR_HIP;
+  }
   if (rocblas_create_handle(&c->blas) != rocblas_status_success) {
     fbse_destroy(c);
     return FBSE_ERR_SOLVER;
@@ -503,6 +509,10 @@ extern "C" void fbse_destroy(fbse_ctx *c) {
     hipEventDestroy(p.second.second);
   }
   if (c->blas) rocblas_destroy_handle(c->blas);
+  if (c->side) hipStreamSynchronize(c->side);
+  if (c->ev_fork) hipEventDestroy(c->ev_fork);
+  if (c->ev_join) hipEventDestroy(c->ev_join);
+  if (c->side) hipStreamDestroy(c->side);
   if (c->own_stream) hipStreamDestroy(c->own_stream);
   delete c;
 }

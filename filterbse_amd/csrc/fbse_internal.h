@@ -116,6 +116,11 @@ struct fbse_ctx {
   int device = 0;
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
+  // second stream of the Newton step: the projection pass runs there beside
+  // the z-slab pass (fork/join events; captured into the window graphs too)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int concurrent_proj = 1;
   rocblas_handle blas = nullptr;
   std::string err;
 
@@ -249,9 +254,12 @@ template <typename T> bool tiles_supported(const fbse_ctx *c);
 // fwd_next: the combine pass also leaves the forward xy transform of out in
 // work (the first pass of the next Newton step); fwd_ready: work already
 // holds the forward xy transform of psi, so the first pass is skipped.
+// join: when set, the stream waits for this event before the combine pass
+// (the projection pass that fills pjd ran on another stream)
 template <typename T>
 int kinetic_local_tiles(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *work, long nstates, T a1,
-                        T a0, const cplx_t<T> *pjd = nullptr, bool fwd_ready = false, bool fwd_next = false);
+                        T a0, const cplx_t<T> *pjd = nullptr, bool fwd_ready = false, bool fwd_next = false,
+                        hipEvent_t join = nullptr);
 template <typename T>
 int precondition_tiles(fbse_ctx *c, const cplx_t<T> *r, cplx_t<T> *out, cplx_t<T> *work, long nstates, T pa, T pb);
 

@@ -523,6 +523,34 @@ template <typename T> static int ensure_pjd(fbse_ctx *c, long nstates) {
   return FBSE_OK;
 }
 
+// The projection pass (it reads psi only) on the side stream, forked from the
+// main stream, so that it can run beside the FFT passes that precede the
+// combine; *joined = false: the main stream must wait for ev_join before
+// anything reads d.  Runs on the main stream when profiling (every launch
+// timed alone) or when the side stream is off.
+template <typename T>
+static int project_forked(fbse_ctx *c, const cplx_t<T> *psi, long nstates, double a1, cplx_t<T> *d, bool *joined) {
+  *joined = true;
+  if (!(c->concurrent_proj && c->side && !c->profile)) return proj_project<T>(c, psi, nstates, a1, d);
+  hipStream_t main_stream = c->stream;
+  FBSE_HIP(c, hipEventRecord(c->ev_fork, main_stream));
+  FBSE_HIP(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+  c->stream = c->side;
+  int rc = proj_project<T>(c, psi, nstates, a1, d);
+  c->stream = main_stream;
+  hipError_t e = hipEventRecord(c->ev_join, c->side);
+  if (e != hipSuccess) {
+    (void)hipStreamSynchronize(c->side);
+    return rc ? rc : set_err(c, FBSE_ERR_HIP, "event record: %s", hipGetErrorString(e));
+  }
+  if (rc) {
+    (void)hipStreamWaitEvent(main_stream, c->ev_join, 0);  // joined again before the error returns
+    return rc;
+  }
+  *joined = false;
+  return FBSE_OK;
+}
+
 template <typename T>
 int apply_h_cplx(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *work, long nstates, T a1, T a0) {
   const long nb = nstates * c->nspin;
@@ -540,15 +568,18 @@ int apply_h_cplx(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *w
     // with one radial projector: projections before the passes (they read
     // psi), the expansion from the rank records after the combine
     cplx_t<T> *d = nullptr;
+    bool joined = true;
     if (proj_fusable(c)) {
       if ((rc = ensure_pjd<T>(c, nstates))) return rc;
       d = (cplx_t<T> *)c->d_pjd;
-      if ((rc = proj_project<T>(c, psi, nstates, (double)a1, d))) return rc;
+      if ((rc = project_forked<T>(c, psi, nstates, (double)a1, d, &joined))) return rc;
     }
-    if ((rc = fft_pass<T>(c, 0, kPassFwd, psi, work, nb, nullptr, 0, 0))) return rc;
-    if ((rc = fft_pass<T>(c, 1, kPassFwd, work, work, nb, nullptr, 0, 0))) return rc;
-    if ((rc = fft_pass<T>(c, 2, kPassKin, work, work, nb, nullptr, 0, 0))) return rc;
-    if ((rc = fft_pass<T>(c, 1, kPassBwd, work, work, nb, nullptr, 0, 0))) return rc;
+    rc = fft_pass<T>(c, 0, kPassFwd, psi, work, nb, nullptr, 0, 0);
+    if (!rc) rc = fft_pass<T>(c, 1, kPassFwd, work, work, nb, nullptr, 0, 0);
+    if (!rc) rc = fft_pass<T>(c, 2, kPassKin, work, work, nb, nullptr, 0, 0);
+    if (!rc) rc = fft_pass<T>(c, 1, kPassBwd, work, work, nb, nullptr, 0, 0);
+    if (!joined) (void)hipStreamWaitEvent(c->stream, c->ev_join, 0);  // the projections are done
+    if (rc) return rc;
     if ((rc = fft_pass<T>(c, 0, kPassBwdCombine, work, out, nb, psi, a1, a0))) return rc;
     if (d) return proj_expand<T>(c, d, out, nstates);
   }
@@ -563,16 +594,23 @@ int apply_h_chain(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *
                   bool first) {
   if (!use_tiles<T>(c)) return apply_h_cplx<T>(c, psi, out, work, nstates, a1, a0);
   const cplx_t<T> *d = nullptr;
+  hipEvent_t join = nullptr;
   int rc;
   if (proj_fusable(c)) {
     if ((rc = ensure_pjd<T>(c, nstates))) return rc;
     d = (cplx_t<T> *)c->d_pjd;
-    if ((rc = proj_project<T>(c, psi, nstates, (double)a1, (cplx_t<T> *)d))) return rc;
+    bool joined = true;
+    if ((rc = project_forked<T>(c, psi, nstates, (double)a1, (cplx_t<T> *)d, &joined))) return rc;
+    if (!joined) join = c->ev_join;
   }
   // the expansion of non-fusable projectors is added to out after the
   // combine pass, so then out's forward transform cannot ride on it
   const bool chain = !c->has_proj || d;
-  if ((rc = kinetic_local_tiles<T>(c, psi, out, work, nstates, a1, a0, d, chain && !first, chain))) return rc;
+  rc = kinetic_local_tiles<T>(c, psi, out, work, nstates, a1, a0, d, chain && !first, chain, join);
+  if (rc) {
+    if (join) (void)hipStreamWaitEvent(c->stream, join, 0);  // never leave the side stream unjoined
+    return rc;
+  }
   if (!chain) return proj_apply<T>(c, psi, out, nstates, (double)a1);
   return FBSE_OK;
 }

@@ -722,17 +722,18 @@ static int tile_pass(fbse_ctx *c, int kind, const cplx_t<T> *src, cplx_t<T> *dst
 // added by the caller); work holds one batch of grids.
 template <typename T>
 int kinetic_local_tiles(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *work, long nstates, T a1,
-                        T a0, const cplx_t<T> *pjd, bool fwd_ready, bool fwd_next) {
+                        T a0, const cplx_t<T> *pjd, bool fwd_ready, bool fwd_next, hipEvent_t join) {
   const long nb = nstates * c->nspin;
   int rc;
   if (!fwd_ready && (rc = tile_pass<T>(c, kTileFwdXY, psi, work, nb, nullptr, 0, 0))) return rc;
   if ((rc = tile_pass<T>(c, kTileKinZ, work, work, nb, nullptr, 0, 0))) return rc;
+  if (join) FBSE_HIP(c, hipStreamWaitEvent(c->stream, join, 0));
   return tile_pass<T>(c, kTileBwdXYCombine, work, out, nb, psi, a1, a0, pjd, fwd_next ? work : nullptr);
 }
 template int kinetic_local_tiles<double>(fbse_ctx *, const double2 *, double2 *, double2 *, long, double, double,
-                                         const double2 *, bool, bool);
+                                         const double2 *, bool, bool, hipEvent_t);
 template int kinetic_local_tiles<float>(fbse_ctx *, const float2 *, float2 *, float2 *, long, float, float,
-                                        const float2 *, bool, bool);
+                                        const float2 *, bool, bool, hipEvent_t);
 
 template <typename T>
 int precondition_tiles(fbse_ctx *c, const cplx_t<T> *r, cplx_t<T> *out, cplx_t<T> *work, long nstates, T pa,
