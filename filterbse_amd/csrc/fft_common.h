@@ -39,13 +39,34 @@ template <typename C, typename T> __device__ __forceinline__ C cmul_i(C a, T s) 
 // Y_m (proj d_so[m] + nl_proj d_nl[m]) summed over m (phamiltonian.c:140-444)
 template <typename T>
 __device__ __forceinline__ cplx_t<T> record_term(const T *v, const cplx_t<T> *dd) {
+  // the record (8 values) and the coefficients (6 complex) are 16-byte
+  // aligned: read them as 16-byte vectors
+  T r[8];
+  cplx_t<T> d[6];
+  if constexpr (sizeof(T) == 4) {
+    const float4 *v4 = reinterpret_cast<const float4 *>(v);
+    const float4 p0 = v4[0], p1 = v4[1];
+    r[0] = p0.x, r[1] = p0.y, r[2] = p0.z, r[3] = p0.w, r[4] = p1.x, r[5] = p1.y, r[6] = p1.z, r[7] = p1.w;
+    const float4 *d4 = reinterpret_cast<const float4 *>(dd);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const float4 t = d4[q];
+      d[2 * q] = cplx_t<T>{t.x, t.y};
+      d[2 * q + 1] = cplx_t<T>{t.z, t.w};
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) r[q] = v[q];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) d[q] = dd[q];
+  }
   cplx_t<T> acc{(T)0, (T)0};
-  const T wso = v[0], wnl = v[1];
+  const T wso = r[0], wnl = r[1];
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
-    const cplx_t<T> a = dd[m], b = dd[3 + m];
+    const cplx_t<T> a = d[m], b = d[3 + m];
     const T sx = wso * a.x + wnl * b.x, sy = wso * a.y + wnl * b.y;
-    const T yx = v[2 + 2 * m], yy = v[3 + 2 * m];
+    const T yx = r[2 + 2 * m], yy = r[3 + 2 * m];
     acc.x += yx * sx - yy * sy;
     acc.y += yx * sy + yy * sx;
   }
