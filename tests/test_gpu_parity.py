@@ -98,7 +98,12 @@ def test_filter_graph_replay(system, precision):
     assert g.graph_replays() == 9
     eager = g.apply_h(random_states(s, 2, seed=3))  # unrelated eager work between replays
     out2 = g.filter(psi_rank, n_starts, ms, an, zn, Vmin, dE_1, precision=precision)
+    # profiling keeps every launch on the main stream (no side-stream projections)
+    g.profile(True)
+    out3 = g.filter(psi_rank, n_starts, ms, an, zn, Vmin, dE_1, precision=precision)
+    g.profile(False)
     g.set_window(0)
+    assert np.array_equal(out3, ref), (name, rel(out3, ref))
     assert np.isfinite(eager).all()
     assert np.array_equal(out1, ref), (name, rel(out1, ref))
     assert np.array_equal(out2, ref), (name, rel(out2, ref))
