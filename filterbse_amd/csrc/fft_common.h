@@ -211,11 +211,19 @@ template <typename T, int P, int Q> struct BflyPQ {
       for (int k1 = 0; k1 < P; ++k1) {
         C v = t[k1];
         if (n2 > 0 && k1 > 0) {
+          // after unrolling j is a constant: roots +-1 and +-i cost no multiply
           const int j = (n2 * k1) % N;
-          C w;
-          w.x = (T)RootTab<N>::c[j];
-          w.y = sgn * (T)RootTab<N>::s[j];
-          v = cmul(v, w);
+          if (RootTab<N>::c[j] == 0.0) {
+            v = cmul_i<C, T>(v, sgn * (T)RootTab<N>::s[j]);
+          } else if (RootTab<N>::s[j] == 0.0) {
+            v.x = (T)RootTab<N>::c[j] * v.x;
+            v.y = (T)RootTab<N>::c[j] * v.y;
+          } else {
+            C w;
+            w.x = (T)RootTab<N>::c[j];
+            w.y = sgn * (T)RootTab<N>::s[j];
+            v = cmul(v, w);
+          }
         }
         x[Q * k1 + n2] = v;  // y[n2][k1]
       }

@@ -85,25 +85,34 @@ __device__ __forceinline__ int mod_small(int i, int p, float inv_p) {
   return r;
 }
 
-// One in-place Stockham pass of radix R over `nlines` lines of length n.
-// Element j of line l sits at l*lstride + j*estride.  The work-group size is
-// a multiple of nlines, so a thread keeps one line l0 = tid % nlines for the
-// whole transform and takes butterflies i0, i0 + di, ... of it; consecutive
-// threads are on consecutive lines: for column transforms (lstride 1) a
-// wavefront touches consecutive LDS words, for row transforms (lstride = odd
-// row pitch) words 2*pitch apart, which spread over all banks.
+// The tile, its twiddle tables and the k^2 tables live in the work-group's
+// dynamic LDS.  The radix passes below address it through this array with
+// element offsets, so every access is an LDS instruction.
+extern __shared__ __align__(16) unsigned char tile_lds[];
+
+// One in-place Stockham pass of radix R over `nlines` lines of length n
+// (m = n / R butterflies per line, p = product of the radices already done,
+// tstep = n / (p R)).  Element j of line l sits at l*lstride + j*estride.  The
+// work-group size is a multiple of nlines, so a thread keeps one line
+// l0 = tid % nlines for the whole transform and takes butterflies i0, i0 + di,
+// ... of it; consecutive threads are on consecutive lines: for column
+// transforms (lstride 1) a wavefront touches consecutive LDS words, for row
+// transforms (lstride = odd row pitch) words 2*pitch apart, which spread over
+// all banks.  Twiddle w^(r kk tstep) is element r*kk*tstep of the length-n
+// table at tw_off (r*kk*tstep < n), conjugated for the backward transform.
 // Each radix pass is a separate (not inlined) function: one kernel holding
 // every radix's inlined pass spills registers, while separate functions keep
 // their own allocation at the cost of a call per pass.
-template <typename T, int R, int KB>
-__device__ __attribute__((noinline)) void tile_radix_pass(cplx_t<T> *buf, const cplx_t<T> *tw, int n, int p, int estride,
-                                                int lstride, int l0, int i0, int di, bool inv) {
+template <typename T, int R, int KB, bool INV>
+__device__ __attribute__((noinline)) void tile_radix_pass(int tw_off, int m, int p, int tstep, int estride,
+                                                          int lstride, int l0, int i0, int di) {
   using C = cplx_t<T>;
-  const int m = n / R;
-  const int tstep = n / (p * R);
-  const float inv_p = 1.0f / (float)p;
-  const T sgn = inv ? (T)1 : (T)-1;
+  C *buf = reinterpret_cast<C *>(tile_lds);
+  const C *tw = buf + tw_off;
+  const float inv_p = __builtin_amdgcn_rcpf((float)p);  // mod_small tolerates the approximate reciprocal
+  constexpr T sgn = INV ? (T)1 : (T)-1;
   const int lbase = l0 * lstride;
+  const int mstride = m * estride, pstride = p * estride;
   C x[KB][R];
   int dsti[KB];
 #pragma unroll
@@ -112,10 +121,23 @@ __device__ __attribute__((noinline)) void tile_radix_pass(cplx_t<T> *buf, const 
     dsti[k] = -1;
     if (i < m) {
       const int kk = mod_small(i, p, inv_p);
-      const C *src = buf + lbase + i * estride;
+      int o = lbase + i * estride;
 #pragma unroll
-      for (int r = 0; r < R; ++r) x[k][r] = src[r * m * estride];
-      if (kk > 0) twiddle_powers<T, R>(x[k], tw, kk, tstep, inv);
+      for (int r = 0; r < R; ++r, o += mstride) x[k][r] = buf[o];
+      const int tj = kk * tstep;
+      int t = 0;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        t += tj;
+        const C w = tw[t];
+        const C v = x[k][r];
+        if (INV) {
+          x[k][r].x = v.x * w.x + v.y * w.y;
+          x[k][r].y = v.y * w.x - v.x * w.y;
+        } else {
+          x[k][r] = cmul(v, w);
+        }
+      }
       dsti[k] = lbase + ((i - kk) * R + kk) * estride;
     }
   }
@@ -124,24 +146,28 @@ __device__ __attribute__((noinline)) void tile_radix_pass(cplx_t<T> *buf, const 
   for (int k = 0; k < KB; ++k) {
     if (dsti[k] >= 0) {
       Bfly<T, R>::run(x[k], sgn);
+      int o = dsti[k];
 #pragma unroll
-      for (int q = 0; q < R; ++q) buf[dsti[k] + q * p * estride] = x[k][Bfly<T, R>::perm(q)];
+      for (int q = 0; q < R; ++q, o += pstride) buf[o] = x[k][Bfly<T, R>::perm(q)];
     }
   }
   __syncthreads();
 }
 
-// 1D transforms of length n over nlines lines (nlines divides blockDim.x).
-template <typename T, int E>
-__device__ __forceinline__ void tile_fft(cplx_t<T> *buf, const cplx_t<T> *tw, int n, int nf, unsigned long long f,
-                                         int estride, int lstride, int nlines, bool inv) {
+// 1D transforms of length n over nlines lines (nlines divides blockDim.x);
+// the twiddle table (n entries) is at element tw_off of the LDS array.
+template <typename T, int E, bool INV>
+__device__ __forceinline__ void tile_fft(int tw_off, int n, int nf, unsigned long long f, int estride, int lstride,
+                                         int nlines) {
   const int l0 = threadIdx.x % nlines, i0 = threadIdx.x / nlines, di = blockDim.x / nlines;
-  int p = 1;
+  int p = 1, rest = n;
   for (int s = 0; s < nf; ++s) {
     const int R = (int)((f >> (4 * s)) & 15u) + 1;
-#define FBSE_PASS(RR)                                                                               \
-  if constexpr (tile_radix_fits(E, RR))                                                             \
-    tile_radix_pass<T, RR, (E + RR - 1) / RR>(buf, tw, n, p, estride, lstride, l0, i0, di, inv)
+#define FBSE_PASS(RR)                                                                                    \
+  if constexpr (tile_radix_fits(E, RR)) {                                                                \
+    rest /= RR;                                                                                          \
+    tile_radix_pass<T, RR, (E + RR - 1) / RR, INV>(tw_off, n / RR, p, rest, estride, lstride, l0, i0, di); \
+  }
     switch (R) {
       case 2: FBSE_PASS(2); break;
       case 3: FBSE_PASS(3); break;
@@ -163,12 +189,12 @@ __device__ __forceinline__ void tile_fft(cplx_t<T> *buf, const cplx_t<T> *tw, in
 template <typename T, int KIND, int E>
 __device__ __forceinline__ void tile_body(const TileArgs<T> &A) {
   using C = cplx_t<T>;
-  extern __shared__ unsigned char smem_raw[];
   const int nc = A.nc, nr = A.nr, ls = A.ls;
   const int NT = blockDim.x;
-  C *buf = reinterpret_cast<C *>(smem_raw);
-  C *twc = buf + nr * ls;
-  C *twr = twc + nc;
+  C *buf = reinterpret_cast<C *>(tile_lds);
+  const int twc_off = nr * ls, twr_off = twc_off + nc;
+  C *twc = buf + twc_off;
+  C *twr = buf + twr_off;
   for (int t = threadIdx.x; t < nc; t += NT) twc[t] = A.twc[t];
   for (int t = threadIdx.x; t < nr; t += NT) twr[t] = A.twr[t];
 
@@ -217,13 +243,13 @@ __device__ __forceinline__ void tile_body(const TileArgs<T> &A) {
   __syncthreads();
 
   if (KIND == kTileFwdXY) {
-    tile_fft<T, E>(buf, twc, nc, A.nfc, A.fc, 1, ls, nr, false);  // x rows
-    tile_fft<T, E>(buf, twr, nr, A.nfr, A.fr, ls, 1, nc, false);  // y columns
+    tile_fft<T, E, false>(twc_off, nc, A.nfc, A.fc, 1, ls, nr);  // x rows
+    tile_fft<T, E, false>(twr_off, nr, A.nfr, A.fr, ls, 1, nc);  // y columns
   } else if (KIND == kTileBwdXY || KIND == kTileBwdXYCombine) {
-    tile_fft<T, E>(buf, twr, nr, A.nfr, A.fr, ls, 1, nc, true);
-    tile_fft<T, E>(buf, twc, nc, A.nfc, A.fc, 1, ls, nr, true);
+    tile_fft<T, E, true>(twr_off, nr, A.nfr, A.fr, ls, 1, nc);
+    tile_fft<T, E, true>(twc_off, nc, A.nfc, A.fc, 1, ls, nr);
   } else {  // z columns: forward, scale by k^2 (or the preconditioner), backward
-    tile_fft<T, E>(buf, twr, nr, A.nfr, A.fr, ls, 1, nc, false);
+    tile_fft<T, E, false>(twr_off, nr, A.nfr, A.fr, ls, 1, nc);
     const T kxy = ksep ? kxl[c0] + A.ky[w] : (T)0;
     const T *ksq = ksep ? nullptr : A.ksqr + xs * nc + c0 + (long)w * nc * A.col_split;
     for (int r = r0; r < nr; r += dr) {
@@ -241,7 +267,7 @@ __device__ __forceinline__ void tile_body(const TileArgs<T> &A) {
       buf[r * ls + c0] = v;
     }
     __syncthreads();
-    tile_fft<T, E>(buf, twr, nr, A.nfr, A.fr, ls, 1, nc, true);
+    tile_fft<T, E, true>(twr_off, nr, A.nfr, A.fr, ls, 1, nc);
   }
 
   cplx_t<T> *dst = A.dst + base + c0;
@@ -315,8 +341,8 @@ __device__ __forceinline__ void tile_body(const TileArgs<T> &A) {
     if (A.fwd_dst) {
       // the next Newton step's first pass on the plane just written
       __syncthreads();
-      tile_fft<T, E>(buf, twc, nc, A.nfc, A.fc, 1, ls, nr, false);  // x rows
-      tile_fft<T, E>(buf, twr, nr, A.nfr, A.fr, ls, 1, nc, false);  // y columns
+      tile_fft<T, E, false>(twc_off, nc, A.nfc, A.fc, 1, ls, nr);  // x rows
+      tile_fft<T, E, false>(twr_off, nr, A.nfr, A.fr, ls, 1, nc);  // y columns
       cplx_t<T> *fd = A.fwd_dst + base + c0;
       for (int r = r0; r < nr; r += dr) fd[r * A.row_stride] = buf[r * ls + c0];
     }
