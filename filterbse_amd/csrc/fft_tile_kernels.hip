@@ -109,7 +109,7 @@ __device__ __attribute__((noinline)) void tile_radix_pass(int tw_off, int m, int
   using C = cplx_t<T>;
   C *buf = reinterpret_cast<C *>(tile_lds);
   const C *tw = buf + tw_off;
-  const float inv_p = __builtin_amdgcn_rcpf((float)p);  // mod_small tolerates the approximate reciprocal
+  const float inv_p = p > 1 ? __builtin_amdgcn_rcpf((float)p) : 1.0f;  // mod_small tolerates the approximate reciprocal
   constexpr T sgn = INV ? (T)1 : (T)-1;
   const int lbase = l0 * lstride;
   const int mstride = m * estride, pstride = p * estride;
@@ -120,22 +120,25 @@ __device__ __attribute__((noinline)) void tile_radix_pass(int tw_off, int m, int
     const int i = i0 + k * di;
     dsti[k] = -1;
     if (i < m) {
-      const int kk = mod_small(i, p, inv_p);
+      // the first pass (p = 1) has kk = 0: no twiddles
+      const int kk = p > 1 ? mod_small(i, p, inv_p) : 0;
       int o = lbase + i * estride;
 #pragma unroll
       for (int r = 0; r < R; ++r, o += mstride) x[k][r] = buf[o];
-      const int tj = kk * tstep;
-      int t = 0;
+      if (p > 1) {
+        const int tj = kk * tstep;
+        int t = 0;
 #pragma unroll
-      for (int r = 1; r < R; ++r) {
-        t += tj;
-        const C w = tw[t];
-        const C v = x[k][r];
-        if (INV) {
-          x[k][r].x = v.x * w.x + v.y * w.y;
-          x[k][r].y = v.y * w.x - v.x * w.y;
-        } else {
-          x[k][r] = cmul(v, w);
+        for (int r = 1; r < R; ++r) {
+          t += tj;
+          const C w = tw[t];
+          const C v = x[k][r];
+          if (INV) {
+            x[k][r].x = v.x * w.x + v.y * w.y;
+            x[k][r].y = v.y * w.x - v.x * w.y;
+          } else {
+            x[k][r] = cmul(v, w);
+          }
         }
       }
       dsti[k] = lbase + ((i - kk) * R + kk) * estride;
