@@ -505,16 +505,18 @@ def main():
     chained = pj is None or ((pj["n_so_proj"] if pj["so"] else 0) <= 1 and (pj["n_nl_proj"] if pj["nl"] else 0) <= 1)
     if fft_path == 1:
         e = {1: 12, 2: 16, 3: 32}.get(fft_inst, 12)
+        # instance 4: the fixed-schedule persistent instance of a cubic grid
+        kfmt = ("fft_fx_f32_n%d<%%d>" % job["nx"]) if fft_inst == 4 else ("fft_tile_f32_e%d<%%d>" % e)
         # dominant kernel: the combine pass.  Algorithmic bytes per launch:
         # the z-slab output and psi_{j-1} read, psi_j written, V_loc read
         # once, and (chained) the next step's forward xy transform written
         launches, ms_tot = prof[1]
         bytes_per_launch = (4.0 if chained else 3.0) * grid_bytes + job["nspin"] * ngrid * 4
-        kname = "fft_tile_f32_e%d<2>" % e
+        kname = kfmt % 2
         kdesc = ("xy-plane backward FFT, V_loc, Newton combine, projector expansion%s"
                  % (", next step's forward xy FFT" if chained else ""))
         zl, zms = prof[0]
-        zslab = dict(kernel="fft_tile_f32_e%d<3>" % e, desc="xz-slab forward z FFT, k^2, backward z FFT",
+        zslab = dict(kernel=kfmt % 3, desc="xz-slab forward z FFT, k^2, backward z FFT",
                      avg_launch_ms=round(zms / max(zl, 1), 4), bytes_per_launch=2.0 * grid_bytes,
                      achieved=round(2.0 * grid_bytes / (zms / max(zl, 1) * 1e-3) / 1e9, 1))
         zslab["frac"] = round(zslab["achieved"] / HBM_PEAK_GBS, 4)

@@ -249,6 +249,8 @@ int fft_pass(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src, cplx_t<T> *d
 // factors into radices 2, 3, 4, 5, 8 and an xy plane and an xz slab fit the
 // 160 KiB LDS of a work-group.  out = a1*(T + V_loc) psi + a0*psi.
 template <typename T> bool tiles_supported(const fbse_ctx *c);
+// the plane/slab passes of precision T run the fixed-schedule instance
+template <typename T> bool tiles_fixed(const fbse_ctx *c);
 // pjd != nullptr: the combine pass also adds the projector expansion with the
 // coefficients of proj_project
 // fwd_next: the combine pass also leaves the forward xy transform of out in

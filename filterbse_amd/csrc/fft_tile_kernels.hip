@@ -22,6 +22,7 @@
 #include "fft_common.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 namespace fbse {
@@ -411,6 +412,298 @@ static void tile_launch(int v, dim3 g, dim3 b, size_t lds, hipStream_t st, const
 }
 
 // ---------------------------------------------------------------------------
+// Fixed-schedule persistent instances: cubic N^3 grids with N = R1 * R2
+// ---------------------------------------------------------------------------
+// Each pass of the instances above is a function call with runtime line
+// lengths, strides and radices, and a call starts by waiting for every
+// outstanding memory operation of the wave.  For the grids below the radix
+// schedule, the line length and every stride are compile-time constants: the
+// passes are inlined with constant LDS offsets and index arithmetic, and each
+// work-group loops over tiles (the launch holds as many work-groups as stay
+// resident), so the twiddle and k^2 tables are loaded once per work-group.
+// Rows of the next tile (PF) and of psi_in / V (PC) can be loaded ahead into
+// registers.  Same arithmetic, in the same order, as the generic passes.
+
+// One Stockham pass of radix R over NL lines of length N, P = product of the
+// radices already done; element j of line l at l*LSTR + j*ES.
+template <typename T, int N, int R, int P, int NL, int ES, int LSTR, int NT, bool INV>
+__device__ __forceinline__ void fx_pass(int tw_off) {
+  using C = cplx_t<T>;
+  constexpr int M = N / R;       // butterflies per line
+  constexpr int DI = NT / NL;    // threads per line
+  constexpr int KB = (M + DI - 1) / DI;
+  constexpr int TSTEP = N / (P * R);
+  constexpr T sgn = INV ? (T)1 : (T)-1;
+  static_assert(NT % NL == 0 && N % R == 0, "fixed pass shape");
+  C *buf = reinterpret_cast<C *>(tile_lds);
+  const C *tw = buf + tw_off;
+  // opaque thread index: the LDS addresses below are recomputed in every
+  // pass (a few integer operations) instead of being hoisted out of the tile
+  // loop, where they would hold registers the butterflies need
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const int l0 = tid % NL, i0 = tid / NL;
+  const int lbase = l0 * LSTR;
+  C x[KB][R];
+  int dsti[KB], tjk[KB];
+#pragma unroll
+  for (int k = 0; k < KB; ++k) {
+    const int i = i0 + k * DI;
+    dsti[k] = -1;
+    if (KB * DI == M || i < M) {
+      const int kk = P > 1 ? i % P : 0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[k][r] = buf[lbase + (i + r * M) * ES];
+      tjk[k] = kk * TSTEP;
+      dsti[k] = lbase + ((i - kk) * R + kk) * ES;
+    }
+  }
+  __syncthreads();
+  // twiddles after the barrier (the table is never written by the passes):
+  // only one butterfly's roots are live at a time
+#pragma unroll
+  for (int k = 0; k < KB; ++k) {
+    if (dsti[k] >= 0) {
+      if (P > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const C w = tw[r * tjk[k]];
+          const C v = x[k][r];
+          if (INV) {
+            x[k][r].x = v.x * w.x + v.y * w.y;
+            x[k][r].y = v.y * w.x - v.x * w.y;
+          } else {
+            x[k][r] = cmul(v, w);
+          }
+        }
+      }
+      Bfly<T, R>::run(x[k], sgn);
+#pragma unroll
+      for (int q = 0; q < R; ++q) buf[dsti[k] + q * P * ES] = x[k][Bfly<T, R>::perm(q)];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int N, int R1, int R2, int NL, int ES, int LSTR, int NT, bool INV>
+__device__ __forceinline__ void fx_fft(int tw_off) {
+  fx_pass<T, N, R1, 1, NL, ES, LSTR, NT, INV>(tw_off);
+  fx_pass<T, N, R2, R1, NL, ES, LSTR, NT, INV>(tw_off);
+}
+
+// PF = rows of the next tile loaded during the current tile's transforms
+// (the rest are loaded when the tile starts), PC = rows of psi_in and V the
+// combine pass loads before its backward transforms (the rest after them):
+// registers held across the transforms, bounded by the occupancy target.
+template <typename T, int KIND, int N, int R1, int R2, int NT, int PF, int PC>
+__device__ __forceinline__ void fx_body(const TileArgs<T> &A) {
+  using C = cplx_t<T>;
+  constexpr int LS = N | 1;         // LDS row pitch (odd)
+  constexpr int E = N * N / NT;     // rows per thread
+  constexpr int DR = NT / N;        // row step
+  static_assert(NT % N == 0 && E * DR == N && R1 * R2 == N, "fixed tile shape");
+  constexpr bool XY = KIND == kTileFwdXY || KIND == kTileBwdXYCombine;
+  constexpr long ROW = XY ? (long)N : (long)N * N;  // global elements between tile rows
+  constexpr long TILE = XY ? (long)N * N : (long)N; // between tiles of one item
+  constexpr long ITEM = (long)N * N * N;
+  constexpr int twc_off = N * LS, twr_off = twc_off + N;
+  C *buf = reinterpret_cast<C *>(tile_lds);
+  C *twc = buf + twc_off;
+  C *twr = buf + twr_off;
+  T *kxl = reinterpret_cast<T *>(twr + N);
+  T *kzl = kxl + N;
+  // k^2 is separable for these instances (the host checks): x and z tables in LDS
+  constexpr bool ksep = KIND == kTileKinZ;
+  for (int t = threadIdx.x; t < N; t += NT) {
+    twc[t] = A.twc[t];
+    twr[t] = A.twr[t];
+    if (ksep) {
+      kxl[t] = A.kx[t];
+      kzl[t] = A.kz[t];
+    }
+  }
+  const int c0 = threadIdx.x % N, r0 = threadIdx.x / N;
+  const unsigned nb = (unsigned)A.nbatch;
+  const unsigned ntiles = nb * (unsigned)A.tiles_per_item;  // < 2^31 (checked by the host)
+  const unsigned G = gridDim.x;
+  // Global rows are addressed as (per-tile base pointer, uniform) + (32-bit
+  // byte offset of the lane's row within the item, < N^3 * sizeof(C)).  The
+  // empty asm at the top of each tile makes the lane offset opaque, so the
+  // per-row offsets are recomputed per tile instead of being hoisted out of
+  // the loop into registers the transforms need.
+  unsigned voff = (unsigned)(c0 + r0 * ROW) * (unsigned)sizeof(C);
+  constexpr unsigned RSTEP = (unsigned)(DR * ROW * sizeof(C));
+  auto row_ptr = [&](const void *tb, int u) -> const C * {
+    return reinterpret_cast<const C *>(reinterpret_cast<const char *>(tb) + (size_t)(voff + (unsigned)u * RSTEP));
+  };
+  auto row_ptr_w = [&](void *tb, int u) -> C * {
+    return reinterpret_cast<C *>(reinterpret_cast<char *>(tb) + (size_t)(voff + (unsigned)u * RSTEP));
+  };
+  auto tile_off = [&](unsigned tt) -> long {
+    const unsigned w = tt / nb, item = tt - w * nb;
+    return (long)item * ITEM + (long)w * TILE;
+  };
+  static_assert(PF >= 0 && PF <= E && PC >= 0 && PC <= E, "prefetch depth");
+  T vr[E], vi[E];
+  auto issue = [&](unsigned tt, auto u0, auto u1) {
+    const C *s = A.src + tile_off(tt);
+#pragma unroll
+    for (int u = decltype(u0)::value; u < decltype(u1)::value; ++u) {
+      const C v = *row_ptr(s, u);
+      vr[u] = v.x;
+      vi[u] = v.y;
+    }
+  };
+  using I0 = std::integral_constant<int, 0>;
+  using IPF = std::integral_constant<int, PF>;
+  using IE = std::integral_constant<int, E>;
+  unsigned t = blockIdx.x;
+  if (t < ntiles) issue(t, I0{}, IPF{});
+  for (; t < ntiles; t += G) {
+    asm volatile("" : "+v"(voff));
+    const unsigned w = t / nb, item = t - w * nb;
+    const long tb = (long)item * ITEM + (long)w * TILE;
+    issue(t, IPF{}, IE{});  // rows not prefetched
+    __syncthreads();  // the previous tile's LDS reads are done (first tile: the tables are written)
+#pragma unroll
+    for (int u = 0; u < E; ++u) buf[(r0 + u * DR) * LS + c0] = C{vr[u], vi[u]};
+    __syncthreads();
+    const bool more = t + G < ntiles;
+    if (KIND == kTileFwdXY) {
+      if (more) issue(t + G, I0{}, IPF{});
+      fx_fft<T, N, R1, R2, N, 1, LS, NT, false>(twc_off);  // x rows
+      fx_fft<T, N, R1, R2, N, LS, 1, NT, false>(twr_off);  // y columns
+      C *d = A.dst + tb;
+#pragma unroll
+      for (int u = 0; u < E; ++u) *row_ptr_w(d, u) = buf[(r0 + u * DR) * LS + c0];
+    } else if (KIND == kTileKinZ) {
+      if (more) issue(t + G, I0{}, IPF{});
+      fx_fft<T, N, R1, R2, N, LS, 1, NT, false>(twr_off);  // z columns
+      const T kxy = kxl[c0] + A.ky[w];
+#pragma unroll
+      for (int u = 0; u < E; ++u) {
+        const int r = r0 + u * DR;
+        T k2 = kxy + kzl[r];
+        if (k2 > A.kcap) k2 = A.kcap;
+        C v = buf[r * LS + c0];
+        v.x *= k2;
+        v.y *= k2;
+        buf[r * LS + c0] = v;
+      }
+      __syncthreads();
+      fx_fft<T, N, R1, R2, N, LS, 1, NT, true>(twr_off);
+      C *d = A.dst + tb;
+#pragma unroll
+      for (int u = 0; u < E; ++u) *row_ptr_w(d, u) = buf[(r0 + u * DR) * LS + c0];
+    } else {  // kTileBwdXYCombine
+      const T a0 = A.a0_tab ? (T)A.a0_tab[*A.step_base + A.step_off] : A.a0;
+      // psi_in and V rows of this tile, in flight during the backward transforms
+      const T *vl = A.vloc + (long)(item % (unsigned)A.nspin) * ITEM + (long)w * TILE + c0;
+      const C *pin = A.psi_in + tb;
+      T pr[E], pim[E], vv[E];
+      auto issue_in = [&](auto u0, auto u1) {
+#pragma unroll
+        for (int u = decltype(u0)::value; u < decltype(u1)::value; ++u) {
+          vv[u] = vl[(long)(r0 + u * DR) * ROW];
+          const C q = *row_ptr(pin, u);
+          pr[u] = q.x;
+          pim[u] = q.y;
+        }
+      };
+      issue_in(I0{}, std::integral_constant<int, PC>{});
+      fx_fft<T, N, R1, R2, N, LS, 1, NT, true>(twr_off);  // y columns
+      fx_fft<T, N, R1, R2, N, 1, LS, NT, true>(twc_off);  // x rows
+      issue_in(std::integral_constant<int, PC>{}, IE{});
+      const bool expand = A.rec_off != nullptr;
+      C *d = A.dst + tb;
+#pragma unroll
+      for (int u = 0; u < E; ++u) {
+        const int r = r0 + u * DR;
+        const C q = buf[r * LS + c0];
+        C o;
+        o.x = A.a1 * (q.x + vv[u] * pr[u]) + a0 * pr[u];
+        o.y = A.a1 * (q.y + vv[u] * pim[u]) + a0 * pim[u];
+        if (expand || A.fwd_dst) buf[r * LS + c0] = o;
+        if (!expand) *row_ptr_w(d, u) = o;
+      }
+      if (more) issue(t + G, I0{}, IPF{});
+      if (expand) {
+        // projector expansion in rounds per rank, as in tile_body
+        __syncthreads();
+        const C *dcol = A.pjd + (long)item * A.pj_natoms * 6;
+        const int2 *pa = reinterpret_cast<const int2 *>(A.rec_pa);
+        for (int k = 0; k < A.pt_ranks; ++k) {
+          const int e0 = A.rec_off[w * A.pt_ranks + k], e1 = A.rec_off[w * A.pt_ranks + k + 1];
+          for (int e = e0 + threadIdx.x; e < e1; e += 2 * NT) {
+            const int eb = e + NT;
+            const bool two = eb < e1;
+            const int2 qa = pa[e];
+            const int2 qb = two ? pa[eb] : qa;
+            const C ta = record_term<T>(A.rec_val + (long)e * 8, dcol + (long)qa.y * 6);
+            const C tb2 = record_term<T>(A.rec_val + (long)(two ? eb : e) * 8, dcol + (long)qb.y * 6);
+            const int pos_a = (qa.x >> 16) * LS + (qa.x & 0xffff);
+            C o = buf[pos_a];
+            o.x += ta.x;
+            o.y += ta.y;
+            buf[pos_a] = o;
+            if (two) {
+              const int pos_b = (qb.x >> 16) * LS + (qb.x & 0xffff);
+              C q = buf[pos_b];
+              q.x += tb2.x;
+              q.y += tb2.y;
+              buf[pos_b] = q;
+            }
+          }
+          __syncthreads();
+        }
+#pragma unroll
+        for (int u = 0; u < E; ++u) *row_ptr_w(d, u) = buf[(r0 + u * DR) * LS + c0];
+      }
+      if (A.fwd_dst) {
+        __syncthreads();
+        fx_fft<T, N, R1, R2, N, 1, LS, NT, false>(twc_off);  // x rows
+        fx_fft<T, N, R1, R2, N, LS, 1, NT, false>(twr_off);  // y columns
+        C *fd = A.fwd_dst + tb;
+#pragma unroll
+        for (int u = 0; u < E; ++u) *row_ptr_w(fd, u) = buf[(r0 + u * DR) * LS + c0];
+      }
+    }
+  }
+}
+
+#define FBSE_FX_KERNEL(NAME, T, N, R1, R2, NT, WAVES, PF, PC)                                          \
+  template <int KIND>                                                                                 \
+  __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WAVES))) void NAME(TileArgs<T> A) { \
+    fx_body<T, KIND, N, R1, R2, NT, PF, PC>(A);                                                       \
+  }
+// No register prefetch (PF = PC = 0): on 128 FP32 grids of 96^3 every
+// prefetch depth tried (6 or 12 rows of the next tile, 6 or 12 rows of
+// psi_in/V) ran 5-55% slower than none, because the rows held across the
+// transforms push the waves past the 80 VGPRs that two resident work-groups
+// per CU allow.  Without it the fixed instances take the forward xy pass from
+// 569 to 396 us, the z-slab pass from 615 to 423 us and the combine pass
+// (with the next forward transform) from 978 to 770 us.
+FBSE_FX_KERNEL(fft_fx_f32_n96, float, 96, 12, 8, 768, 6, 0, 0)
+FBSE_FX_KERNEL(fft_fx_f32_n64, float, 64, 8, 8, 512, 4, 0, 0)
+FBSE_FX_KERNEL(fft_fx_f64_n96, double, 96, 12, 8, 768, 3, 0, 0)
+FBSE_FX_KERNEL(fft_fx_f64_n64, double, 64, 8, 8, 512, 4, 0, 0)
+#undef FBSE_FX_KERNEL
+
+// the fixed instance for a grid (0 when there is none)
+template <typename T> static int fixed_n(const fbse_ctx *c) {
+  if (c->nx != c->ny || c->nx != c->nz || !c->ksqr_sep) return 0;
+  return (c->nx == 96 || c->nx == 64) ? (int)c->nx : 0;
+}
+template <typename T, int KIND> static const void *fx_fn(int n) {
+  if constexpr (sizeof(T) == 4) {
+    return n == 96 ? (const void *)fft_fx_f32_n96<KIND> : (const void *)fft_fx_f32_n64<KIND>;
+  } else {
+    return n == 96 ? (const void *)fft_fx_f64_n96<KIND> : (const void *)fft_fx_f64_n64<KIND>;
+  }
+}
+static int fx_threads(int n) { return n == 96 ? 768 : 512; }
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 static bool radices_ok(const AxisPlan &pl) {
@@ -495,6 +788,16 @@ template <typename T> bool tiles_supported(const fbse_ctx *c) {
 template bool tiles_supported<double>(const fbse_ctx *);
 template bool tiles_supported<float>(const fbse_ctx *);
 
+template <typename T> bool tiles_fixed(const fbse_ctx *c) {
+  if (!fixed_n<T>(c)) return false;
+  int nv;
+  tile_variants<T>(&nv);
+  if (c->force_variant == 1 && c->force_lines > 0) return c->force_lines == nv + 1;
+  return c->tile_variant[sizeof(T) == 8 ? 0 : 1] == nv;
+}
+template bool tiles_fixed<double>(const fbse_ctx *);
+template bool tiles_fixed<float>(const fbse_ctx *);
+
 template <typename T> static const cplx_t<T> *twiddles(const fbse_ctx *c, int axis);
 template <> const double2 *twiddles<double>(const fbse_ctx *c, int axis) { return c->d_tw[axis]; }
 template <> const float2 *twiddles<float>(const fbse_ctx *c, int axis) { return c->d_tw_f[axis]; }
@@ -506,6 +809,48 @@ template <typename T, int KIND> static void set_lds_limit(int device, int v) {
     (void)hipFuncSetAttribute(tile_fn<T, KIND>(v), hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (device >= 0 && device < 64) done[device][v] = true;
   }
+}
+
+template <typename T, int KIND>
+static void fx_launch(int n, dim3 g, dim3 b, size_t lds, hipStream_t st, const TileArgs<T> &A) {
+  if constexpr (sizeof(T) == 4) {
+    if (n == 96)
+      hipLaunchKernelGGL((fft_fx_f32_n96<KIND>), g, b, lds, st, A);
+    else
+      hipLaunchKernelGGL((fft_fx_f32_n64<KIND>), g, b, lds, st, A);
+  } else {
+    if (n == 96)
+      hipLaunchKernelGGL((fft_fx_f64_n96<KIND>), g, b, lds, st, A);
+    else
+      hipLaunchKernelGGL((fft_fx_f64_n64<KIND>), g, b, lds, st, A);
+  }
+}
+
+// Work-groups of a persistent fixed-instance launch: the number resident at
+// once (occupancy x CUs, cached per device, precision, size and pass), at
+// most one per tile; 0 when the instance cannot run.
+template <typename T, int KIND> static int fx_grid(fbse_ctx *c, int n, long ntiles) {
+  static int cache[64][2][2][5] = {};
+  const int d = c->device, pi = sizeof(T) == 8 ? 0 : 1, ni = n == 96 ? 1 : 0;
+  if (d < 0 || d >= 64) return 0;
+  int &res = cache[d][pi][ni][KIND];
+  if (res == 0) {
+    const void *fn = fx_fn<T, KIND>(n);
+    const size_t lds = tile_lds_bytes<T>(n, n);
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess) {
+      (void)hipGetLastError();
+      return 0;
+    }
+    int per_cu = 0, ncu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, fx_threads(n), lds) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, d) != hipSuccess) {
+      (void)hipGetLastError();
+      return 0;
+    }
+    res = per_cu > 0 && ncu > 0 ? per_cu * ncu : -1;
+  }
+  if (res < 0) return 0;
+  return (int)std::min<long>(res, ntiles);
 }
 
 template <typename T>
@@ -574,6 +919,31 @@ static int tile_pass_v(fbse_ctx *c, int v, int kind, const cplx_t<T> *src, cplx_
   if (ntiles > 0x7fffffffL) return set_err(c, FBSE_ERR_UNSUPPORTED, "%ld FFT tiles in one launch", ntiles);
   int nv;
   const TileVariant *tv = tile_variants<T>(&nv);
+  if (v == nv) {  // the fixed-schedule persistent instance
+    const int n = fixed_n<T>(c);
+    if (!n || A.col_split != 1 || kind == kTileBwdXY || kind == kTilePrecZ)
+      return set_err(c, FBSE_ERR_UNSUPPORTED, "no fixed FFT instance for this pass");
+    int g;
+    switch (kind) {
+      case kTileFwdXY: g = fx_grid<T, kTileFwdXY>(c, n, ntiles); break;
+      case kTileKinZ: g = fx_grid<T, kTileKinZ>(c, n, ntiles); break;
+      default: g = fx_grid<T, kTileBwdXYCombine>(c, n, ntiles); break;
+    }
+    if (g <= 0) return set_err(c, FBSE_ERR_UNSUPPORTED, "fixed FFT instance not resident");
+    const size_t lds = tile_lds_bytes<T>(n, n);
+    hipEvent_t ev = nullptr;
+    const int pk = kind == kTileBwdXYCombine ? 1 : 0;
+    prof_begin(c, pk, &ev);
+    const dim3 gd((unsigned)g), bd((unsigned)fx_threads(n));
+    switch (kind) {
+      case kTileFwdXY: fx_launch<T, kTileFwdXY>(n, gd, bd, lds, c->stream, A); break;
+      case kTileKinZ: fx_launch<T, kTileKinZ>(n, gd, bd, lds, c->stream, A); break;
+      default: fx_launch<T, kTileBwdXYCombine>(n, gd, bd, lds, c->stream, A); break;
+    }
+    FBSE_HIP(c, hipGetLastError());
+    prof_end(c, pk, ev);
+    return FBSE_OK;
+  }
   A.nfc = tile_schedule(pc.n, tv[v].e, &A.fc);
   A.nfr = tile_schedule(pr.n, tv[v].e, &A.fr);
   const int nt = tile_threads(A.nr, A.nc, tv[v].e, tv[v].max_nt);
@@ -673,33 +1043,30 @@ template <typename T> static int slab_split(fbse_ctx *c, int v, const cplx_t<T> 
 // Instance choice: the autotuner times every instance that fits on the
 // first batch of at least 1M grid points (forward xy pass into a scratch
 // buffer, best of 3) and keeps the fastest per precision; smaller batches
-// before that use the first instance that fits.
+// before that use the first instance that fits.  Instance index nv (one past
+// the generic ones) is the fixed-schedule persistent instance of the grid,
+// when it has one; it takes whole z slabs and has no preconditioner passes
+// (those run on the first generic instance that fits).
 template <typename T>
 static int tile_pass(fbse_ctx *c, int kind, const cplx_t<T> *src, cplx_t<T> *dst, long nbatch,
                      const cplx_t<T> *psi_in, T a1, T a0, const cplx_t<T> *pjd = nullptr,
                      cplx_t<T> *fwd_dst = nullptr) {
   const int pi = sizeof(T) == 8 ? 0 : 1;
-  if (kind == kTileKinZ || kind == kTilePrecZ) {
-    int nv;
-    tile_variants<T>(&nv);
-    const int v = c->tile_variant[pi] >= 0 ? c->tile_variant[pi] : first_tile_variant<T>(c);
-    const int vv = (c->force_variant == 1 && c->force_lines > 0) ? c->force_lines - 1 : v;
-    if (vv < 0 || vv >= nv) return set_err(c, FBSE_ERR_ARG, "FFT tile instance %d of %d", vv + 1, nv);
-    const int zs = slab_split<T>(c, vv, src, nbatch);
-    return tile_pass_v<T>(c, vv, kind, src, dst, nbatch, psi_in, a1, a0, pjd, fwd_dst, zs);
-  }
-  if (c->force_variant == 1 && c->force_lines > 0) {  // forced instance (1-based)
-    int nv;
-    tile_variants<T>(&nv);
-    if (c->force_lines > nv) return set_err(c, FBSE_ERR_ARG, "FFT tile instance %d of %d", c->force_lines, nv);
-    return tile_pass_v<T>(c, c->force_lines - 1, kind, src, dst, nbatch, psi_in, a1, a0, pjd, fwd_dst);
-  }
-  if (c->tile_variant[pi] < 0) {
-    int nv;
-    const TileVariant *tv = tile_variants<T>(&nv);
+  int nv;
+  const TileVariant *tv = tile_variants<T>(&nv);
+  const bool has_fx = fixed_n<T>(c) > 0;
+  const bool fx_kind = kind == kTileFwdXY || kind == kTileKinZ || kind == kTileBwdXYCombine;
+  const bool forced = c->force_variant == 1 && c->force_lines > 0;
+  if (forced && (c->force_lines > nv + 1 || (c->force_lines == nv + 1 && !has_fx)))
+    return set_err(c, FBSE_ERR_ARG, "FFT tile instance %d of %d", c->force_lines, nv + (has_fx ? 1 : 0));
+  if (!forced && c->tile_variant[pi] < 0) {
     int v0 = first_tile_variant<T>(c);
-    if (!(c->autotune && nbatch * c->ngrid >= (1L << 20)))
+    if (!(c->autotune && nbatch * c->ngrid >= (1L << 20))) {
+      if (kind == kTileKinZ || kind == kTilePrecZ)
+        return tile_pass_v<T>(c, v0, kind, src, dst, nbatch, psi_in, a1, a0, pjd, fwd_dst,
+                              slab_split<T>(c, v0, src, nbatch));
       return tile_pass_v<T>(c, v0, kind, src, dst, nbatch, psi_in, a1, a0, pjd, fwd_dst);
+    }
     cplx_t<T> *scratch = nullptr;
     const size_t bytes = (size_t)nbatch * c->ngrid * sizeof(cplx_t<T>);
     int best_v = v0;
@@ -710,9 +1077,10 @@ static int tile_pass(fbse_ctx *c, int kind, const cplx_t<T> *src, cplx_t<T> *dst
       const int saved_profile = c->profile;
       c->profile = 0;
       double best = 1e30;
-      for (int v = 0; v < nv; ++v) {
-        if (!tile_threads((int)c->ny, (int)c->nx, tv[v].e, tv[v].max_nt) ||
-            !tile_threads((int)c->nz, (int)c->nx, tv[v].e, tv[v].max_nt))
+      for (int v = 0; v <= nv; ++v) {
+        if (v == nv ? !has_fx
+                    : (!tile_threads((int)c->ny, (int)c->nx, tv[v].e, tv[v].max_nt) ||
+                       !tile_threads((int)c->nz, (int)c->nx, tv[v].e, tv[v].max_nt)))
           continue;
         double tmin = 1e30;
         for (int rep = 0; rep < 3; ++rep) {
@@ -744,7 +1112,14 @@ static int tile_pass(fbse_ctx *c, int kind, const cplx_t<T> *src, cplx_t<T> *dst
     }
     c->tile_variant[pi] = best_v;
   }
-  return tile_pass_v<T>(c, c->tile_variant[pi], kind, src, dst, nbatch, psi_in, a1, a0, pjd, fwd_dst);
+  int v = forced ? c->force_lines - 1 : c->tile_variant[pi];
+  if (v == nv && !fx_kind) v = first_tile_variant<T>(c);
+  if (v < 0 || v > nv) return set_err(c, FBSE_ERR_ARG, "FFT tile instance %d of %d", v + 1, nv);
+  if (kind == kTileKinZ || kind == kTilePrecZ) {
+    const int zs = v == nv ? 1 : slab_split<T>(c, v, src, nbatch);
+    return tile_pass_v<T>(c, v, kind, src, dst, nbatch, psi_in, a1, a0, pjd, fwd_dst, zs);
+  }
+  return tile_pass_v<T>(c, v, kind, src, dst, nbatch, psi_in, a1, a0, pjd, fwd_dst);
 }
 
 // out = a1*(T + V) psi + a0*psi over nstates*nspin grids (projectors are

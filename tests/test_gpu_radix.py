@@ -38,3 +38,32 @@ def test_tile_radices(grid):
             out = g.apply_h(psi)
             assert rel(out, ref) < 1e-12, (inst, split)
     g.set_slab_split(0)
+
+
+@pytest.mark.parametrize("n", [64, 96])
+def test_fixed_instances(n):
+    """The fixed-schedule persistent instances of cubic 64^3 and 96^3 grids
+    (FP64 instance 3, FP32 instance 4) against the generic instance 1: H in
+    FP64 to FFT rounding, and the mixed filter (FP32 recurrence through the
+    fixed forward, z-slab and combine passes with the projector expansion)
+    to FP32 rounding."""
+    s = small_system(n=(n, n, n), dx=0.5, nspin=2, is_complex=1, so=1, nl=1, natoms=6,
+                     n_so_proj=1, n_nl_proj=1)
+    print("case fixed instance %d^3" % n, flush=True)
+    g = gpu_context(s)
+    psi = random_states(s, 2, seed=7)
+    g.set_fft_kernel(1, 1)
+    ref = g.apply_h(psi)
+    g.set_fft_kernel(1, 3)
+    out = g.apply_h(psi)
+    assert rel(out, ref) < 1e-12
+    # mixed filter: 2 starts x 2 targets, 12 Newton steps
+    from test_gpu_parity import filter_inputs
+    psi_rank, stlen, an, zn, Vmin, dE_1 = filter_inputs(s, 2, 2, 12)
+    g.set_fft_kernel(1, 1)
+    fref = g.filter(psi_rank, 2, 2, an, zn, Vmin, dE_1, precision=1)
+    g.set_fft_kernel(1, 4)
+    fout = g.filter(psi_rank, 2, 2, an, zn, Vmin, dE_1, precision=1)
+    g.set_fft_kernel(-1, 0)
+    assert rel(fout, fref) < 1e-5
+    g.close()
