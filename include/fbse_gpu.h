@@ -192,7 +192,9 @@ int fbse_set_window(fbse_ctx *ctx, int window);
  * 160 KiB of LDS; otherwise the per-axis passes run).  lines > 0 fixes,
  * instead of autotuning, the tile height of the per-axis passes (variant 0)
  * or the plane/slab kernel instance, 1-based (variant 1: FP32 instances
- * 1..3 stage 12, 16, 32 elements per thread, FP64 instances 1..2 stage 12, 16). */
+ * 1..3 stage 12, 16, 32 elements per thread, FP64 instances 1..2 stage 12, 16;
+ * on cubic 64^3 and 96^3 grids with a separable k^2 table one more instance,
+ * FP32 4 / FP64 3, runs the fixed-schedule persistent passes). */
 int fbse_set_fft_kernel(fbse_ctx *ctx, int variant, int lines);
 
 /* Plane/slab passes: the z-slab pass cuts each xz slab into `split` tiles of
