@@ -524,7 +524,7 @@ def main():
     else:
         launches, ms_tot = prof[0]
         bytes_per_launch = 2.0 * grid_bytes
-        kname = "fft_axis_kernel<float>"
+        kname = "per-axis FFT passes (axis_fx_f32_n%d_l* where the axis length has a fixed instance, else fft_axis_kernel<float>)" % job["nx"]
         kdesc = "x/y forward, z kinetic, y backward passes"
         zslab = None
     avg_ms = ms_tot / max(launches, 1)

@@ -550,7 +550,8 @@ extern "C" int fbse_set_window(fbse_ctx *c, int w) {
   return FBSE_OK;
 }
 extern "C" int fbse_set_fft_kernel(fbse_ctx *c, int variant, int lines) {
-  if (!c || variant < -1 || variant > 1 || lines < 0 || lines > 256) return FBSE_ERR_ARG;
+  if (!c || variant < -1 || variant > 1 || lines < 0 || (lines > 256 && !(variant == 0 && (lines == 1016 || lines == 1032))))
+    return FBSE_ERR_ARG;
   c->force_variant = variant;
   c->force_lines = lines;
   return FBSE_OK;

@@ -66,6 +66,7 @@ template <typename T> struct AxisArgs {
   int step_off;
   // preconditioner (z pass): g = pa / (1 + k2 * pb)
   T pa, pb;
+  long ntiles;             // tiles of the launch (fixed-schedule instances loop over them)
 };
 
 enum PassKind { kPassFwd = 0, kPassBwd = 1, kPassKin = 2, kPassBwdCombine = 3, kPassPrec = 4 };
@@ -251,6 +252,10 @@ int fft_pass(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src, cplx_t<T> *d
 template <typename T> bool tiles_supported(const fbse_ctx *c);
 // the plane/slab passes of precision T run the fixed-schedule instance
 template <typename T> bool tiles_fixed(const fbse_ctx *c);
+// Fixed-schedule per-axis passes (fft_tile_kernels.hip): axis lengths with an
+// instance, and the launch of one (lines = 16 or 32; A.ntiles tiles)
+bool axis_fx_length(int n);
+template <typename T> int axis_fx_pass(fbse_ctx *c, int kind, const AxisArgs<T> &A, int lines);
 // pjd != nullptr: the combine pass also adds the projector expansion with the
 // coefficients of proj_project
 // fwd_next: the combine pass also leaves the forward xy transform of out in

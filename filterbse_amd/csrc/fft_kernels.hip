@@ -344,8 +344,12 @@ static int fft_pass_lines(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src,
   A.n = pl.n;
   A.nf = sch.nf;
   for (int i = 0; i < sch.nf; ++i) A.f[i] = sch.f[i];
-  // lines per tile: the requested value, clamped so the tile never exceeds the data
-  int lines = lines_req > 0 ? lines_req : pl.lines;
+  // lines per tile: the requested value, clamped so the tile never exceeds the
+  // data; 1000 + L asks for the fixed-schedule instance with L lines, which
+  // keeps L (a last partial tile is masked) where the axis length has one
+  const bool kin = kind == kPassKin || kind == kPassPrec;
+  const bool fx = lines_req >= 1000 && axis_fx_length(pl.n) && !(kin && !c->ksqr_sep);
+  int lines = lines_req >= 1000 ? lines_req - 1000 : (lines_req > 0 ? lines_req : pl.lines);
   A.psi_in = psi_in;
   A.a1 = a1;
   A.a0 = a0;
@@ -362,20 +366,20 @@ static int fft_pass_lines(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src,
     A.xaxis = 1;
     A.inner = 1;
     A.total_lines = nbatch * c->ny * c->nz;
-    if (lines > A.total_lines) lines = (int)A.total_lines;
+    if (!fx && lines > A.total_lines) lines = (int)A.total_lines;
     ntiles = (A.total_lines + lines - 1) / lines;
   } else {
     A.xaxis = 0;
     A.inner = axis == 1 ? c->nx : c->nx * c->ny;
     A.inner_count = A.inner;
     A.outer_stride = A.inner * pl.n;
-    if (lines > A.inner_count) lines = (int)A.inner_count;
+    if (!fx && lines > A.inner_count) lines = (int)A.inner_count;
     A.tiles_per_outer = (A.inner_count + lines - 1) / lines;
     const long nouter = axis == 1 ? nbatch * c->nz : nbatch;
     ntiles = nouter * A.tiles_per_outer;
   }
   // keep the tile within the LDS a work-group may take
-  while (lines > 1 && axis_lds_bytes(pl.n, lines, sizeof(cplx_t<T>)) > kAxisLdsCap) --lines;
+  while (!fx && lines > 1 && axis_lds_bytes(pl.n, lines, sizeof(cplx_t<T>)) > kAxisLdsCap) --lines;
   if (axis == 0) {
     ntiles = (A.total_lines + lines - 1) / lines;
   } else {
@@ -383,6 +387,19 @@ static int fft_pass_lines(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src,
     ntiles = (axis == 1 ? nbatch * c->nz : nbatch) * A.tiles_per_outer;
   }
   A.lines = lines;
+  A.ntiles = ntiles;
+  if (fx) {
+    hipEvent_t ev = nullptr;
+    const int pk = (kind == kPassBwdCombine) ? 1 : 0;
+    prof_begin(c, pk, &ev);
+    const int rc = axis_fx_pass<T>(c, kind, A, lines);
+    if (rc) {
+      if (ev) c->ev_pool.push_back(ev);
+      return rc;
+    }
+    prof_end(c, pk, ev);
+    return FBSE_OK;
+  }
   const size_t lds = axis_lds_bytes(pl.n, lines, sizeof(cplx_t<T>));
   if (lds > kAxisLdsCap) return set_err(c, FBSE_ERR_UNSUPPORTED, "axis length %d needs %zu bytes of LDS", pl.n, lds);
   hipEvent_t ev = nullptr;
@@ -428,7 +445,9 @@ static int tune_pass(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src, long
   const int pi = sizeof(T) == 8 ? 0 : 1, cls = pass_class(kind);
   const int bk_ = nbatch * c->ngrid >= (32L << 20) ? 1 : 0;
   const AxisPlan &pl = c->plan[axis];
-  const int cand[] = {4, 8, 16, 32, 64};
+  // tile heights of the generic kernel, then the fixed-schedule instances
+  // (1000 + lines) where the axis length has one
+  const int cand[] = {4, 8, 16, 32, 64, 1016, 1032};
   const size_t bytes = (size_t)nbatch * c->ngrid * sizeof(cplx_t<T>);
   cplx_t<T> *scratch = nullptr;
   if (hipMalloc(&scratch, bytes) != hipSuccess) {
@@ -444,6 +463,7 @@ static int tune_pass(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src, long
   double best = 1e30;
   int bl = pl.lines;
   for (int L : cand) {
+    if (L >= 1000 && !axis_fx_length(pl.n)) continue;
     double tmin = 1e30;
     bool ok = true;
     for (int rep = 0; rep < 3 && ok; ++rep) {

@@ -689,6 +689,221 @@ FBSE_FX_KERNEL(fft_fx_f64_n96, double, 96, 12, 8, 768, 3, 0, 0)
 FBSE_FX_KERNEL(fft_fx_f64_n64, double, 64, 8, 8, 512, 4, 0, 0)
 #undef FBSE_FX_KERNEL
 
+// ---------------------------------------------------------------------------
+// Fixed-schedule per-axis passes (axis lengths 160 = 16*10 and 256 = 16*16)
+// ---------------------------------------------------------------------------
+// The per-axis design of fft_kernels.hip (tiles of L lines of one axis, five
+// sweeps per H) with the compile-time machinery above: the lines of a tile
+// sit in LDS with an odd pitch and are transformed in place by the inlined
+// passes, and each work-group loops over tiles.  x tiles are L consecutive
+// rows; y and z tiles are L consecutive inner indices (each axis point a
+// contiguous 8*L- or 16*L-byte segment).
+template <typename T, int KIND, int N, int R1, int R2, int L, int NT, bool XAX>
+__device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
+  using C = cplx_t<T>;
+  constexpr int LS = N | 1, E = N * L / NT;
+  constexpr int LB = E % 8 == 0 ? 8 : (E % 5 == 0 ? 5 : E);  // global rows in flight per batch
+  static_assert(NT % L == 0 && E * NT == N * L && E % LB == 0 && R1 * R2 == N, "fixed axis tile shape");
+  constexpr int tw_off = L * LS;
+  C *buf = reinterpret_cast<C *>(tile_lds);
+  C *tw = buf + tw_off;
+  T *kzl = reinterpret_cast<T *>(tw + N);
+  T *kxyl = kzl + N;
+  constexpr bool KIN = KIND == kPassKin || KIND == kPassPrec;
+  constexpr bool INV = KIND == kPassBwd || KIND == kPassBwdCombine;
+  for (int t = threadIdx.x; t < N; t += NT) {
+    tw[t] = A.tw[t];
+    if (KIN) kzl[t] = A.kz[t];
+  }
+  // element e = tid + u*NT of a tile: line l, point p; global offset (within
+  // the tile, elements) XAX ? e : p*inner + l, below 2^31 / sizeof(C)
+  int tid = threadIdx.x;
+  for (long t = blockIdx.x; t < A.ntiles; t += gridDim.x) {
+    asm volatile("" : "+v"(tid));  // per-tile recomputation of the lane's offsets
+    long base, i0 = 0, line0 = 0;
+    int nl;
+    if (XAX) {
+      line0 = t * L;
+      base = line0 * N;
+      const long rem = A.total_lines - line0;
+      nl = rem < L ? (int)rem : L;
+    } else {
+      const long o = t / A.tiles_per_outer;
+      i0 = (t - o * A.tiles_per_outer) * L;
+      base = o * A.outer_stride + i0;
+      const long rem = A.inner_count - i0;
+      nl = rem < L ? (int)rem : L;
+    }
+    const unsigned inner = (unsigned)A.inner;
+    auto goff = [&](int e) -> unsigned {
+      return XAX ? (unsigned)e : (unsigned)(e / L) * inner + (unsigned)(e % L);
+    };
+    auto lpos = [&](int e) -> int { return XAX ? (e / N) * LS + e % N : (e % L) * LS + e / L; };
+    auto line = [&](int e) -> int { return XAX ? e / N : e % L; };
+    const C *src = A.src + base;
+    __syncthreads();  // the previous tile's LDS reads are done (first tile: tables written)
+#pragma unroll
+    for (int ub = 0; ub < E; ub += LB) {
+      T vr[LB], vi[LB];
+#pragma unroll
+      for (int u = 0; u < LB; ++u) {
+        const int e = tid + (ub + u) * NT;
+        if (line(e) < nl) {
+          const C v = src[goff(e)];
+          vr[u] = v.x;
+          vi[u] = v.y;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < LB; ++u) {
+        const int e = tid + (ub + u) * NT;
+        if (line(e) < nl) buf[lpos(e)] = C{vr[u], vi[u]};
+      }
+    }
+    if (KIN) {
+      for (int l = tid; l < nl; l += NT) {
+        const long in = i0 + l;
+        const long jy = in / A.nx, jx = in - jy * A.nx;
+        kxyl[l] = A.kx[jx] + A.ky[jy];
+      }
+    }
+    __syncthreads();
+    // lines l >= nl of a last partial tile hold stale values: transformed, never stored
+    fx_fft<T, N, R1, R2, L, 1, LS, NT, INV>(tw_off);
+    if (KIN) {
+#pragma unroll
+      for (int u = 0; u < E; ++u) {
+        const int e = tid + u * NT;
+        const int l = e / N, p = e % N;
+        T k2 = kxyl[l] + kzl[p];
+        if (k2 > A.kcap) k2 = A.kcap;
+        if (KIND == kPassPrec) k2 = A.pa / ((T)1 + k2 * A.pb);
+        C v = buf[l * LS + p];
+        v.x *= k2;
+        v.y *= k2;
+        buf[l * LS + p] = v;
+      }
+      __syncthreads();
+      fx_fft<T, N, R1, R2, L, 1, LS, NT, true>(tw_off);
+    }
+    C *dst = A.dst + base;
+    if (KIND == kPassBwdCombine) {  // x axis: out = a1*(T psi + V psi_in) + a0*psi_in
+      const T a0 = A.a0_tab ? (T)A.a0_tab[*A.step_base + A.step_off] : A.a0;
+      const C *pin = A.psi_in + base;
+#pragma unroll
+      for (int u = 0; u < E; ++u) {
+        const int e = tid + u * NT;
+        const int l = e / N, p = e % N;
+        if (l < nl) {
+          const long row = (line0 + l) % A.vloc_rows;
+          const T vv = A.vloc[row * N + p];
+          const C q = pin[e];
+          const C r = buf[l * LS + p];
+          C o;
+          o.x = A.a1 * (r.x + vv * q.x) + a0 * q.x;
+          o.y = A.a1 * (r.y + vv * q.y) + a0 * q.y;
+          dst[e] = o;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < E; ++u) {
+        const int e = tid + u * NT;
+        if (line(e) < nl) dst[goff(e)] = buf[lpos(e)];
+      }
+    }
+  }
+}
+
+#define FBSE_AXIS_FX(NAME, T, N, R1, R2, L, NT, WAVES)                                                      \
+  template <int KIND, bool XAX>                                                                          \
+  __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WAVES))) void NAME(AxisArgs<T> A) {  \
+    axis_fx_body<T, KIND, N, R1, R2, L, NT, XAX>(A);                                                     \
+  }
+FBSE_AXIS_FX(axis_fx_f32_n256_l16, float, 256, 16, 16, 16, 256, 4)
+FBSE_AXIS_FX(axis_fx_f32_n256_l32, float, 256, 16, 16, 32, 512, 4)
+FBSE_AXIS_FX(axis_fx_f32_n160_l16, float, 160, 16, 10, 16, 256, 4)
+FBSE_AXIS_FX(axis_fx_f32_n160_l32, float, 160, 16, 10, 32, 512, 4)
+FBSE_AXIS_FX(axis_fx_f64_n256_l16, double, 256, 16, 16, 16, 256, 2)
+FBSE_AXIS_FX(axis_fx_f64_n256_l32, double, 256, 16, 16, 32, 512, 2)
+FBSE_AXIS_FX(axis_fx_f64_n160_l16, double, 160, 16, 10, 16, 256, 2)
+FBSE_AXIS_FX(axis_fx_f64_n160_l32, double, 160, 16, 10, 32, 512, 2)
+#undef FBSE_AXIS_FX
+
+bool axis_fx_length(int n) { return n == 160 || n == 256; }
+
+template <typename T, int KIND, bool XAX> static const void *axis_fx_fn(int n, int lines) {
+  if constexpr (sizeof(T) == 4) {
+    if (n == 256)
+      return lines == 16 ? (const void *)axis_fx_f32_n256_l16<KIND, XAX> : (const void *)axis_fx_f32_n256_l32<KIND, XAX>;
+    return lines == 16 ? (const void *)axis_fx_f32_n160_l16<KIND, XAX> : (const void *)axis_fx_f32_n160_l32<KIND, XAX>;
+  } else {
+    if (n == 256)
+      return lines == 16 ? (const void *)axis_fx_f64_n256_l16<KIND, XAX> : (const void *)axis_fx_f64_n256_l32<KIND, XAX>;
+    return lines == 16 ? (const void *)axis_fx_f64_n160_l16<KIND, XAX> : (const void *)axis_fx_f64_n160_l32<KIND, XAX>;
+  }
+}
+
+template <typename T, int KIND, bool XAX>
+static int axis_fx_launch(fbse_ctx *c, const AxisArgs<T> &A, int n, int lines) {
+  // device, precision, length, lines, kind, axis -> resident work-groups
+  static int cache[64][2][2][2][5][2] = {};
+  const int d = c->device, pi = sizeof(T) == 8 ? 0 : 1, ni = n == 256 ? 1 : 0, li = lines == 32 ? 1 : 0;
+  const int nt = lines == 32 ? 512 : 256;
+  const size_t lds = (size_t)(lines * (n | 1) + n) * sizeof(cplx_t<T>) + (size_t)(n + lines) * sizeof(T);
+  const void *fn = axis_fx_fn<T, KIND, XAX>(n, lines);
+  if (d < 0 || d >= 64) return set_err(c, FBSE_ERR_UNSUPPORTED, "device %d", d);
+  int &res = cache[d][pi][ni][li][KIND][XAX ? 1 : 0];
+  if (res == 0) {
+    int per_cu = 0, ncu = 0;
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, nt, lds) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, d) != hipSuccess) {
+      (void)hipGetLastError();
+      per_cu = 0;
+    }
+    res = per_cu > 0 && ncu > 0 ? per_cu * ncu : -1;
+  }
+  if (res < 0) return set_err(c, FBSE_ERR_UNSUPPORTED, "fixed axis instance %d/%d not resident", n, lines);
+  if (A.ntiles <= 0) return FBSE_OK;
+  const dim3 g((unsigned)std::min<long>(res, A.ntiles)), b((unsigned)nt);
+  if constexpr (sizeof(T) == 4) {
+    if (n == 256 && lines == 16) hipLaunchKernelGGL((axis_fx_f32_n256_l16<KIND, XAX>), g, b, lds, c->stream, A);
+    else if (n == 256) hipLaunchKernelGGL((axis_fx_f32_n256_l32<KIND, XAX>), g, b, lds, c->stream, A);
+    else if (lines == 16) hipLaunchKernelGGL((axis_fx_f32_n160_l16<KIND, XAX>), g, b, lds, c->stream, A);
+    else hipLaunchKernelGGL((axis_fx_f32_n160_l32<KIND, XAX>), g, b, lds, c->stream, A);
+  } else {
+    if (n == 256 && lines == 16) hipLaunchKernelGGL((axis_fx_f64_n256_l16<KIND, XAX>), g, b, lds, c->stream, A);
+    else if (n == 256) hipLaunchKernelGGL((axis_fx_f64_n256_l32<KIND, XAX>), g, b, lds, c->stream, A);
+    else if (lines == 16) hipLaunchKernelGGL((axis_fx_f64_n160_l16<KIND, XAX>), g, b, lds, c->stream, A);
+    else hipLaunchKernelGGL((axis_fx_f64_n160_l32<KIND, XAX>), g, b, lds, c->stream, A);
+  }
+  FBSE_HIP(c, hipGetLastError());
+  return FBSE_OK;
+}
+
+template <typename T> int axis_fx_pass(fbse_ctx *c, int kind, const AxisArgs<T> &A, int lines) {
+  const bool kin = kind == kPassKin || kind == kPassPrec;
+  if (!axis_fx_length(A.n) || (lines != 16 && lines != 32) || A.lines != lines || (A.ksqr && kin) ||
+      (kin && A.xaxis) || (kind == kPassBwdCombine && !A.xaxis))
+    return set_err(c, FBSE_ERR_UNSUPPORTED, "no fixed axis instance for length %d, %d lines", A.n, lines);
+  // instances: forward and backward on either layout, k^2 passes on z (strided),
+  // the combine on x (rows)
+  switch (kind) {
+    case kPassFwd:
+      return A.xaxis ? axis_fx_launch<T, kPassFwd, true>(c, A, A.n, lines)
+                     : axis_fx_launch<T, kPassFwd, false>(c, A, A.n, lines);
+    case kPassBwd:
+      return A.xaxis ? axis_fx_launch<T, kPassBwd, true>(c, A, A.n, lines)
+                     : axis_fx_launch<T, kPassBwd, false>(c, A, A.n, lines);
+    case kPassKin: return axis_fx_launch<T, kPassKin, false>(c, A, A.n, lines);
+    case kPassPrec: return axis_fx_launch<T, kPassPrec, false>(c, A, A.n, lines);
+    default: return axis_fx_launch<T, kPassBwdCombine, true>(c, A, A.n, lines);
+  }
+}
+template int axis_fx_pass<double>(fbse_ctx *, int, const AxisArgs<double> &, int);
+template int axis_fx_pass<float>(fbse_ctx *, int, const AxisArgs<float> &, int);
+
 // the fixed instance for a grid (0 when there is none)
 template <typename T> static int fixed_n(const fbse_ctx *c) {
   if (c->nx != c->ny || c->nx != c->nz || !c->ksqr_sep) return 0;

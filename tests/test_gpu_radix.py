@@ -67,3 +67,29 @@ def test_fixed_instances(n):
     g.set_fft_kernel(-1, 0)
     assert rel(fout, fref) < 1e-5
     g.close()
+
+
+@pytest.mark.parametrize("grid", [(256, 160, 12), (160, 12, 256)])
+def test_fixed_axis_instances(grid):
+    """The fixed-schedule per-axis instances (axis lengths 160 and 256, 16 or
+    32 lines per tile: x rows forward and combine, y lines forward and
+    backward, z lines with k^2) against the generic per-axis kernel: H in FP64
+    to FFT rounding, the mixed filter to FP32 rounding."""
+    s = small_system(n=grid, dx=0.5, nspin=2, is_complex=1, so=1, nl=1, natoms=6, n_so_proj=1, n_nl_proj=1)
+    print("case fixed axis instances %s" % (grid,), flush=True)
+    g = gpu_context(s)
+    psi = random_states(s, 2, seed=9)
+    g.set_fft_kernel(0, 16)
+    ref = g.apply_h(psi)
+    assert rel(ref, oracle_ham(s).apply(psi)) < 1e-11
+    from test_gpu_parity import filter_inputs
+    psi_rank, stlen, an, zn, Vmin, dE_1 = filter_inputs(s, 2, 2, 12)
+    fref = g.filter(psi_rank, 2, 2, an, zn, Vmin, dE_1, precision=1)
+    for lines in (1016, 1032):
+        g.set_fft_kernel(0, lines)
+        out = g.apply_h(psi)
+        assert rel(out, ref) < 1e-12, lines
+        fout = g.filter(psi_rank, 2, 2, an, zn, Vmin, dE_1, precision=1)
+        assert rel(fout, fref) < 1e-5, lines
+    g.set_fft_kernel(-1, 0)
+    g.close()
