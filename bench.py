@@ -78,20 +78,30 @@ def one_pass(S, precision, stage_times=None, refine=False, profile=False):
     S["psi_rank"].view(S["n_starts"], S["ms"], S["stlen"])[:, 0, :].copy_(S["starts_dev"])
     torch.cuda.synchronize()
     ptr = S["psi_rank"].data_ptr()
+    free_gb = torch.cuda.mem_get_info()[0] / 1e9
+    held_gb = [round(b / 1e9, 1) for b in ctx.held_bytes()]
     t = [time.perf_counter()]
     if profile:
         ctx.profile(True)
     ctx.filter_dev(ptr, S["n_starts"], S["ms"], S["an"], S["zn"], S["Vmin"], 4.0 / S["dE"], precision)
+    # accumulation window depth the filter took, and the HBM free before it
+    S.setdefault("windows", []).append(dict(precision=int(precision), profiled=bool(profile), K=ctx.last_window(),
+                                            free_gb=round(free_gb, 1), held_gb=held_gb))
     if profile:
         S["prof"] = {k: ctx.profile_read(k) for k in range(5)}
         ctx.profile(False)
     t.append(time.perf_counter())
+    mem = lambda: round(torch.cuda.mem_get_info()[0] / 1e9, 1)  # noqa: E731
+    free_after = dict(filter=mem())
     ctx.normalize_all_dev(ptr, n)
     rank, sv = ctx.ortho_dev(ptr, n)
     t.append(time.perf_counter())
+    free_after["ortho"] = mem()
     ev = ctx.rayleigh_ritz_dev(ptr, rank)
     t.append(time.perf_counter())
+    free_after["rr"] = mem()
     sig, ev2 = ctx.sigma_dev(ptr, rank)
+    free_after["sigma"] = mem()
     if refine:
         # one FP64 residual-refinement sweep over the Ritz pairs near the
         # targets that are within two decades of the threshold
@@ -100,7 +110,10 @@ def one_pass(S, precision, stage_times=None, refine=False, profile=False):
                                      sigma=sig, sigma_max=100.0 * RESID)
         if len(ev_r):
             ev, sig, rank = ev_r, sig_r, len(ev_r)
+        free_after["refine"] = mem()
     t.append(time.perf_counter())
+    # HBM free after each stage (GB): what the next pass's filter can use
+    S["windows"][-1]["free_after_gb"] = free_after
     if stage_times is not None:
         stage_times.append(np.diff(t))
     return ev, sig, rank
@@ -572,6 +585,7 @@ def main():
         "energy_range_s": round(S["t_range"], 3),
         "roofline": roofline,
         "kernels_ms": kernels,
+        "filter_windows": S.get("windows"),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

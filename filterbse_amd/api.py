@@ -111,6 +111,8 @@ GPU_SIGNATURES = {
                                    C.c_long, C.c_long, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                    C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]),
     "fbse_graph_replays": (C.c_long, [C.c_void_p]),
+    "fbse_last_window": (C.c_int, [C.c_void_p]),
+    "fbse_held_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
     "fbse_trim": (C.c_int, [C.c_void_p]),
     "fbse_fwrite_states": (C.c_int, [C.c_void_p, C.c_void_p, LP, C.c_long, C.c_int, C.c_void_p]),
     "fbse_write_states": (C.c_int, [C.c_void_p, C.c_void_p, LP, C.c_long, C.c_int, C.c_char_p]),
@@ -469,6 +471,16 @@ class Context:
 
     def set_io_chunk(self, nbytes):
         self._check(gpu().fbse_set_io_chunk(self.ctx, nbytes))
+
+    def held_bytes(self):
+        """Bytes held between calls: (ring/work, second work buffer, partial Grams, rocBLAS workspace)."""
+        out = (C.c_size_t * 4)()
+        self._check(gpu().fbse_held_bytes(self.ctx, out))
+        return tuple(int(v) for v in out)
+
+    def last_window(self):
+        """Accumulation window depth of the last filter call."""
+        return gpu().fbse_last_window(self.ctx)
 
     def graph_replays(self):
         """hipGraph replays of the last filter call."""

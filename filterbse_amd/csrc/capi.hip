@@ -49,14 +49,37 @@ int ensure_red(fbse_ctx *c, size_t b) { return grow(c, (void **)&c->d_red, &c->r
 
 // Releases the context's work buffers (the filter's ring, FFT work space,
 // partial Grams): they are allocated again on demand.
-static void release_work(fbse_ctx *c) {
+static void release_bufs(fbse_ctx *c, int first) {
   (void)hipStreamSynchronize(c->stream);
   void **bufs[] = {&c->d_scratch, &c->d_scratch2, &c->d_gram};
   size_t *sizes[] = {&c->scratch_bytes, &c->scratch2_bytes, &c->gram_bytes};
-  for (int i = 0; i < 3; ++i) {
+  for (int i = first; i < 3; ++i) {
     if (*bufs[i]) (void)hipFree(*bufs[i]);
     *bufs[i] = nullptr;
     *sizes[i] = 0;
+  }
+}
+static void release_work(fbse_ctx *c) { release_bufs(c, 0); }
+// Before a filter sizes its ring from the free memory: the second scratch
+// buffer and the partial Grams that ortho, Rayleigh-Ritz or the refinement
+// left behind (tens of GB on a 256^3 block) are released, so the window depth
+// does not depend on what ran before (the ring's own buffer counts as free).
+static size_t blas_workspace_bytes(fbse_ctx *c) {
+  size_t sz = 0;
+  if (rocblas_get_device_memory_size(c->blas, &sz) != rocblas_status_success) return 0;
+  return sz;
+}
+static void release_aux(fbse_ctx *c) {
+  if (c->d_scratch2 || c->d_gram) release_bufs(c, 1);
+  // rocBLAS keeps the largest workspace any call asked for (the triangular
+  // solves and factorisations of ortho / Rayleigh-Ritz); above 1 GiB it is
+  // freed here and allocated again on demand
+  if (blas_workspace_bytes(c) > (size_t(1) << 30)) {
+    (void)hipStreamSynchronize(c->stream);
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wdeprecated-declarations"
+    (void)rocblas_set_device_memory_size(c->blas, 0);
+#pragma clang diagnostic pop
   }
 }
 
@@ -580,6 +603,15 @@ extern "C" int fbse_trim(fbse_ctx *c) {
   return FBSE_OK;
 }
 extern "C" long fbse_graph_replays(const fbse_ctx *c) { return c ? c->graph_replays : -1; }
+extern "C" int fbse_last_window(const fbse_ctx *c) { return c ? c->last_window : -1; }
+extern "C" int fbse_held_bytes(fbse_ctx *c, size_t *out) {
+  if (!c || !out) return FBSE_ERR_ARG;
+  out[0] = c->scratch_bytes;
+  out[1] = c->scratch2_bytes;
+  out[2] = c->gram_bytes;
+  out[3] = blas_workspace_bytes(c);
+  return FBSE_OK;
+}
 extern "C" int fbse_fft_choice(fbse_ctx *c, int precision, int *path, int *instance) {
   if (!c || !path || (precision != FBSE_PREC_FP64 && precision != FBSE_PREC_MIXED)) return FBSE_ERR_ARG;
   *path = fft_path(c, precision == FBSE_PREC_MIXED ? 1 : 0, instance);
@@ -881,6 +913,7 @@ template <typename T> struct FilterRun {
   int prepare(int K) {
     int rc;
     R = K + 1;
+    c->last_window = K;
     if ((rc = ensure_scratch(c, slot_bytes() * (size_t)(R + 1)))) return rc;
     ring = (C *)c->d_scratch;
     work = ring + (size_t)R * slot_elems;
@@ -1012,6 +1045,7 @@ template <typename T>
 static int filter_impl(fbse_ctx *c, double *d_psi_rank, long n_starts, long ms, long M, const double *an,
                        const double *zn, double Vmin, double dE_1) {
   FilterRun<T> r(c, d_psi_rank, n_starts, ms, M, an, zn, Vmin, dE_1);
+  release_aux(c);
   int rc = r.prepare(window_depth(c->window, r.slot_bytes(), c->scratch_bytes));
   while (!rc && r.in_window0()) rc = r.step();  // eager (runs the autotuners)
   if (!rc) rc = r.capture();
@@ -1034,6 +1068,7 @@ static int filter_many_impl(int n, fbse_ctx *const *ctx, double *const *psi, con
   for (int d = 0; d < n; ++d) {
     runs.emplace_back(new FilterRun<T>(ctx[d], psi[d], n_starts[d], ms, M, an[d], zn[d], Vmin[d], dE_1[d]));
     ring_slot += runs.back()->slot_bytes();
+    release_aux(ctx[d]);
     held += ctx[d]->scratch_bytes;
     if (ctx[d]->window > 0) forced = forced ? std::min(forced, ctx[d]->window) : ctx[d]->window;
   }
@@ -1480,15 +1515,22 @@ static int ortho_dev_impl(fbse_ctx *c, double *A, long n, double svd_eps, long *
       cleanup();
       return set_err(c, FBSE_ERR_SOLVER, "ortho: Cholesky re-orthogonalisation failed (info %d)", info);
     }
-    if (la.cplx) {
-      const rocblas_double_complex one = {1.0, 0.0};
-      st = rocblas_ztrsm(c->blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
-                         rocblas_diagonal_non_unit, (int)la.m, (int)k, &one, (const rocblas_double_complex *)G, (int)k,
-                         (rocblas_double_complex *)A, (int)la.m);
-    } else {
-      const double one = 1.0;
-      st = rocblas_dtrsm(c->blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
-                         rocblas_diagonal_non_unit, (int)la.m, (int)k, &one, G, (int)k, A, (int)la.m);
+    // right-side solves act on rows independently: blocks of rows keep the
+    // workspace rocBLAS allocates (and its handle then keeps) at rows x k
+    // elements instead of the whole block (137 GB for 256 states of 256^3)
+    constexpr long kTrsmRows = 262144;
+    for (long r0 = 0; r0 < la.m && st == rocblas_status_success; r0 += kTrsmRows) {
+      const int nr = (int)std::min(kTrsmRows, (long)la.m - r0);
+      if (la.cplx) {
+        const rocblas_double_complex one = {1.0, 0.0};
+        st = rocblas_ztrsm(c->blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                           rocblas_diagonal_non_unit, nr, (int)k, &one, (const rocblas_double_complex *)G, (int)k,
+                           (rocblas_double_complex *)A + r0, (int)la.m);
+      } else {
+        const double one = 1.0;
+        st = rocblas_dtrsm(c->blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                           rocblas_diagonal_non_unit, nr, (int)k, &one, G, (int)k, A + r0, (int)la.m);
+      }
     }
     if (st != rocblas_status_success) {
       cleanup();

@@ -200,6 +200,7 @@ struct fbse_ctx {
   int dyn_step_off = 0;
   long *d_step = nullptr;       // device step counter of the replayed window
   long graph_replays = 0;       // replays in the last filter
+  int last_window = 0;          // accumulation window depth K of the last filter
   double tune_ms[2][2][3][3] = {};
   int autotune = 1;
   int force_variant = -1;       // fbse_set_fft_kernel: -1 auto, 0 per-axis passes, 1 plane/slab passes

@@ -223,6 +223,13 @@ int fbse_trim(fbse_ctx *ctx);
 
 /* Graph replays of the last filter call (0 when every window ran eagerly). */
 long fbse_graph_replays(const fbse_ctx *ctx);
+/* Accumulation window depth K of the last filter call (fbse_set_window, or
+ * the depth the free memory allowed); for reporting. */
+int fbse_last_window(const fbse_ctx *ctx);
+/* Device memory the context holds between calls: out[0..3] = the filter ring /
+ * work buffer, the second work buffer, the partial Grams, the rocBLAS
+ * workspace (bytes).  For reporting; fbse_trim releases the first three. */
+int fbse_held_bytes(fbse_ctx *ctx, size_t *out);
 
 /* The FFT path the context uses for a precision: *path = 1 plane/slab
  * passes, 0 per-axis passes; *instance (may be NULL) = the plane/slab kernel

@@ -48,3 +48,30 @@ def test_wide_ortho_and_rr(wide):
     HV = O.as_matrix(h.apply(rot), m, stlen, 1)
     Hs = s["dv"] * (V.conj().T @ HV)
     assert np.abs(Hs - np.diag(w)).max() < 1e-9 * np.abs(w).max()
+
+
+def test_tall_ortho_blocked_solve():
+    """A block taller than one row block of the Cholesky re-orthogonalisation's
+    triangular solve (262,144 rows per solve): 64 x 64 x 48 spinor, 393,216
+    complex rows, two blocks.  Rank and singular values against LAPACK, and an
+    orthonormal basis of the same span."""
+    s = small_system(n=(64, 64, 48), nspin=2, is_complex=1)
+    print("case grid 64x64x48 spinor, 24 columns", flush=True)
+    g = gpu_context(s)
+    stlen = 2 * s["nspin"] * s["nx"] * s["ny"] * s["nz"]
+    n = 24
+    rng = np.random.default_rng(8)
+    A = rng.standard_normal((n, stlen))
+    A[n - 1] = A[2] + 1e-9 * A[n - 1]
+    psi = A.ravel()
+    out, k, sv = g.ortho(psi, n)
+    _, sv_ref, k_ref = O.ortho(psi, n, stlen, 1, s["dv"])
+    assert k == k_ref == n
+    assert np.allclose(sv[: n - 1], sv_ref[: n - 1], rtol=1e-9, atol=0)
+    U = O.as_matrix(out, k, stlen, 1)
+    assert np.abs(s["dv"] * (U.conj().T @ U) - np.eye(k)).max() < 1e-10
+    # same span: the projection of the inputs onto U reproduces them
+    Am = O.as_matrix(psi, n, stlen, 1)
+    resid = Am - U @ (s["dv"] * (U.conj().T @ Am))
+    assert np.abs(resid).max() < 1e-8 * np.abs(Am).max()
+    g.close()
