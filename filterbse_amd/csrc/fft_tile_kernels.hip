@@ -628,19 +628,40 @@ __device__ __forceinline__ void fx_body(const TileArgs<T> &A) {
       }
       if (more) issue(t + G, I0{}, IPF{});
       if (expand) {
-        // projector expansion in rounds per rank, as in tile_body
+        // projector expansion in rounds per rank, as in tile_body; the
+        // positions and atoms of a thread's next two records are loaded one
+        // step ahead (across the round barrier too), so each step waits once
+        // (records and coefficients) instead of twice (positions, then the
+        // coefficients of their atoms)
         __syncthreads();
         const C *dcol = A.pjd + (long)item * A.pj_natoms * 6;
         const int2 *pa = reinterpret_cast<const int2 *>(A.rec_pa);
-        for (int k = 0; k < A.pt_ranks; ++k) {
-          const int e0 = A.rec_off[w * A.pt_ranks + k], e1 = A.rec_off[w * A.pt_ranks + k + 1];
-          for (int e = e0 + threadIdx.x; e < e1; e += 2 * NT) {
+        const int *ro = A.rec_off + w * A.pt_ranks;
+        const int nr = A.pt_ranks;
+        int2 qa = int2{0, 0}, qb = int2{0, 0};
+        int pf = -1;  // qa, qb hold the first step of round pf
+        auto first_of = [&](int k, int2 &a, int2 &b) {
+          const int s0 = ro[k] + (int)threadIdx.x, s1 = ro[k + 1];
+          if (s0 < s1) a = pa[s0];
+          if (s0 + NT < s1) b = pa[s0 + NT];
+        };
+        for (int k = 0; k < nr; ++k) {
+          const int e1 = ro[k + 1];
+          if (pf != k) first_of(k, qa, qb);
+          for (int e = ro[k] + threadIdx.x; e < e1; e += 2 * NT) {
             const int eb = e + NT;
             const bool two = eb < e1;
-            const int2 qa = pa[e];
-            const int2 qb = two ? pa[eb] : qa;
+            int2 na = qa, nb = qb;
+            const int en = e + 2 * NT;
+            if (en < e1) {
+              na = pa[en];
+              if (en + NT < e1) nb = pa[en + NT];
+            } else if (k + 1 < nr) {
+              first_of(k + 1, na, nb);
+              pf = k + 1;
+            }
             const C ta = record_term<T>(A.rec_val + (long)e * 8, dcol + (long)qa.y * 6);
-            const C tb2 = record_term<T>(A.rec_val + (long)(two ? eb : e) * 8, dcol + (long)qb.y * 6);
+            const C tb = record_term<T>(A.rec_val + (long)(two ? eb : e) * 8, dcol + (long)(two ? qb.y : qa.y) * 6);
             const int pos_a = (qa.x >> 16) * LS + (qa.x & 0xffff);
             C o = buf[pos_a];
             o.x += ta.x;
@@ -649,10 +670,12 @@ __device__ __forceinline__ void fx_body(const TileArgs<T> &A) {
             if (two) {
               const int pos_b = (qb.x >> 16) * LS + (qb.x & 0xffff);
               C q = buf[pos_b];
-              q.x += tb2.x;
-              q.y += tb2.y;
+              q.x += tb.x;
+              q.y += tb.y;
               buf[pos_b] = q;
             }
+            qa = na;
+            qb = nb;
           }
           __syncthreads();
         }
