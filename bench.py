@@ -431,6 +431,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--window", type=int, default=0,
+                    help="accumulation window depth K (0 = as deep as 40%% of the free HBM allows)")
     ap.add_argument("--config", default="c1_so_96")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -466,6 +468,8 @@ def main():
         return
     S = setup(args.config, dev, args.ncheby)
     S["cfg_name"] = args.config
+    if args.window:
+        S["ctx"].set_window(args.window)
     mixed, fp64 = api.FBSE_PREC_MIXED, api.FBSE_PREC_FP64
 
     for _ in range(args.warmup):

@@ -882,6 +882,7 @@ template <typename T> struct FilterRun {
   long n_items = 0, nsg = 0, sl = 0, slot_elems = 0;
   C *ring = nullptr, *work = nullptr;
   double *d_an = nullptr, *d_a0 = nullptr;
+  std::vector<double> an_t;  // host copy of the step-major coefficients (outlives its upload)
   long acc_lo = 0, j = 1, n_full = 0, w_next = 1;
   bool graph = false;
   hipGraph_t g = nullptr;
@@ -919,8 +920,23 @@ template <typename T> struct FilterRun {
     work = ring + (size_t)R * slot_elems;
     real_an = 1;  // all an.im == 0: the accumulation needs only the real parts
     for (long k = 0; k < ms * M && real_an; ++k) real_an = an[2 * k + 1] == 0.0;
-    FBSE_HIP(c, hipMalloc(&d_an, (size_t)ms * M * 2 * sizeof(double)));
-    FBSE_HIP(c, hipMemcpyAsync(d_an, an, (size_t)ms * M * 2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    // the accumulation reads the coefficients step-major, anT[j*ms + t] (real
+    // parts only when all an.im are 0 or the states are packed real pairs,
+    // which accumulate .re alone, filter.c:353-357), else complex pairs
+    const bool real_layout = real_an || packed;
+    an_t.assign((size_t)M * ms * (real_layout ? 1 : 2), 0.0);
+    for (long jj = 0; jj < M; ++jj)
+      for (long t = 0; t < ms; ++t) {
+        const double *a = an + 2 * (t * M + jj);
+        if (real_layout) {
+          an_t[jj * ms + t] = a[0];
+        } else {
+          an_t[2 * (jj * ms + t)] = a[0];
+          an_t[2 * (jj * ms + t) + 1] = a[1];
+        }
+      }
+    FBSE_HIP(c, hipMalloc(&d_an, an_t.size() * sizeof(double)));
+    FBSE_HIP(c, hipMemcpyAsync(d_an, an_t.data(), an_t.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     {
       double2 *stage;
       if constexpr (sizeof(T) == 8) stage = (double2 *)ring;

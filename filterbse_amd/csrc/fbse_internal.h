@@ -15,7 +15,7 @@
 namespace fbse {
 
 // deepest Newton accumulation window K (ring of K+1 slots)
-constexpr int kMaxWindow = 63;
+constexpr int kMaxWindow = 127;
 
 template <typename T> struct cplx_of;
 template <> struct cplx_of<double> { using type = double2; };

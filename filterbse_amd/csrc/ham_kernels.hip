@@ -559,19 +559,24 @@ constexpr int kAccGroup = 4;  // ring slots loaded per group in accumulate_kerne
 // The coefficients are wave-uniform (same step j and target t in every lane):
 // read through a restrict-qualified pointer they become scalar loads and
 // enter the FMAs as SGPR operands, instead of one LDS broadcast per (t, j)
-// whose 16-byte-per-lane return would cost more than the FMAs it feeds.
+// whose 16-byte-per-lane return would cost more than the FMAs it feeds.  They
+// are stored step-major (anT[j*ms + t], the transpose of the reference's
+// an[t*ncheby + j]): one step's targets are contiguous (128 bytes for 16 real
+// coefficients, fetched by wide scalar loads), so a window of K steps keeps
+// K*ms*8 bytes in the scalar cache instead of one cache line per (t, j).
 // REAL_A: every an.im is 0 (the Gaussian filter, coeff.c:89-105; always for
-// packed real states): 2 FMAs per (t, j) instead of 4.
+// packed real states): real parts only, 2 FMAs per (t, j) instead of 4.
 template <typename T, bool REAL_A>
 __global__ __launch_bounds__(256) void accumulate_kernel(const cplx_t<T> *__restrict__ ring, int ring_slots,
                                                          long slot_stride, long j_lo, int nj,
-                                                         const double *__restrict__ an, int ms, long ncheby,
+                                                         const double *__restrict__ an, int ms,
                                                          double *__restrict__ psi_rank, long n_starts, long n_items,
                                                          int packed, long nsg, const long *__restrict__ jlo_dev) {
   const long item = blockIdx.y;  // complex: start state; packed: start pair
   if (jlo_dev) j_lo = *jlo_dev;
   const long slot0 = j_lo % ring_slots;
-  const double *a_win = an + 2 * j_lo;  // an[(t*ncheby + j_lo + j)*2 + {0,1}]
+  // REAL_A: anT[(j_lo + j)*ms + t]; else anT[((j_lo + j)*ms + t)*2 + {0,1}]
+  const double *a_win = an + (REAL_A ? 1 : 2) * j_lo * ms;
   for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < nsg; g += (long)gridDim.x * blockDim.x) {
     const cplx_t<T> *src = ring + item * nsg + g;
     double2 acc[kMaxMs];
@@ -598,16 +603,16 @@ __global__ __launch_bounds__(256) void accumulate_kernel(const cplx_t<T> *__rest
       for (int u = 0; u < kAccGroup; ++u) {
         if (j0 + u < nj) {
           const double vx = (double)cur[u].x, vy = (double)cur[u].y;
-          const double *aj = a_win + 2 * (j0 + u);
+          const double *aj = a_win + (REAL_A ? 1 : 2) * (long)(j0 + u) * ms;
 #pragma unroll
           for (int t = 0; t < kMaxMs; ++t) {
             if (t < ms) {
-              const double ar = aj[2 * t * ncheby];
+              const double ar = REAL_A ? aj[t] : aj[2 * t];
               if (REAL_A) {
                 acc[t].x += ar * vx;
                 acc[t].y += ar * vy;
               } else {
-                const double ai = aj[2 * t * ncheby + 1];
+                const double ai = aj[2 * t + 1];
                 acc[t].x += ar * vx - ai * vy;
                 acc[t].y += ar * vy + ai * vx;
               }
@@ -658,10 +663,10 @@ int accumulate_window(fbse_ctx *c, const cplx_t<T> *ring, int ring_slots, long s
   const dim3 grid((unsigned)blocks, (unsigned)n_items);
   if (real_coef || packed)
     hipLaunchKernelGGL((accumulate_kernel<T, true>), grid, dim3(256), 0, c->stream, ring, ring_slots, slot_stride,
-                       j_lo, nj, d_an, (int)ms, ncheby, psi_rank, n_starts, n_items, packed, c->nsg, jlo_dev);
+                       j_lo, nj, d_an, (int)ms, psi_rank, n_starts, n_items, packed, c->nsg, jlo_dev);
   else
     hipLaunchKernelGGL((accumulate_kernel<T, false>), grid, dim3(256), 0, c->stream, ring, ring_slots, slot_stride,
-                       j_lo, nj, d_an, (int)ms, ncheby, psi_rank, n_starts, n_items, packed, c->nsg, jlo_dev);
+                       j_lo, nj, d_an, (int)ms, psi_rank, n_starts, n_items, packed, c->nsg, jlo_dev);
   FBSE_HIP(c, hipGetLastError());
   prof_end(c, 3, ev);
   return FBSE_OK;

@@ -66,6 +66,25 @@ def test_filter_fp64(system, window):
     assert rel(out, ref) < 1e-10, name
 
 
+@pytest.mark.parametrize("precision,tol", [(0, 1e-10), (1, 3e-5)])
+def test_filter_complex_coefficients(system, precision, tol):
+    """Newton coefficients with nonzero imaginary parts (the general path of
+    filter.c:331-357: complex states accumulate a*psi, real states only
+    Re(a)*psi): the accumulation's complex-coefficient kernel against the
+    oracle, FP64 to rounding and mixed to FP32 rounding; window 3 so that
+    several accumulations run."""
+    name, s, h, g = system
+    n_starts, ms, M = 2, 3, 30
+    psi_rank, stlen, an, zn, Vmin, dE_1 = filter_inputs(s, n_starts, ms, M)
+    an = an.copy()
+    an[1::2] = 0.4 * an[0::2] * np.cos(np.arange(ms * M))
+    ref = h.filter_cycles(psi_rank, n_starts, ms, an, zn, Vmin, dE_1)
+    g.set_window(3)
+    out = g.filter(psi_rank, n_starts, ms, an, zn, Vmin, dE_1, precision=precision)
+    g.set_window(0)
+    assert rel(out, ref) < tol, name
+
+
 @pytest.mark.parametrize("variant,lines", [(-1, 0), (0, 0), (1, 1), (1, 2), (1, 3)])
 def test_filter_mixed(system, variant, lines):
     """The FP32 recurrence with FP64 accumulation stays within 3e-5 of the FP64
