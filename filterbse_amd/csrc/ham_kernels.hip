@@ -64,8 +64,9 @@ __global__ __launch_bounds__(256) void proj_color_kernel(ProjArgs<T> P) {
 #pragma unroll
   for (int r = 0; r < NR; ++r) acc[r].x = acc[r].y = 0;
   if (live) {
-    // four points per lane in flight: the index loads, then the gathers
-    constexpr int U = 4;
+    // points per lane in flight: the index loads, then the gathers (eight
+    // FP32 gathers per lane fit the registers that the occupancy leaves)
+    constexpr int U = sizeof(T) == 4 ? 8 : 4;
     for (int g0 = lane; g0 < cnt; g0 += U * kPjLanes) {
       int gi[U];
       C vv[U];
