@@ -706,10 +706,31 @@ __device__ __forceinline__ void fx_body(const TileArgs<T> &A) {
 // per CU allow.  Without it the fixed instances take the forward xy pass from
 // 569 to 396 us, the z-slab pass from 615 to 423 us and the combine pass
 // (with the next forward transform) from 978 to 770 us.
-FBSE_FX_KERNEL(fft_fx_f32_n96, float, 96, 12, 8, 768, 6, 0, 0)
-FBSE_FX_KERNEL(fft_fx_f32_n64, float, 64, 8, 8, 512, 4, 0, 0)
-FBSE_FX_KERNEL(fft_fx_f64_n96, double, 96, 12, 8, 768, 3, 0, 0)
-FBSE_FX_KERNEL(fft_fx_f64_n64, double, 64, 8, 8, 512, 4, 0, 0)
+//
+// Instances (N, radices, threads, waves per SIMD): threads = N * rows-step,
+// chosen so that every thread stages the same number of rows and the radix
+// passes keep at most two butterflies per thread; the waves hint lets two
+// work-groups share a CU where the tile's LDS allows it (N <= 90 in FP32),
+// one otherwise.  FP64 tiles above 96 do not fit the LDS.
+#define FBSE_FX_F32(X)            \
+  X(60, 10, 6, 600, 5)            \
+  X(64, 8, 8, 512, 4)             \
+  X(72, 9, 8, 648, 6)             \
+  X(80, 10, 8, 640, 5)            \
+  X(90, 10, 9, 540, 5)            \
+  X(96, 12, 8, 768, 6)            \
+  X(100, 10, 10, 1000, 4)         \
+  X(108, 12, 9, 972, 4)           \
+  X(120, 15, 8, 960, 4)
+#define FBSE_FX_F64(X)            \
+  X(64, 8, 8, 512, 4)             \
+  X(96, 12, 8, 768, 3)
+#define FBSE_FX_K32(N, R1, R2, NT, W) FBSE_FX_KERNEL(fft_fx_f32_n##N, float, N, R1, R2, NT, W, 0, 0)
+#define FBSE_FX_K64(N, R1, R2, NT, W) FBSE_FX_KERNEL(fft_fx_f64_n##N, double, N, R1, R2, NT, W, 0, 0)
+FBSE_FX_F32(FBSE_FX_K32)
+FBSE_FX_F64(FBSE_FX_K64)
+#undef FBSE_FX_K32
+#undef FBSE_FX_K64
 #undef FBSE_FX_KERNEL
 
 // ---------------------------------------------------------------------------
@@ -927,19 +948,49 @@ template <typename T> int axis_fx_pass(fbse_ctx *c, int kind, const AxisArgs<T> 
 template int axis_fx_pass<double>(fbse_ctx *, int, const AxisArgs<double> &, int);
 template int axis_fx_pass<float>(fbse_ctx *, int, const AxisArgs<float> &, int);
 
+constexpr int kFxMaxN = 120;  // largest fixed-instance size (sizes are even)
+
 // the fixed instance for a grid (0 when there is none)
+template <typename T> static bool fx_has(int n) {
+#define FBSE_FX_HAS(N, R1, R2, NT, W) \
+  if (n == N) return true;
+  if constexpr (sizeof(T) == 4) {
+    FBSE_FX_F32(FBSE_FX_HAS)
+  } else {
+    FBSE_FX_F64(FBSE_FX_HAS)
+  }
+#undef FBSE_FX_HAS
+  return false;
+}
 template <typename T> static int fixed_n(const fbse_ctx *c) {
   if (c->nx != c->ny || c->nx != c->nz || !c->ksqr_sep) return 0;
-  return (c->nx == 96 || c->nx == 64) ? (int)c->nx : 0;
+  return fx_has<T>((int)c->nx) ? (int)c->nx : 0;
 }
 template <typename T, int KIND> static const void *fx_fn(int n) {
+#define FBSE_FX_FN32(N, R1, R2, NT, W) \
+  if (n == N) return (const void *)fft_fx_f32_n##N<KIND>;
+#define FBSE_FX_FN64(N, R1, R2, NT, W) \
+  if (n == N) return (const void *)fft_fx_f64_n##N<KIND>;
   if constexpr (sizeof(T) == 4) {
-    return n == 96 ? (const void *)fft_fx_f32_n96<KIND> : (const void *)fft_fx_f32_n64<KIND>;
+    FBSE_FX_F32(FBSE_FX_FN32)
   } else {
-    return n == 96 ? (const void *)fft_fx_f64_n96<KIND> : (const void *)fft_fx_f64_n64<KIND>;
+    FBSE_FX_F64(FBSE_FX_FN64)
   }
+#undef FBSE_FX_FN32
+#undef FBSE_FX_FN64
+  return nullptr;
 }
-static int fx_threads(int n) { return n == 96 ? 768 : 512; }
+template <typename T> static int fx_threads(int n) {
+#define FBSE_FX_NT(N, R1, R2, NT, W) \
+  if (n == N) return NT;
+  if constexpr (sizeof(T) == 4) {
+    FBSE_FX_F32(FBSE_FX_NT)
+  } else {
+    FBSE_FX_F64(FBSE_FX_NT)
+  }
+#undef FBSE_FX_NT
+  return 0;
+}
 
 // ---------------------------------------------------------------------------
 // host side
@@ -1051,36 +1102,37 @@ template <typename T, int KIND> static void set_lds_limit(int device, int v) {
 
 template <typename T, int KIND>
 static void fx_launch(int n, dim3 g, dim3 b, size_t lds, hipStream_t st, const TileArgs<T> &A) {
+#define FBSE_FX_L32(N, R1, R2, NT, W) \
+  if (n == N) hipLaunchKernelGGL((fft_fx_f32_n##N<KIND>), g, b, lds, st, A);
+#define FBSE_FX_L64(N, R1, R2, NT, W) \
+  if (n == N) hipLaunchKernelGGL((fft_fx_f64_n##N<KIND>), g, b, lds, st, A);
   if constexpr (sizeof(T) == 4) {
-    if (n == 96)
-      hipLaunchKernelGGL((fft_fx_f32_n96<KIND>), g, b, lds, st, A);
-    else
-      hipLaunchKernelGGL((fft_fx_f32_n64<KIND>), g, b, lds, st, A);
+    FBSE_FX_F32(FBSE_FX_L32)
   } else {
-    if (n == 96)
-      hipLaunchKernelGGL((fft_fx_f64_n96<KIND>), g, b, lds, st, A);
-    else
-      hipLaunchKernelGGL((fft_fx_f64_n64<KIND>), g, b, lds, st, A);
+    FBSE_FX_F64(FBSE_FX_L64)
   }
+#undef FBSE_FX_L32
+#undef FBSE_FX_L64
 }
 
 // Work-groups of a persistent fixed-instance launch: the number resident at
 // once (occupancy x CUs, cached per device, precision, size and pass), at
 // most one per tile; 0 when the instance cannot run.
 template <typename T, int KIND> static int fx_grid(fbse_ctx *c, int n, long ntiles) {
-  static int cache[64][2][2][5] = {};
-  const int d = c->device, pi = sizeof(T) == 8 ? 0 : 1, ni = n == 96 ? 1 : 0;
-  if (d < 0 || d >= 64) return 0;
+  static int cache[64][2][kFxMaxN / 2 + 1][5] = {};  // sizes are even
+  const int d = c->device, pi = sizeof(T) == 8 ? 0 : 1, ni = n / 2;
+  if (d < 0 || d >= 64 || n % 2 || n > kFxMaxN) return 0;
   int &res = cache[d][pi][ni][KIND];
   if (res == 0) {
     const void *fn = fx_fn<T, KIND>(n);
+    if (!fn) return 0;
     const size_t lds = tile_lds_bytes<T>(n, n);
     if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess) {
       (void)hipGetLastError();
       return 0;
     }
     int per_cu = 0, ncu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, fx_threads(n), lds) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, fx_threads<T>(n), lds) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, d) != hipSuccess) {
       (void)hipGetLastError();
       return 0;
@@ -1172,7 +1224,7 @@ static int tile_pass_v(fbse_ctx *c, int v, int kind, const cplx_t<T> *src, cplx_
     hipEvent_t ev = nullptr;
     const int pk = kind == kTileBwdXYCombine ? 1 : 0;
     prof_begin(c, pk, &ev);
-    const dim3 gd((unsigned)g), bd((unsigned)fx_threads(n));
+    const dim3 gd((unsigned)g), bd((unsigned)fx_threads<T>(n));
     switch (kind) {
       case kTileFwdXY: fx_launch<T, kTileFwdXY>(n, gd, bd, lds, c->stream, A); break;
       case kTileKinZ: fx_launch<T, kTileKinZ>(n, gd, bd, lds, c->stream, A); break;

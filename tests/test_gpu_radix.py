@@ -99,3 +99,24 @@ def test_fixed_axis_instances(grid):
         assert np.abs(r - rref).max() < 1e-10 * np.abs(rref).max(), lines
     g.set_fft_kernel(-1, 0)
     g.close()
+
+
+@pytest.mark.parametrize("n", [60, 72, 80, 90, 100, 108, 120])
+def test_fixed_instances_fp32_sizes(n):
+    """The FP32 fixed-schedule instances of the other cubic sizes (the uneven
+    dots' grids: radices 10*6, 9*8, 10*8, 10*9, 10*10, 12*9, 15*8): the mixed
+    filter through the fixed forward, z-slab and combine passes (projector
+    expansion included) against the generic per-axis passes, to FP32 rounding."""
+    s = small_system(n=(n, n, n), dx=0.5, nspin=2, is_complex=1, so=1, nl=1, natoms=6,
+                     n_so_proj=1, n_nl_proj=1)
+    print("case fixed FP32 instance %d^3" % n, flush=True)
+    g = gpu_context(s)
+    from test_gpu_parity import filter_inputs
+    psi_rank, stlen, an, zn, Vmin, dE_1 = filter_inputs(s, 2, 2, 12)
+    g.set_fft_kernel(0, 16)
+    fref = g.filter(psi_rank, 2, 2, an, zn, Vmin, dE_1, precision=1)
+    g.set_fft_kernel(1, 4)
+    fout = g.filter(psi_rank, 2, 2, an, zn, Vmin, dE_1, precision=1)
+    g.set_fft_kernel(-1, 0)
+    assert rel(fout, fref) < 1e-5
+    g.close()
