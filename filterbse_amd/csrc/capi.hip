@@ -573,7 +573,7 @@ extern "C" int fbse_set_window(fbse_ctx *c, int w) {
   return FBSE_OK;
 }
 extern "C" int fbse_set_fft_kernel(fbse_ctx *c, int variant, int lines) {
-  if (!c || variant < -1 || variant > 1 || lines < 0 || (lines > 256 && !(variant == 0 && (lines == 1016 || lines == 1032))))
+  if (!c || variant < -1 || variant > 1 || lines < 0 || (lines > 256 && !(variant == 0 && (lines == 1016 || lines == 1032 || lines == 1064))))
     return FBSE_ERR_ARG;
   c->force_variant = variant;
   c->force_lines = lines;

@@ -348,7 +348,8 @@ static int fft_pass_lines(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src,
   // data; 1000 + L asks for the fixed-schedule instance with L lines, which
   // keeps L (a last partial tile is masked) where the axis length has one
   const bool kin = kind == kPassKin || kind == kPassPrec;
-  const bool fx = lines_req >= 1000 && axis_fx_length(pl.n) && !(kin && !c->ksqr_sep);
+  const bool fx = lines_req >= 1000 && axis_fx_length(pl.n) && !(kin && !c->ksqr_sep) &&
+                  !(sizeof(T) == 8 && lines_req == 1064);  // 64-line instances are FP32 only
   int lines = lines_req >= 1000 ? lines_req - 1000 : (lines_req > 0 ? lines_req : pl.lines);
   A.psi_in = psi_in;
   A.a1 = a1;
@@ -447,7 +448,7 @@ static int tune_pass(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src, long
   const AxisPlan &pl = c->plan[axis];
   // tile heights of the generic kernel, then the fixed-schedule instances
   // (1000 + lines) where the axis length has one
-  const int cand[] = {4, 8, 16, 32, 64, 1016, 1032};
+  const int cand[] = {4, 8, 16, 32, 64, 1016, 1032, 1064};
   const size_t bytes = (size_t)nbatch * c->ngrid * sizeof(cplx_t<T>);
   cplx_t<T> *scratch = nullptr;
   if (hipMalloc(&scratch, bytes) != hipSuccess) {

@@ -815,7 +815,9 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
     // lines l >= nl of a last partial tile hold stale values: transformed, never stored
     fx_fft<T, N, R1, R2, L, 1, LS, NT, INV>(tw_off);
     if (KIN) {
-#pragma unroll
+      // partly unrolled: the scaling step's loads stay within the registers
+      // the transforms leave free
+#pragma unroll 4
       for (int u = 0; u < E; ++u) {
         const int e = tid + u * NT;
         const int l = e / N, p = e % N;
@@ -868,6 +870,10 @@ FBSE_AXIS_FX(axis_fx_f32_n256_l16, float, 256, 16, 16, 16, 256, 4)
 FBSE_AXIS_FX(axis_fx_f32_n256_l32, float, 256, 16, 16, 32, 512, 4)
 FBSE_AXIS_FX(axis_fx_f32_n160_l16, float, 160, 16, 10, 16, 256, 4)
 FBSE_AXIS_FX(axis_fx_f32_n160_l32, float, 160, 16, 10, 32, 512, 4)
+// 64 lines: the y and z tiles read 512-byte segments per axis point (FP32
+// only; one work-group per CU)
+FBSE_AXIS_FX(axis_fx_f32_n256_l64, float, 256, 16, 16, 64, 1024, 4)
+FBSE_AXIS_FX(axis_fx_f32_n160_l64, float, 160, 16, 10, 64, 1024, 4)
 FBSE_AXIS_FX(axis_fx_f64_n256_l16, double, 256, 16, 16, 16, 256, 2)
 FBSE_AXIS_FX(axis_fx_f64_n256_l32, double, 256, 16, 16, 32, 512, 2)
 FBSE_AXIS_FX(axis_fx_f64_n160_l16, double, 160, 16, 10, 16, 256, 2)
@@ -878,6 +884,8 @@ bool axis_fx_length(int n) { return n == 160 || n == 256; }
 
 template <typename T, int KIND, bool XAX> static const void *axis_fx_fn(int n, int lines) {
   if constexpr (sizeof(T) == 4) {
+    if (lines == 64)
+      return n == 256 ? (const void *)axis_fx_f32_n256_l64<KIND, XAX> : (const void *)axis_fx_f32_n160_l64<KIND, XAX>;
     if (n == 256)
       return lines == 16 ? (const void *)axis_fx_f32_n256_l16<KIND, XAX> : (const void *)axis_fx_f32_n256_l32<KIND, XAX>;
     return lines == 16 ? (const void *)axis_fx_f32_n160_l16<KIND, XAX> : (const void *)axis_fx_f32_n160_l32<KIND, XAX>;
@@ -891,9 +899,10 @@ template <typename T, int KIND, bool XAX> static const void *axis_fx_fn(int n, i
 template <typename T, int KIND, bool XAX>
 static int axis_fx_launch(fbse_ctx *c, const AxisArgs<T> &A, int n, int lines) {
   // device, precision, length, lines, kind, axis -> resident work-groups
-  static int cache[64][2][2][2][5][2] = {};
-  const int d = c->device, pi = sizeof(T) == 8 ? 0 : 1, ni = n == 256 ? 1 : 0, li = lines == 32 ? 1 : 0;
-  const int nt = lines == 32 ? 512 : 256;
+  static int cache[64][2][2][3][5][2] = {};
+  const int d = c->device, pi = sizeof(T) == 8 ? 0 : 1, ni = n == 256 ? 1 : 0;
+  const int li = lines == 64 ? 2 : (lines == 32 ? 1 : 0);
+  const int nt = lines * 16;  // 256, 512 or 1024 threads
   const size_t lds = (size_t)(lines * (n | 1) + n) * sizeof(cplx_t<T>) + (size_t)(n + lines) * sizeof(T);
   const void *fn = axis_fx_fn<T, KIND, XAX>(n, lines);
   if (d < 0 || d >= 64) return set_err(c, FBSE_ERR_UNSUPPORTED, "device %d", d);
@@ -913,9 +922,11 @@ static int axis_fx_launch(fbse_ctx *c, const AxisArgs<T> &A, int n, int lines) {
   const dim3 g((unsigned)std::min<long>(res, A.ntiles)), b((unsigned)nt);
   if constexpr (sizeof(T) == 4) {
     if (n == 256 && lines == 16) hipLaunchKernelGGL((axis_fx_f32_n256_l16<KIND, XAX>), g, b, lds, c->stream, A);
-    else if (n == 256) hipLaunchKernelGGL((axis_fx_f32_n256_l32<KIND, XAX>), g, b, lds, c->stream, A);
+    else if (n == 256 && lines == 32) hipLaunchKernelGGL((axis_fx_f32_n256_l32<KIND, XAX>), g, b, lds, c->stream, A);
+    else if (n == 256) hipLaunchKernelGGL((axis_fx_f32_n256_l64<KIND, XAX>), g, b, lds, c->stream, A);
     else if (lines == 16) hipLaunchKernelGGL((axis_fx_f32_n160_l16<KIND, XAX>), g, b, lds, c->stream, A);
-    else hipLaunchKernelGGL((axis_fx_f32_n160_l32<KIND, XAX>), g, b, lds, c->stream, A);
+    else if (lines == 32) hipLaunchKernelGGL((axis_fx_f32_n160_l32<KIND, XAX>), g, b, lds, c->stream, A);
+    else hipLaunchKernelGGL((axis_fx_f32_n160_l64<KIND, XAX>), g, b, lds, c->stream, A);
   } else {
     if (n == 256 && lines == 16) hipLaunchKernelGGL((axis_fx_f64_n256_l16<KIND, XAX>), g, b, lds, c->stream, A);
     else if (n == 256) hipLaunchKernelGGL((axis_fx_f64_n256_l32<KIND, XAX>), g, b, lds, c->stream, A);
@@ -928,7 +939,8 @@ static int axis_fx_launch(fbse_ctx *c, const AxisArgs<T> &A, int n, int lines) {
 
 template <typename T> int axis_fx_pass(fbse_ctx *c, int kind, const AxisArgs<T> &A, int lines) {
   const bool kin = kind == kPassKin || kind == kPassPrec;
-  if (!axis_fx_length(A.n) || (lines != 16 && lines != 32) || A.lines != lines || (A.ksqr && kin) ||
+  const bool lines_ok = lines == 16 || lines == 32 || (lines == 64 && sizeof(T) == 4);
+  if (!axis_fx_length(A.n) || !lines_ok || A.lines != lines || (A.ksqr && kin) ||
       (kin && A.xaxis) || (kind == kPassBwdCombine && !A.xaxis))
     return set_err(c, FBSE_ERR_UNSUPPORTED, "no fixed axis instance for length %d, %d lines", A.n, lines);
   // instances: forward and backward on either layout, k^2 passes on z (strided),

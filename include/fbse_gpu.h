@@ -195,10 +195,11 @@ int fbse_set_window(fbse_ctx *ctx, int window);
  * 1..3 stage 12, 16, 32 elements per thread, FP64 instances 1..2 stage 12, 16;
  * on cubic grids with a separable k^2 table of a size with a fixed instance
  * (FP32: 60, 64, 72, 80, 90, 96, 100, 108, 120; FP64: 64, 96) one more
- * instance, FP32 4 / FP64 3, runs the fixed-schedule persistent passes).  Variant 0
- * with lines = 1016 or 1032 forces the fixed-schedule per-axis instances
- * (16 or 32 lines per tile) on axes of length 160 or 256 (the generic kernel
- * with 16 or 32 lines elsewhere); the autotuner considers both. */
+ * instance, FP32 4 / FP64 3, runs the fixed-schedule persistent passes).
+ * Variant 0 with lines = 1016, 1032 or 1064 (FP32 only) forces the
+ * fixed-schedule per-axis instances (16, 32 or 64 lines per tile) on axes of
+ * length 160 or 256 (the generic kernel with that many lines elsewhere); the
+ * autotuner considers all of them. */
 int fbse_set_fft_kernel(fbse_ctx *ctx, int variant, int lines);
 
 /* Plane/slab passes: the z-slab pass cuts each xz slab into `split` tiles of

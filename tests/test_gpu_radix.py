@@ -71,8 +71,8 @@ def test_fixed_instances(n):
 
 @pytest.mark.parametrize("grid", [(256, 160, 12), (160, 12, 256)])
 def test_fixed_axis_instances(grid):
-    """The fixed-schedule per-axis instances (axis lengths 160 and 256, 16 or
-    32 lines per tile: x rows forward and combine, y lines forward and
+    """The fixed-schedule per-axis instances (axis lengths 160 and 256, 16,
+    32 or 64 lines per tile: x rows forward and combine, y lines forward and
     backward, z lines with k^2) against the generic per-axis kernel: H in FP64
     to FFT rounding, the mixed filter to FP32 rounding."""
     s = small_system(n=grid, dx=0.5, nspin=2, is_complex=1, so=1, nl=1, natoms=6, n_so_proj=1, n_nl_proj=1)
@@ -89,7 +89,7 @@ def test_fixed_axis_instances(grid):
     u, _, _ = g.ortho(random_states(s, 4, seed=10), 4)
     ev0 = np.linspace(-0.5, 0.5, 4)
     _, rref, _ = g.refine(u, 4, ev0, -1.0, 1.0, 8)
-    for lines in (1016, 1032):
+    for lines in (1016, 1032, 1064):  # 1064: FP32 fixed instance, FP64 generic 64-line tiles
         g.set_fft_kernel(0, lines)
         out = g.apply_h(psi)
         assert rel(out, ref) < 1e-12, lines
