@@ -432,7 +432,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--window", type=int, default=0,
-                    help="accumulation window depth K (0 = as deep as 40%% of the free HBM allows)")
+                    help="accumulation window depth K (0 = as deep as 60%% of the free HBM allows)")
     ap.add_argument("--config", default="c1_so_96")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")

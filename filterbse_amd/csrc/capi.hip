@@ -996,6 +996,7 @@ template <typename T> struct FilterRun {
     if ((rc = fill_zero(c, c->d_step, sizeof(long))) || (rc = step_advance(c, c->d_step, R))) return rc;
     FBSE_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int crc = FBSE_OK;
+    c->capturing = 1;
     c->dyn_a0_tab = d_a0;
     c->dyn_step_base = c->d_step;
     for (int i = 0; i < R && !crc; ++i) {
@@ -1011,6 +1012,7 @@ template <typename T> struct FilterRun {
                                  packed, real_an, c->d_step);
     if (!crc) crc = step_advance(c, c->d_step, R);
     hipError_t ee = hipStreamEndCapture(c->stream, &g);
+    c->capturing = 0;
     hipError_t ei = (crc || ee != hipSuccess) ? hipSuccess : hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
     if (crc) return crc;
     if (ee != hipSuccess || ei != hipSuccess)
@@ -1045,15 +1047,21 @@ template <typename T> struct FilterRun {
   }
 };
 
-// window depth: forced, or as deep as ~40% of free memory allows for all the
-// runs' rings together (<= kMaxWindow)
+// window depth: forced, or as deep as 60% of free memory allows for all the
+// runs' rings together (<= kMaxWindow); the rest covers the FP32 staging
+// slot, the autotuners' scratch and the projection coefficients.  The rings
+// stay within kMaxRingBytes: a 165 GB FP64 ring (K = 89 at 96^3) made the
+// filter twice as slow as a 110 GB one (K = 59), while rings up to 117 GB ran
+// at full speed (FP32, K = 127).
+constexpr double kMaxRingBytes = 120e9;
 static int window_depth(int forced, size_t ring_bytes_per_slot, size_t held) {
   if (forced > 0) return std::min(forced, kMaxWindow);
   size_t fr = 0, tot = 0;
   (void)hipMemGetInfo(&fr, &tot);
   // rings reuse the contexts' scratch buffers: what they hold counts as free
   fr += held;
-  long kk = (long)((0.4 * (double)fr) / (double)ring_bytes_per_slot) - 2;
+  const double room = std::min(0.6 * (double)fr, kMaxRingBytes);
+  long kk = (long)(room / (double)ring_bytes_per_slot) - 2;
   return (int)std::max(1L, std::min((long)kMaxWindow, kk));
 }
 

@@ -122,6 +122,7 @@ struct fbse_ctx {
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int concurrent_proj = 1;
+  int capturing = 0;            // the filter is capturing a Newton window into a hipGraph
   rocblas_handle blas = nullptr;
   std::string err;
 

@@ -549,10 +549,17 @@ template <typename T> static int ensure_pjd(fbse_ctx *c, long nstates) {
 // combine; *joined = false: the main stream must wait for ev_join before
 // anything reads d.  Runs on the main stream when profiling (every launch
 // timed alone) or when the side stream is off.
+// Not while the main stream is being captured into a window graph: a fork
+// captured there (event record and wait nodes, the projection launched on the
+// side stream) kept the memory it touched referenced after the graph was
+// destroyed, so the filter's ring was not returned by hipFree (measured: 11 GB
+// lost per graph-replayed filter of a 160^3 test grid, 60 GB at 256^3;
+// nothing with the projection on the main stream).
 template <typename T>
 static int project_forked(fbse_ctx *c, const cplx_t<T> *psi, long nstates, double a1, cplx_t<T> *d, bool *joined) {
   *joined = true;
-  if (!(c->concurrent_proj && c->side && !c->profile)) return proj_project<T>(c, psi, nstates, a1, d);
+  if (!(c->concurrent_proj && c->side && !c->profile) || c->capturing)
+    return proj_project<T>(c, psi, nstates, a1, d);
   hipStream_t main_stream = c->stream;
   FBSE_HIP(c, hipEventRecord(c->ev_fork, main_stream));
   FBSE_HIP(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
