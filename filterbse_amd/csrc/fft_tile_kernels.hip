@@ -1225,6 +1225,7 @@ static int tile_pass_v(fbse_ctx *c, int v, int kind, const cplx_t<T> *src, cplx_
     const int n = fixed_n<T>(c);
     if (!n || A.col_split != 1 || kind == kTileBwdXY || kind == kTilePrecZ)
       return set_err(c, FBSE_ERR_UNSUPPORTED, "no fixed FFT instance for this pass");
+    if (ntiles <= 0) return FBSE_OK;
     int g;
     switch (kind) {
       case kTileFwdXY: g = fx_grid<T, kTileFwdXY>(c, n, ntiles); break;
