@@ -11,7 +11,15 @@
 // loads one tile into LDS with rows of nx contiguous elements, transforms it
 // in place with register-staged Stockham passes (every thread reads its
 // butterflies' inputs into registers, the group syncs, outputs are written to
-// their self-sorted positions), and writes it back.  Radices 2, 3, 4, 5 and 8.
+// their self-sorted positions), and writes it back.  Radices 2, 3, 4, 5, 8 and
+// the composite register butterflies 6, 9, 10, 12, 15, 16.
+//
+// Two kinds of instance: generic ones (any line lengths and radix schedule,
+// chosen at run time, one function call per radix pass) and fixed-schedule
+// persistent ones for given sizes (fx_body: the schedule and strides are
+// compile-time constants, the passes are inlined, work-groups loop over
+// tiles).  The fixed-schedule per-axis instances (axis_fx_body) reuse the same
+// inlined passes on the per-axis tiles of fft_kernels.hip.
 //
 // Tile geometry (element (r, c) of tile t of batch item b):
 //   xy plane : rows r = y (ny), cols c = x (nx), tile t = z,
