@@ -447,7 +447,7 @@ def main():
                     help="only measure state-file streaming of N FP64 spinor states of 96^3, then exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the filter on a single MI355X; "
+        print("bench.py measures one GPU: the filter runs on a single MI355X; "
               "--gpus %d is not supported (start states would shard across GPUs with no exchange "
               "until one all-to-all before ortho)" % args.gpus, file=sys.stderr)
         sys.exit(2)

@@ -17,7 +17,7 @@ No counterpart in the reference, which starts every filter from ran2 noise
    on the scale of the dot, then multiplied by the square root of the
    network's channel-2 prediction on the new dot (fbse_warm_start_dev):
    psi_0 -> sqrt(rho) * lowpass(psi_0), renormalised.  The filter, ortho, RR
-   and sigma_E then run unchanged and judge the result; the plain random
+   and sigma_E then run unchanged and measure the result; the plain random
    start stays the fallback.
 """
 import math

@@ -8,7 +8,7 @@
  * its band-edge states; fbse_warm_start_dev turns a predicted density into
  * start states in the reference psi_rank layout (an envelope sqrt(rho) on the
  * ran2 start states), so the plain random-start filter stays the fallback and
- * the judge of the result.
+ * the measure of the result.
  *
  * Network: n_in -> width -> width -> width -> n_out, SiLU between layers, a
  * linear output; BF16 weights and activations, FP32 accumulation (MFMA
