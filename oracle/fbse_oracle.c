@@ -9,7 +9,7 @@
  * (compile with -ffp-contract=off) so that integer/index work is bit-exact and
  * floating-point work differs only where the reference used FFTW/MKL.
  *
- * Parity pins (what checks each function; see tests/test_oracle_*.py):
+ * Parity pins (what checks each function; tests/test_oracle.py):
  *   or_ran_nrc            Numerical Recipes ran2 known answers + structure
  *   or_init_kvec/ksqr     closed form k = j*dk, KE cap, separability
  *   or_init_grid_params   CsPbI3 run.dat grid (44^3, xmin=-11) + rules
@@ -22,6 +22,11 @@
  *   or_filter_cycle       Newton filter on an operator with known spectrum
  *   or_calc_sigma_E       sigma = 0 on eigenstates; = ||H psi - E psi||
  *   the whole chain       CsPbI3 reference eval.dat (GPU end-to-end test)
+ *
+ * Parity unpinned: the exact rounding of the reference's FFTW transforms (no
+ * reference test holds transformed grids; only the composed eigenvalues pin
+ * them), and the Emin of get_energy_range where the reference's
+ * tau -= 0.025 guard (energy.c:253-259) fires, which depends on rounding order.
  */
 #include <math.h>
 #include <stdlib.h>

@@ -6,6 +6,11 @@ cpu_baseline leg of bench.py, never by the filterbse_amd package.  It is the
 checker, not the thing measured or shipped.
 
 Reference citations are relative to the FilterBSE tree (filter_mpi/).
+
+Parity unpinned: the exact output of the reference's MKL ?gesvd/?heev calls
+(eigenvector phases, the basis U of the SVD); numpy's LAPACK stands in for
+them and only the composed eigenvalues against the reference's eval.dat pin
+them.
 """
 import ctypes as C
 import os
