@@ -71,10 +71,12 @@ GPU_SIGNATURES = {
     "fbse_rayleigh_ritz_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, DP]),
     "fbse_sigma": (C.c_int, [C.c_void_p, DP, C.c_long, DP, DP]),
     "fbse_sigma_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, DP, DP]),
-    "fbse_refine": (C.c_int, [C.c_void_p, DP, C.c_long, DP, DP, C.c_double, C.c_double, C.c_double, C.c_long,
-                              C.c_double, DP, DP, LP]),
+    "fbse_refine": (C.c_int, [C.c_void_p, DP, C.c_long, DP, DP, C.c_double, C.c_double, C.c_double, C.c_double,
+                              DP, DP, LP]),
     "fbse_refine_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, DP, DP, C.c_double, C.c_double, C.c_double,
-                                  C.c_long, C.c_double, DP, DP, LP]),
+                                  C.c_double, DP, DP, LP]),
+    "fbse_overlap_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_void_p, C.c_long, DP]),
+    "fbse_rotate_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, DP, C.c_long]),
     "fbse_energy_range": (C.c_int, [C.c_void_p, DP, DP, C.c_long, C.c_double, C.c_double, DP]),
     "fbse_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "fbse_profile_read": (C.c_int, [C.c_void_p, C.c_int, LP, DP]),
@@ -120,7 +122,6 @@ GPU_SIGNATURES = {
                                          C.c_long, C.c_long, LP]),
     "fbse_set_io_chunk": (C.c_int, [C.c_void_p, C.c_size_t]),
     "fbse_set_slab_split": (C.c_int, [C.c_void_p, C.c_int]),
-    "fbse_set_refine_basis": (C.c_int, [C.c_void_p, C.c_int]),
     "fbse_dev_alloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     "fbse_dev_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fbse_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
@@ -404,23 +405,40 @@ class Context:
         self._check(gpu().fbse_sigma_dev(self.ctx, C.c_void_p(d_psi), n, _dp(s), _dp(e)))
         return s, e
 
-    def refine(self, psi, n_ritz, eval_, e_lo, e_hi, capacity, e_p=1.0, sigma=None, sigma_max=0.0):
-        buf = np.zeros(capacity * self.stlen)
-        buf[: n_ritz * self.stlen] = _f64(psi)[: n_ritz * self.stlen]
-        ev, sg, n = np.zeros(capacity), np.zeros(capacity), C.c_long(0)
-        sp = _dp(_f64(sigma)) if sigma is not None else None
-        self._check(gpu().fbse_refine(self.ctx, _dp(buf), n_ritz, _dp(_f64(eval_)), sp, sigma_max, e_lo, e_hi,
-                                      capacity, e_p, _dp(ev), _dp(sg), C.byref(n)))
-        k = n.value
-        return buf[: k * self.stlen], ev[:k], sg[:k]
+    def overlap_dev(self, d_a, n1, d_b, n2):
+        """dv * A^H B (n1 x n2; complex for complex states) of two device state blocks."""
+        cw = 2 if self.is_complex else 1
+        s = np.zeros(n1 * n2 * cw)
+        self._check(gpu().fbse_overlap_dev(self.ctx, C.c_void_p(d_a), n1, C.c_void_p(d_b), n2, _dp(s)))
+        if self.is_complex:
+            return s.view(np.complex128).reshape(n2, n1).T
+        return s.reshape(n2, n1).T
 
-    def refine_dev(self, d_psi, n_ritz, eval_, e_lo, e_hi, capacity, e_p=1.0, sigma=None, sigma_max=0.0):
-        ev, sg, n = np.zeros(capacity), np.zeros(capacity), C.c_long(0)
-        sp = _dp(_f64(sigma)) if sigma is not None else None
-        self._check(gpu().fbse_refine_dev(self.ctx, C.c_void_p(d_psi), n_ritz, _dp(_f64(eval_)), sp, sigma_max,
-                                          e_lo, e_hi, capacity, e_p, _dp(ev), _dp(sg), C.byref(n)))
-        k = n.value
-        return ev[:k], sg[:k]
+    def rotate_dev(self, d_a, v):
+        """In place: the first nout states of the device block become A V (V nin x nout)."""
+        v = np.asarray(v)
+        nin, nout = v.shape
+        if self.is_complex:
+            buf = np.ascontiguousarray(v.T, dtype=np.complex128).view(np.float64).ravel()
+        else:
+            buf = np.ascontiguousarray(v.T, dtype=np.float64).ravel()
+        self._check(gpu().fbse_rotate_dev(self.ctx, C.c_void_p(d_a), nin, _dp(buf), nout))
+
+    def refine(self, psi, n_ritz, eval_, sigma, e_lo, e_hi, e_p=1.0, sigma_max=1e-2):
+        """One guarded FP64 refinement sweep (fbse_refine) of host Ritz vectors:
+        returns (psi, eval, sigma, replaced), the pairs in their slots."""
+        buf = _f64(psi)[: n_ritz * self.stlen].copy()
+        ev, sg, n = np.zeros(n_ritz), np.zeros(n_ritz), C.c_long(0)
+        self._check(gpu().fbse_refine(self.ctx, _dp(buf), n_ritz, _dp(_f64(eval_)), _dp(_f64(sigma)), sigma_max,
+                                      e_lo, e_hi, e_p, _dp(ev), _dp(sg), C.byref(n)))
+        return buf, ev, sg, n.value
+
+    def refine_dev(self, d_psi, n_ritz, eval_, sigma, e_lo, e_hi, e_p=1.0, sigma_max=1e-2):
+        """fbse_refine_dev: returns (eval, sigma, replaced) of the n_ritz pairs."""
+        ev, sg, n = np.zeros(n_ritz), np.zeros(n_ritz), C.c_long(0)
+        self._check(gpu().fbse_refine_dev(self.ctx, C.c_void_p(d_psi), n_ritz, _dp(_f64(eval_)), _dp(_f64(sigma)),
+                                          sigma_max, e_lo, e_hi, e_p, _dp(ev), _dp(sg), C.byref(n)))
+        return ev, sg, n.value
 
     def set_fft_kernel(self, variant, lines=0):
         self._check(gpu().fbse_set_fft_kernel(self.ctx, variant, lines))
@@ -428,10 +446,6 @@ class Context:
     def set_graph(self, enable):
         """Newton-window hipGraph replay on (1, default) or off (0)."""
         self._check(gpu().fbse_set_graph(self.ctx, int(enable)))
-
-    def set_refine_basis(self, full):
-        """fbse_refine's basis: 0 selected pairs + corrections, 1 all Ritz vectors + corrections."""
-        self._check(gpu().fbse_set_refine_basis(self.ctx, int(full)))
 
     def trim(self):
         """Release the context's work buffers (allocated again on demand)."""

@@ -19,7 +19,7 @@ GPU_LIB = os.path.join(HERE, "libfbse_gpu.so")
 HOST_LIB = os.path.join(HERE, "libfbse_host.so")
 
 HIP_SOURCES = ["fft_kernels.hip", "fft_tile_kernels.hip", "ham_kernels.hip", "capi.hip", "mlp_kernels.hip",
-               "build_kernels.hip", "io.hip"]
+               "build_kernels.hip", "io.hip", "gemm_kernels.hip"]
 HOST_SOURCES = ["fbse_host.c"]
 
 HIPCC_FLAGS = [

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 using namespace fbse;
@@ -587,11 +588,6 @@ extern "C" int fbse_set_slab_split(fbse_ctx *c, int split) {
 extern "C" int fbse_set_graph(fbse_ctx *c, int enable) {
   if (!c || enable < 0 || enable > 1) return FBSE_ERR_ARG;
   c->use_graph = enable;
-  return FBSE_OK;
-}
-extern "C" int fbse_set_refine_basis(fbse_ctx *c, int full) {
-  if (!c || full < 0 || full > 1) return FBSE_ERR_ARG;
-  c->refine_full = full;
   return FBSE_OK;
 }
 extern "C" int fbse_trim(fbse_ctx *c) {
@@ -1239,9 +1235,9 @@ __global__ void mirror_upper_kernel(double *H, long n, int cplx) {
   }
 }
 
-// Split-K target of the Gram products: about two work-groups per CU, slabs of
-// at least this many rows.
-static const long kGramGroups = 512;
+// Split-K target of the Gram products: about four work-groups per CU (two
+// are resident at a time), slabs of at least this many rows.
+static const long kGramGroups = 2048;
 static const long kGramMinRows = 4096;
 
 struct LinAlg {
@@ -1250,97 +1246,54 @@ struct LinAlg {
   long m;   // rows (complex elements or reals)
   size_t es() const { return cplx ? 16 : 8; }
   int cw() const { return cplx ? 2 : 1; }  // doubles per element
-  // C(n1 x n2, ldc) = A^H B over k rows; A, B with leading dimension m
-  int gemm_h(const double *A, long n1, const double *B, long n2, long k, double *Cm, long ldc) {
-    if (cplx) {
-      const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
-      FBSE_BLAS(c, rocblas_zgemm(c->blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, (int)n1,
-                                 (int)n2, (int)k, &one, (const rocblas_double_complex *)A, (int)m,
-                                 (const rocblas_double_complex *)B, (int)m, &zero, (rocblas_double_complex *)Cm,
-                                 (int)ldc));
-    } else {
-      const double one = 1.0, zero = 0.0;
-      FBSE_BLAS(c, rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, (int)n1, (int)n2,
-                                 (int)k, &one, A, (int)m, B, (int)m, &zero, Cm, (int)ldc));
+  // C (n1 x n2, ldc) = A^H B over the m rows (A, B with leading dimension m)
+  // on the FP64 matrix cores (gemm_kernels.hip).  Each work-group forms one
+  // 64 x 64 output tile over one slab of rows; with few tiles the rows are
+  // split into P slabs whose partial Grams are summed in a fixed order, so
+  // results stay deterministic.  upper (n1 == n2): only the tiles on or above
+  // the diagonal are formed (the lower triangle of C is then left to the caller).
+  int gram_core(const double *A, long n1, const double *B, long n2, double *Cm, long ldc, bool upper) {
+    const long tiles = fbse::gram_tile_count(n1, n2, upper);
+    long P = std::max(1L, std::min((kGramGroups + tiles - 1) / tiles, m / kGramMinRows));
+    long slab = (m + P - 1) / P;
+    slab = (slab + 15) / 16 * 16;
+    P = (m + slab - 1) / slab;
+    if (P <= 1) {
+      if (fbse::launch_gram_mfma(c->stream, cplx, A, m, B, m, m, n1, n2, upper, m, 1, Cm, ldc, 0))
+        return set_err(c, FBSE_ERR_HIP, "Gram kernel launch (%ld x %ld over %ld rows)", n1, n2, m);
+      return FBSE_OK;
     }
-    return FBSE_OK;
-  }
-  // C(n1 x n2) = A^H B   with A (m x n1), B (m x n2), lda = ldb = m.
-  // The library GEMM runs one work-group per 64 x 64 output tile over all m
-  // rows, so a product with few tiles leaves most CUs idle (a 32 x 32 Gram of
-  // 1.8M rows takes as long as a 1024 x 1024 one).  Such products are split
-  // into P slabs of rows (a strided-batched GEMM writes P partial Grams) and
-  // the partials are summed in a fixed order, so results stay deterministic.
-  int gram(const double *A, long n1, const double *B, long n2, double *Cm, long ldc) {
-    const long tiles = ((n1 + 63) / 64) * ((n2 + 63) / 64);
-    long P = 1;
-    if (tiles < kGramGroups) P = std::min((kGramGroups + tiles - 1) / tiles, m / kGramMinRows);
-    if (P <= 1) return gemm_h(A, n1, B, n2, m, Cm, ldc);
-    const long kc = m / P, rem = m - P * kc, parts = P + (rem > 0 ? 1 : 0);
     const long blk = n1 * n2;  // elements of one partial Gram
-    int rc = grow(c, &c->d_gram, &c->gram_bytes, (size_t)parts * blk * es());
+    int rc = grow(c, &c->d_gram, &c->gram_bytes, (size_t)P * blk * es());
     if (rc) return rc;
     double *part = (double *)c->d_gram;
-    if (cplx) {
-      const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
-      FBSE_BLAS(c, rocblas_zgemm_strided_batched(
-                       c->blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, (int)n1, (int)n2,
-                       (int)kc, &one, (const rocblas_double_complex *)A, (int)m, (rocblas_stride)kc,
-                       (const rocblas_double_complex *)B, (int)m, (rocblas_stride)kc, &zero,
-                       (rocblas_double_complex *)part, (int)n1, (rocblas_stride)blk, (int)P));
-    } else {
-      const double one = 1.0, zero = 0.0;
-      FBSE_BLAS(c, rocblas_dgemm_strided_batched(c->blas, rocblas_operation_transpose, rocblas_operation_none, (int)n1,
-                                                 (int)n2, (int)kc, &one, A, (int)m, (rocblas_stride)kc, B, (int)m,
-                                                 (rocblas_stride)kc, &zero, part, (int)n1, (rocblas_stride)blk,
-                                                 (int)P));
-    }
-    if (rem > 0 && (rc = gemm_h(A + (size_t)P * kc * cw(), n1, B + (size_t)P * kc * cw(), n2, rem,
-                                part + (size_t)P * blk * cw(), n1)))
-      return rc;
+    if (fbse::launch_gram_mfma(c->stream, cplx, A, m, B, m, m, n1, n2, upper, slab, P, part, n1, blk))
+      return set_err(c, FBSE_ERR_HIP, "Gram kernel launch (%ld x %ld over %ld rows, %ld slabs)", n1, n2, m, P);
     const long tot = blk * cw();
     hipLaunchKernelGGL(sum_parts_kernel, dim3((unsigned)std::min(2048L, (tot + 255) / 256)), dim3(256), 0, c->stream,
-                       (const double *)part, parts, n1, tot, cw(), Cm, ldc);
+                       (const double *)part, P, n1, tot, cw(), Cm, ldc);
     FBSE_HIP(c, hipGetLastError());
     return FBSE_OK;
   }
+  // C(n1 x n2, ldc) = A^H B   with A (m x n1), B (m x n2), lda = ldb = m.
+  int gram(const double *A, long n1, const double *B, long n2, double *Cm, long ldc) {
+    return gram_core(A, n1, B, n2, Cm, ldc, false);
+  }
   // G (n x n) = A^H B for a product known to be Hermitian (B = A, or B = H A).
-  // Only the blocks on and above the diagonal are formed (column block J of B
-  // against columns 0..end of J of A), then the lower triangle is mirrored:
-  // 9/16 of the full product's flops at n = 1024.
+  // Only the tiles on and above the diagonal are formed, then the lower
+  // triangle is mirrored from the upper one.
   int gram_upper(const double *A, const double *B, long n, double *G) {
-    const long w = 128;
-    if (n < 4 * w) {
-      if (A == B) return gram(A, n, B, n, G, n);
-      int rc = gram(A, n, B, n, G, n);
-      if (rc) return rc;
-      hipLaunchKernelGGL(hermitize_kernel, dim3(256), dim3(256), 0, c->stream, G, n, cplx ? 1 : 0);
-      FBSE_HIP(c, hipGetLastError());
-      return FBSE_OK;
-    }
-    int rc;
-    for (long j0 = 0; j0 < n; j0 += w) {
-      const long nw = std::min(w, n - j0);
-      if ((rc = gram(A, j0 + nw, B + (size_t)j0 * m * cw(), nw, G + (size_t)j0 * n * cw(), n))) return rc;
-    }
+    int rc = gram_core(A, n, B, n, G, n, true);
+    if (rc) return rc;
     hipLaunchKernelGGL(mirror_upper_kernel, dim3(256), dim3(256), 0, c->stream, G, n, cplx ? 1 : 0);
     FBSE_HIP(c, hipGetLastError());
     return FBSE_OK;
   }
   int gram_self(const double *A, long n, double *G) { return gram_upper(A, A, n, G); }
-  // Out (m x nout) = A[:, 0:nin] * V (nin x nout, ldv)
-  int product(const double *A, long nin, const double *V, long ldv, long nout, double *Out) {
-    if (cplx) {
-      const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
-      FBSE_BLAS(c, rocblas_zgemm(c->blas, rocblas_operation_none, rocblas_operation_none, (int)m, (int)nout,
-                                 (int)nin, &one, (const rocblas_double_complex *)A, (int)m,
-                                 (const rocblas_double_complex *)V, (int)ldv, &zero, (rocblas_double_complex *)Out,
-                                 (int)m));
-    } else {
-      const double one = 1.0, zero = 0.0;
-      FBSE_BLAS(c, rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, (int)m, (int)nout,
-                                 (int)nin, &one, A, (int)m, V, (int)ldv, &zero, Out, (int)m));
-    }
+  // Out (m x nout) (+)= A[:, 0:nin] * V (nin x nout, ldv), on the FP64 matrix cores
+  int product(const double *A, long nin, const double *V, long ldv, long nout, double *Out, bool acc = false) {
+    if (fbse::launch_rot_mfma(c->stream, cplx, A, m, m, nin, V, ldv, nout, Out, m, acc))
+      return set_err(c, FBSE_ERR_HIP, "rotation kernel launch (%ld rows, %ld -> %ld)", m, nin, nout);
     return FBSE_OK;
   }
   // Out (r x nout) = X (r x r2, ldx = r) * Y (r2 x nout, ldy = r2): small dense products
@@ -1357,26 +1310,20 @@ struct LinAlg {
     }
     return FBSE_OK;
   }
-  // A[:, 0:nout] = A[:, 0:nin] * V (nin x nout, ldv), in place by row blocks
+  // A[:, 0:nout] = A[:, 0:nin] * V (nin x nout, ldv), in place by row blocks:
+  // each block's product goes to a scratch block (at most 1 GiB) on the FP64
+  // matrix cores, then back into A
   int rotate(double *A, long nin, const double *V, long ldv, long nout) {
-    long rb = 32768;
-    while (rb > 1024 && (size_t)rb * nout * es() > (1ull << 30)) rb /= 2;
+    long rb = (long)((1ull << 30) / ((size_t)nout * es()));
+    rb = std::max(64L, std::min(rb, m)) / 64 * 64;
+    if (rb > m) rb = m;
     int rc = ensure_scratch2(c, (size_t)rb * nout * es());
     if (rc) return rc;
     for (long r0 = 0; r0 < m; r0 += rb) {
       const long nr = std::min(rb, m - r0);
-      if (cplx) {
-        const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
-        FBSE_BLAS(c, rocblas_zgemm(c->blas, rocblas_operation_none, rocblas_operation_none, (int)nr, (int)nout,
-                                   (int)nin, &one, (const rocblas_double_complex *)A + r0, (int)m,
-                                   (const rocblas_double_complex *)V, (int)ldv, &zero,
-                                   (rocblas_double_complex *)c->d_scratch2, (int)nr));
-      } else {
-        const double one = 1.0, zero = 0.0;
-        FBSE_BLAS(c, rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, (int)nr, (int)nout,
-                                   (int)nin, &one, A + r0, (int)m, V, (int)ldv, &zero, (double *)c->d_scratch2,
-                                   (int)nr));
-      }
+      if (fbse::launch_rot_mfma(c->stream, cplx, A + (size_t)r0 * cw(), m, nr, nin, V, ldv, nout,
+                                (double *)c->d_scratch2, nr))
+        return set_err(c, FBSE_ERR_HIP, "rotation kernel launch (%ld rows, %ld -> %ld)", nr, nin, nout);
       char *dst = (char *)A + (size_t)r0 * es();
       FBSE_HIP(c, hipMemcpy2DAsync(dst, (size_t)m * es(), c->d_scratch2, (size_t)nr * es(), (size_t)nr * es(), nout,
                                    hipMemcpyDeviceToDevice, c->stream));
@@ -1654,6 +1601,46 @@ extern "C" int fbse_rayleigh_ritz(fbse_ctx *c, double *psitot, long n, double *e
   });
 }
 
+// Subspace products on the FP64 matrix cores, for callers that form their
+// own subspace matrices (the dotp / dotpreal loops of diag_H, Hmat.c:141-164,
+// 301-338, and its rotation into the eigenvectors, Hmat.c:230-281).
+extern "C" int fbse_overlap_dev(fbse_ctx *c, const double *d_a, long n1, const double *d_b, long n2, double *s) {
+  if (!c || !d_a || !d_b || !s || n1 < 1 || n2 < 1) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipSetDevice(c->device));
+  LinAlg la{c, c->is_complex != 0, c->nsg};
+  const size_t bytes = (size_t)n1 * n2 * la.es();
+  double *d_s = nullptr;
+  FBSE_HIP(c, hipMalloc(&d_s, bytes));
+  int rc = la.gram(d_a, n1, d_b, n2, d_s, n1);
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(s, d_s, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = set_err(c, FBSE_ERR_HIP, "overlap copy: %s", hipGetErrorString(e));
+  }
+  hipStreamSynchronize(c->stream);
+  hipFree(d_s);
+  if (rc) return rc;
+  const size_t nd = (size_t)n1 * n2 * la.cw();
+  for (size_t i = 0; i < nd; ++i) s[i] *= c->dv;
+  return FBSE_OK;
+}
+
+extern "C" int fbse_rotate_dev(fbse_ctx *c, double *d_a, long nin, const double *v, long nout) {
+  if (!c || !d_a || !v || nin < 1 || nout < 1 || nout > nin) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipSetDevice(c->device));
+  LinAlg la{c, c->is_complex != 0, c->nsg};
+  const size_t bytes = (size_t)nin * nout * la.es();
+  double *d_v = nullptr;
+  FBSE_HIP(c, hipMalloc(&d_v, bytes));
+  int rc = FBSE_OK;
+  hipError_t e = hipMemcpyAsync(d_v, v, bytes, hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) rc = set_err(c, FBSE_ERR_HIP, "rotation matrix copy: %s", hipGetErrorString(e));
+  if (!rc) rc = la.rotate(d_a, nin, d_v, nin, nout);
+  hipStreamSynchronize(c->stream);
+  hipFree(d_v);
+  return rc;
+}
+
 // ---------------------------------------------------------------------------
 // FP64 residual refinement of the Ritz pairs in a window (mixed-precision path)
 // ---------------------------------------------------------------------------
@@ -1686,89 +1673,233 @@ int fbse::precondition_dev(fbse_ctx *c, const double *d_r, double *d_out, long n
   return FBSE_OK;
 }
 
-extern "C" int fbse_refine_dev(fbse_ctx *c, double *d_psi, long n_ritz, const double *eval, const double *sigma,
-                               double sigma_max, double e_lo, double e_hi, long capacity, double e_p,
-                               double *eval_out, double *sigma_out, long *n_out) {
-  if (!c || !d_psi || !eval || !eval_out || !sigma_out || !n_out || n_ritz < 0 || capacity < 2) return FBSE_ERR_ARG;
-  FBSE_HIP(c, hipSetDevice(c->device));
-  if (e_p <= 0.0) e_p = 1.0;
+// M[i + i*ld] = d[i] for i < n (raw diagonal of the Rayleigh-Ritz block)
+__global__ void set_diag_kernel(double *M, long ld, long n, const double *d, int cplx) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    if (cplx)
+      ((double2 *)M)[i + i * ld] = make_double2(d[i], 0.0);
+    else
+      M[i + i * ld] = d[i];
+  }
+}
+
+// One guarded FP64 refinement sweep (Davidson step with a kinetic-energy
+// preconditioner) over the Ritz pairs of a preceding Rayleigh-Ritz:
+//   1. the k selected pairs (eval in [e_lo, e_hi], sigma < sigma_max, lowest
+//      sigma first; k limited by memory) get residuals r = H u - theta u and
+//      corrections w = P r, P = 1 / (1 + 0.5|k|^2 / e_p) in k-space;
+//   2. the corrections are orthogonalised against all n_ritz Ritz vectors
+//      (twice) and among themselves (rank cut), then H w is formed;
+//   3. Rayleigh-Ritz over [U, W]: U^H H U = diag(theta) (U are Ritz vectors),
+//      U^H H W and W^H H W from the matrix-core Grams;
+//   4. a new pair whose coefficient on the old Ritz vector u_i carries more
+//      than half its weight (so the match is one to one) replaces u_i only
+//      when its own residual is lower than u_i's.
+// Every pair stays where it was; none gets a larger residual.
+static int refine_impl(fbse_ctx *c, double *d_psi, long n, const double *eval, const double *sigma,
+                       double sigma_max, double e_lo, double e_hi, double e_p, double *eval_out, double *sigma_out,
+                       long *n_replaced) {
+  LinAlg la{c, c->is_complex != 0, c->nsg};
   const long sl = stlen_of(c);
+  const size_t es = la.es();
+  const int cw = la.cw();
+  const double dv = c->dv;
+  for (long i = 0; i < n; ++i) {
+    eval_out[i] = eval[i];
+    sigma_out[i] = sigma[i];
+  }
+  *n_replaced = 0;
   std::vector<long> sel;
-  for (long i = 0; i < n_ritz; ++i)
-    if (eval[i] >= e_lo && eval[i] <= e_hi && (!sigma || sigma[i] < sigma_max)) sel.push_back(i);
-  *n_out = 0;
+  for (long i = 0; i < n; ++i)
+    if (eval[i] >= e_lo && eval[i] <= e_hi && sigma[i] < sigma_max) sel.push_back(i);
   if (sel.empty()) return FBSE_OK;
-  // Compact basis (default): the selected pairs are moved to the front and the
-  // basis is [u_sel, P r_sel] (k <= capacity / 2).  Full basis
-  // (fbse_set_refine_basis(ctx, 1)) when [all n_ritz Ritz vectors, P r of every
-  // selected pair] fits in capacity: the corrections stay orthogonal to the
-  // whole filtered subspace, at the price of an ortho + RR over n_ritz + k.
-  const bool full = c->refine_full && n_ritz + (long)sel.size() <= capacity;
-  const long k = full ? (long)sel.size() : std::min((long)sel.size(), capacity / 2);
-  const long nbase = full ? n_ritz : k;  // corrections go to columns nbase .. nbase+k-1
-  std::vector<double> theta(k);
-  for (long j = 0; j < k; ++j) theta[j] = eval[sel[j]];
-  double *d_u = nullptr, *d_hu = nullptr, *d_th = nullptr;
-  FBSE_HIP(c, big_malloc(c, (void **)&d_hu, (size_t)k * sl * 8));
-  FBSE_HIP(c, hipMalloc(&d_th, (size_t)k * 8));
+  std::stable_sort(sel.begin(), sel.end(), [&](long a, long b) { return sigma[a] < sigma[b]; });
+  // two blocks of k states (corrections W and H W / the new vectors); the
+  // filter's ring is not needed any more
+  if (c->scratch_bytes) release_work(c);
+  const long kfit = states_that_fit(c, (size_t)2 * sl * 8, 0.6) ;
+  const long k = std::max(1L, std::min((long)sel.size(), kfit));
+  sel.resize(k);
   int rc = FBSE_OK;
-  if (full) {
-    // gather the selected vectors into a work block (they stay in place too)
-    if (big_malloc(c, (void **)&d_u, (size_t)k * sl * 8) != hipSuccess)
-      rc = set_err(c, FBSE_ERR_NOMEM, "refine work block");
-    for (long j = 0; j < k && !rc; ++j) {
-      hipError_t e = hipMemcpyAsync(d_u + j * sl, d_psi + sel[j] * sl, sl * 8, hipMemcpyDeviceToDevice, c->stream);
-      if (e != hipSuccess) rc = set_err(c, FBSE_ERR_HIP, "copy: %s", hipGetErrorString(e));
+  double *X = nullptr, *Y = nullptr, *d_th = nullptr, *Cm = nullptr, *M = nullptr, *d_w = nullptr;
+  auto cleanup = [&]() {
+    hipStreamSynchronize(c->stream);
+    hipFree(X);
+    hipFree(Y);
+    hipFree(d_th);
+    hipFree(Cm);
+    hipFree(M);
+    hipFree(d_w);
+  };
+  if (big_malloc(c, (void **)&X, (size_t)k * sl * 8) != hipSuccess ||
+      big_malloc(c, (void **)&Y, (size_t)k * sl * 8) != hipSuccess) {
+    cleanup();
+    return set_err(c, FBSE_ERR_NOMEM, "refine: two blocks of %ld states", k);
+  }
+  std::vector<double> th(std::max(n, k));
+  for (long j = 0; j < k; ++j) th[j] = eval[sel[j]];
+#define REF_HIP(call)                                                                                  \
+  do {                                                                                                 \
+    hipError_t e_ = (call);                                                                            \
+    if (e_ != hipSuccess) {                                                                            \
+      rc = set_err(c, FBSE_ERR_HIP, "refine: %s: %s", #call, hipGetErrorString(e_));                 \
+      cleanup();                                                                                       \
+      return rc;                                                                                       \
+    }                                                                                                  \
+  } while (0)
+#define REF_RC(expr)     \
+  do {                   \
+    if ((rc = (expr))) { \
+      cleanup();         \
+      return rc;         \
+    }                    \
+  } while (0)
+  REF_HIP(hipMalloc(&d_th, (size_t)std::max(n, k) * 8));
+  REF_HIP(hipMemcpyAsync(d_th, th.data(), k * 8, hipMemcpyHostToDevice, c->stream));
+  // 1. residuals of the selected pairs and their corrections W (in X)
+  for (long j = 0; j < k; ++j)
+    REF_HIP(hipMemcpyAsync(X + j * sl, d_psi + sel[j] * sl, sl * 8, hipMemcpyDeviceToDevice, c->stream));
+  REF_RC(apply_h_dev_impl(c, X, Y, k));
+  REF_RC(states_sub_scaled(c, X, Y, k, sl, d_th));
+  REF_RC(precondition_dev(c, Y, X, k, e_p));
+  // 2. W <- W - U (dv U^H W), twice; then orthonormal W Z (W^H W dv = 1)
+  REF_HIP(hipMalloc(&Cm, (size_t)std::max(n, k) * k * es));
+  for (int pass = 0; pass < 2; ++pass) {
+    REF_RC(la.gram(d_psi, n, X, k, Cm, n));
+    std::vector<double> ch((size_t)n * k * cw);
+    REF_HIP(hipMemcpyAsync(ch.data(), Cm, ch.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    REF_HIP(hipStreamSynchronize(c->stream));
+    for (auto &v : ch) v *= -dv;
+    REF_HIP(hipMemcpyAsync(Cm, ch.data(), ch.size() * 8, hipMemcpyHostToDevice, c->stream));
+    REF_RC(la.product(d_psi, n, Cm, n, k, X, true));
+    REF_HIP(hipStreamSynchronize(c->stream));
+  }
+  REF_RC(la.gram_self(X, k, Cm));
+  REF_HIP(hipMalloc(&d_w, (size_t)std::max(n, k) * 8));
+  REF_RC(la.eigh(Cm, k, d_w));
+  std::vector<double> lam(k), q((size_t)k * k * cw);
+  REF_HIP(hipMemcpyAsync(lam.data(), d_w, k * 8, hipMemcpyDeviceToHost, c->stream));
+  REF_HIP(hipMemcpyAsync(q.data(), Cm, q.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  REF_HIP(hipStreamSynchronize(c->stream));
+  const double lmax = std::max(lam[k - 1], 0.0);
+  long k0 = 0;  // eigenvalues ascending: drop directions at the rounding level
+  while (k0 < k && lam[k0] <= 1e-10 * lmax) ++k0;
+  const long k2 = k - k0;
+  if (k2 == 0 || lmax == 0.0) {
+    cleanup();
+    return FBSE_OK;
+  }
+  {
+    std::vector<double> z((size_t)k * k2 * cw);
+    for (long jj = 0; jj < k2; ++jj) {
+      const double f = 1.0 / sqrt(lam[k0 + jj] * dv);
+      for (long r = 0; r < k * cw; ++r) z[(size_t)jj * k * cw + r] = q[(size_t)(k0 + jj) * k * cw + r] * f;
     }
-  } else {
-    // compact the selected Ritz vectors to the front (indices increase, so no overlap)
-    d_u = d_psi;
-    for (long j = 0; j < k && !rc; ++j) {
-      if (sel[j] == j) continue;
-      hipError_t e = hipMemcpyAsync(d_psi + j * sl, d_psi + sel[j] * sl, sl * 8, hipMemcpyDeviceToDevice, c->stream);
-      if (e != hipSuccess) rc = set_err(c, FBSE_ERR_HIP, "copy: %s", hipGetErrorString(e));
+    REF_HIP(hipMemcpyAsync(Cm, z.data(), z.size() * 8, hipMemcpyHostToDevice, c->stream));
+    REF_RC(la.rotate(X, k, Cm, k, k2));
+  }
+  // 3. H W, and the Rayleigh-Ritz matrix of [U, W] (order nt = n + k2)
+  REF_RC(apply_h_dev_impl(c, X, Y, k2));
+  const long nt = n + k2;
+  REF_HIP(hipMalloc(&M, (size_t)nt * nt * es));
+  REF_RC(fill_zero(c, M, (size_t)nt * nt * es));
+  for (long i = 0; i < n; ++i) th[i] = eval[i] / dv;
+  REF_HIP(hipMemcpyAsync(d_th, th.data(), n * 8, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(set_diag_kernel, dim3(64), dim3(256), 0, c->stream, M, nt, n, (const double *)d_th, cw - 1);
+  REF_HIP(hipGetLastError());
+  REF_RC(la.gram(d_psi, n, Y, k2, M + (size_t)n * nt * cw, nt));
+  REF_RC(la.gram(X, k2, Y, k2, M + ((size_t)n * nt + n) * cw, nt));
+  hipLaunchKernelGGL(mirror_upper_kernel, dim3(256), dim3(256), 0, c->stream, M, nt, cw - 1);
+  REF_HIP(hipGetLastError());
+  hipFree(d_w);
+  d_w = nullptr;
+  REF_HIP(hipMalloc(&d_w, (size_t)nt * 8));
+  REF_RC(la.eigh(M, nt, d_w));
+  std::vector<double> mu(nt), Yv((size_t)nt * nt * cw);
+  REF_HIP(hipMemcpyAsync(mu.data(), d_w, nt * 8, hipMemcpyDeviceToHost, c->stream));
+  REF_HIP(hipMemcpyAsync(Yv.data(), M, Yv.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  REF_HIP(hipStreamSynchronize(c->stream));
+  // 4. one-to-one matches: new pair j whose weight on old Ritz vector i > 1/2
+  std::vector<char> is_sel(n, 0);
+  for (long j = 0; j < k; ++j) is_sel[sel[j]] = 1;
+  std::vector<long> cand_j, cand_i;
+  for (long j = 0; j < nt && (long)cand_j.size() < k; ++j) {
+    long best = -1;
+    double bw = 0.0;
+    for (long i = 0; i < n; ++i) {
+      const double *y = &Yv[((size_t)j * nt + i) * cw];
+      const double w2 = cw == 2 ? y[0] * y[0] + y[1] * y[1] : y[0] * y[0];
+      if (w2 > bw) {
+        bw = w2;
+        best = i;
+      }
+    }
+    if (best >= 0 && bw > 0.5 && is_sel[best]) {
+      cand_j.push_back(j);
+      cand_i.push_back(best);
     }
   }
-  if (!rc) {
-    hipError_t e = hipMemcpyAsync(d_th, theta.data(), k * 8, hipMemcpyHostToDevice, c->stream);
-    if (e != hipSuccess) rc = set_err(c, FBSE_ERR_HIP, "copy: %s", hipGetErrorString(e));
+  const long nc = (long)cand_j.size();
+  if (nc == 0) {
+    cleanup();
+    return FBSE_OK;
   }
-  if (!rc) rc = apply_h_dev_impl(c, d_u, d_hu, k);
-  if (!rc) rc = states_sub_scaled(c, d_u, d_hu, k, sl, d_th);
-  if (!rc) rc = precondition_dev(c, d_hu, d_psi + nbase * sl, k, e_p);
-  hipStreamSynchronize(c->stream);
-  if (full) hipFree(d_u);
-  hipFree(d_hu);
-  hipFree(d_th);
-  if (rc) return rc;
-  long rank = 0;
-  if ((rc = ortho_dev_impl(c, d_psi, nbase + k, 1.0e-10, &rank, nullptr))) return rc;
-  if ((rc = rr_dev_impl(c, d_psi, rank, eval_out))) return rc;
-  std::vector<double> ev(rank);
-  if ((rc = energies_dev_impl(c, d_psi, rank, ev.data(), sigma_out))) return rc;
-  *n_out = rank;
+  // new vectors [U, W] y_j into Y: U (n x nc block of y) then += W (k2 x nc)
+  {
+    std::vector<double> yt((size_t)n * nc * cw), yb((size_t)k2 * nc * cw);
+    for (long q2 = 0; q2 < nc; ++q2) {
+      const double *col = &Yv[(size_t)cand_j[q2] * nt * cw];
+      memcpy(&yt[(size_t)q2 * n * cw], col, (size_t)n * cw * 8);
+      memcpy(&yb[(size_t)q2 * k2 * cw], col + (size_t)n * cw, (size_t)k2 * cw * 8);
+    }
+    double *d_yt = Cm;  // n x k >= n x nc
+    REF_HIP(hipMemcpyAsync(d_yt, yt.data(), yt.size() * 8, hipMemcpyHostToDevice, c->stream));
+    REF_RC(la.product(d_psi, n, d_yt, n, nc, Y, false));
+    REF_HIP(hipStreamSynchronize(c->stream));
+    REF_HIP(hipMemcpyAsync(d_yt, yb.data(), yb.size() * 8, hipMemcpyHostToDevice, c->stream));
+    REF_RC(la.product(X, k2, d_yt, k2, nc, Y, true));
+  }
+  REF_RC(normalize_dev_impl(c, Y, nc));
+  std::vector<double> ev2(nc), sg2(nc);
+  REF_RC(energies_dev_impl(c, Y, nc, ev2.data(), sg2.data()));
+  long nrep = 0;
+  for (long q2 = 0; q2 < nc; ++q2) {
+    const long i = cand_i[q2];
+    if (std::isfinite(sg2[q2]) && sg2[q2] < sigma[i]) {
+      REF_HIP(hipMemcpyAsync(d_psi + i * sl, Y + q2 * sl, sl * 8, hipMemcpyDeviceToDevice, c->stream));
+      eval_out[i] = ev2[q2];
+      sigma_out[i] = sg2[q2];
+      ++nrep;
+    }
+  }
+#undef REF_HIP
+#undef REF_RC
+  *n_replaced = nrep;
+  cleanup();
   return FBSE_OK;
 }
+
+extern "C" int fbse_refine_dev(fbse_ctx *c, double *d_psi, long n_ritz, const double *eval, const double *sigma,
+                               double sigma_max, double e_lo, double e_hi, double e_p, double *eval_out,
+                               double *sigma_out, long *n_replaced) {
+  if (!c || !d_psi || !eval || !sigma || !eval_out || !sigma_out || !n_replaced || n_ritz < 1) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipSetDevice(c->device));
+  if (e_p <= 0.0) e_p = 1.0;
+  return refine_impl(c, d_psi, n_ritz, eval, sigma, sigma_max, e_lo, e_hi, e_p, eval_out, sigma_out, n_replaced);
+}
 extern "C" int fbse_refine(fbse_ctx *c, double *psitot, long n_ritz, const double *eval, const double *sigma,
-                           double sigma_max, double e_lo, double e_hi, long capacity, double e_p, double *eval_out,
-                           double *sigma_out, long *n_out) {
-  if (!c || !psitot) return FBSE_ERR_ARG;
+                           double sigma_max, double e_lo, double e_hi, double e_p, double *eval_out,
+                           double *sigma_out, long *n_replaced) {
+  if (!c || !psitot || !eval || !sigma || !eval_out || !sigma_out || !n_replaced || n_ritz < 1) return FBSE_ERR_ARG;
   FBSE_HIP(c, hipSetDevice(c->device));
   const long sl = stlen_of(c);
-  double *d = nullptr;
-  FBSE_HIP(c, hipMalloc(&d, (size_t)capacity * sl * 8));
-  int rc = FBSE_OK;
-  hipError_t e = hipMemcpyAsync(d, psitot, (size_t)n_ritz * sl * 8, hipMemcpyHostToDevice, c->stream);
-  if (e != hipSuccess) rc = set_err(c, FBSE_ERR_HIP, "copy: %s", hipGetErrorString(e));
-  if (!rc)
-    rc = fbse_refine_dev(c, d, n_ritz, eval, sigma, sigma_max, e_lo, e_hi, capacity, e_p, eval_out, sigma_out, n_out);
-  if (!rc) {
-    e = hipMemcpyAsync(psitot, d, (size_t)(*n_out) * sl * 8, hipMemcpyDeviceToHost, c->stream);
-    if (e != hipSuccess) rc = set_err(c, FBSE_ERR_HIP, "copy: %s", hipGetErrorString(e));
-  }
-  hipStreamSynchronize(c->stream);
-  hipFree(d);
-  return rc;
+  return with_device_copy(c, psitot, n_ritz, [&](double *d) -> int {
+    int rc = fbse_refine_dev(c, d, n_ritz, eval, sigma, sigma_max, e_lo, e_hi, e_p, eval_out, sigma_out, n_replaced);
+    if (rc) return rc;
+    FBSE_HIP(c, hipMemcpyAsync(psitot, d, (size_t)n_ritz * sl * 8, hipMemcpyDeviceToHost, c->stream));
+    FBSE_HIP(c, hipStreamSynchronize(c->stream));
+    return FBSE_OK;
+  });
 }
 
 // ---------------------------------------------------------------------------

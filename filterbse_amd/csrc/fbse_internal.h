@@ -163,7 +163,6 @@ struct fbse_ctx {
   void *d_pjd = nullptr;        // mixed projection coefficients (column x atom x 6), grown on demand
   size_t pjd_bytes = 0;
   int fuse_proj = 1;            // expansion inside the plane passes when the tables allow it
-  int refine_full = 0;          // fbse_set_refine_basis
 
   // scratch (grown on demand)
   void *d_scratch = nullptr;
@@ -330,6 +329,19 @@ int state_scale(fbse_ctx *c, double *psi, long nstates, long elems_per_state, co
 int axpby_states(fbse_ctx *c, const double *x, double *y, long n, double a, double b);  // y = a*x + b*y
 // y_s -= theta_s * x_s for nstates states of elems doubles (theta on the device)
 int states_sub_scaled(fbse_ctx *c, const double *x, double *y, long nstates, long elems, const double *d_theta);
+
+// FP64 matrix-core products (gemm_kernels.hip).  Columns of m elements
+// (complex double2 or real double), column-major.
+// Gram: part[p*pstride + i + j*ldp] = sum over the rows of slab p of
+// conj(A[r, i]) B[r, j]; upper (n1 == n2): only the 64 x 64 tiles on or above
+// the diagonal are formed.  Returns 0 or -1 (launch error / shape limits).
+long gram_tile_count(long n1, long n2, bool upper);
+int launch_gram_mfma(hipStream_t s, bool cplx, const double *A, long lda, const double *B, long ldb, long m, long n1,
+                     long n2, bool upper, long slab, long parts, double *part, long ldp, long pstride);
+// Rotation: Out[r, j] (+)= sum_k A[r, k] V[k, j], r < nrows, k < nin, j < nout
+// (accumulate: added to Out).
+int launch_rot_mfma(hipStream_t s, bool cplx, const double *A, long lda, long nrows, long nin, const double *V,
+                    long ldv, long nout, double *Out, long ldo, bool accumulate = false);
 
 // window accumulation (ham_kernels.hip)
 // jlo_dev != nullptr: the window starts at step *jlo_dev (read on the device,

@@ -144,25 +144,35 @@ int fbse_rayleigh_ritz_dev(fbse_ctx *ctx, double *d_psitot, long n, double *eval
 int fbse_sigma(fbse_ctx *ctx, const double *psitot, long n, double *sigma_E, double *eval_aux);
 int fbse_sigma_dev(fbse_ctx *ctx, const double *d_psitot, long n, double *sigma_E, double *eval_aux);
 
-/* One FP64 residual-refinement sweep after a mixed-precision filter (no
- * counterpart in the reference, whose filter is FP64 throughout).  The k Ritz
- * pairs of a preceding fbse_rayleigh_ritz with eval in [e_lo, e_hi] (and, when
- * sigma is not NULL, sigma[i] < sigma_max) get their residuals r = H u - e u
- * preconditioned in k-space (r / (1 + 0.5|k|^2 / e_p)).  By default the
- * selected pairs are compacted to the front and the 2k vectors [u, P r] are
- * orthogonalised and diagonalised again in FP64 (k limited to capacity / 2);
- * after fbse_set_refine_basis(ctx, 1), when n_ritz + k <= capacity, the k
- * corrections are appended after all n_ritz Ritz vectors and the n_ritz + k
- * vectors are used instead.  psitot must hold capacity
- * states.  On return psitot holds *n_out Ritz vectors with eval_out /
- * sigma_out (capacity-long host arrays) as from fbse_rayleigh_ritz and
- * fbse_sigma.  e_p <= 0 selects 1 Ha. */
+/* Subspace products on the FP64 matrix cores (device state blocks, column i
+ * of a block at d_a + i*stlen).  fbse_overlap_dev: s[i + j*n1] (host, n1 x n2,
+ * complex interleaved for complex states) = dv * sum conj(a_i) b_j, the
+ * quantity dotp / dotpreal return (Hmat.c:301-338) and diag_H assembles into
+ * H_ij (Hmat.c:141-164).  fbse_rotate_dev: in place, column j < nout of the
+ * block becomes sum_k a_k v[k + j*nin] (host v, nin x nout, nout <= nin), the
+ * change of basis of diag_H (Hmat.c:230-281). */
+int fbse_overlap_dev(fbse_ctx *ctx, const double *d_a, long n1, const double *d_b, long n2, double *s);
+int fbse_rotate_dev(fbse_ctx *ctx, double *d_a, long nin, const double *v, long nout);
+
+/* One guarded FP64 refinement sweep after a mixed-precision filter (no
+ * counterpart in the reference, whose filter is FP64 throughout).  Input: the
+ * n_ritz orthonormal Ritz vectors of a preceding fbse_rayleigh_ritz with their
+ * eval and sigma (fbse_sigma).  The pairs with eval in [e_lo, e_hi] and
+ * sigma < sigma_max (lowest sigma first, as many as the device memory holds
+ * two blocks of) get residuals r = H u - e u, preconditioned in k-space
+ * (r / (1 + 0.5|k|^2 / e_p); e_p <= 0 selects 1 Ha), orthogonalised against
+ * all n_ritz Ritz vectors; Rayleigh-Ritz over [Ritz vectors, corrections]
+ * then gives new pairs, and a new pair replaces the Ritz pair it overlaps by
+ * more than one half only when its residual is lower.  On return psitot holds
+ * n_ritz pairs in the same slots, eval_out / sigma_out (n_ritz-long host
+ * arrays) their values, *n_replaced how many changed: no pair is dropped and
+ * none gets a larger residual. */
 int fbse_refine(fbse_ctx *ctx, double *psitot, long n_ritz, const double *eval, const double *sigma,
-                double sigma_max, double e_lo, double e_hi, long capacity, double e_p, double *eval_out,
-                double *sigma_out, long *n_out);
+                double sigma_max, double e_lo, double e_hi, double e_p, double *eval_out, double *sigma_out,
+                long *n_replaced);
 int fbse_refine_dev(fbse_ctx *ctx, double *d_psitot, long n_ritz, const double *eval, const double *sigma,
-                    double sigma_max, double e_lo, double e_hi, long capacity, double e_p, double *eval_out,
-                    double *sigma_out, long *n_out);
+                    double sigma_max, double e_lo, double e_hi, double e_p, double *eval_out, double *sigma_out,
+                    long *n_replaced);
 
 /* get_energy_range iterative branch (energy.c:222-300) and range inflation
  * (energy.c:343-369).  psi0_min / psi0_max are the two host-drawn start vectors
@@ -215,9 +225,6 @@ int fbse_set_slab_split(fbse_ctx *ctx, int split);
  * window; 0 launches every kernel from the host instead.  Results are
  * identical.  No counterpart in the reference. */
 int fbse_set_graph(fbse_ctx *ctx, int enable);
-/* Refinement basis of fbse_refine: 0 = the selected pairs and their
- * corrections (default), 1 = every Ritz vector plus the corrections. */
-int fbse_set_refine_basis(fbse_ctx *ctx, int full);
 
 /* Releases the context's work buffers (the filter's ring of Newton terms,
  * FFT work space); they are allocated again on demand.  For callers that keep

@@ -892,3 +892,120 @@ void or_get_energy_range(const or_ham *h, long max_iter, double nl_lambda_min, d
   free(psi);
   free(phi);
 }
+
+/* ------------------------------------------------------------------ */
+/* Threaded Newton step, for the CPU baseline of the benchmark only     */
+/* (tests pin it against the serial functions above).  The reference    */
+/* threads p_potential with OpenMP over atoms and adds each atom's      */
+/* projector expansion with atomics (phamiltonian.c:288-303,            */
+/* 420-435); the same is done here.  The kinetic FFT pair is left to    */
+/* the caller (scipy.fft with worker threads in cpu_newton.py).         */
+/* ------------------------------------------------------------------ */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* SO + NL + V_loc (p_potential, phamiltonian.c:65-137): psi_out holds the
+ * kinetic part T psi on entry and H psi on exit; psi_tmp is psi. */
+void or_potential_omp(or_cplx *psi_out, const or_cplx *psi_tmp, const or_ham *h, int nthreads)
+{
+  int spin_arr[6], m_arr[6];
+  for (int s = 0; s < 2; s++)
+    for (int m = 0; m < 3; m++) { spin_arr[s * 3 + m] = s; m_arr[s * 3 + m] = m; }
+  if (h->so == 1) {
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+    for (long jat = 0; jat < h->n_atoms; jat++) {
+      long off = jat * h->n_gridpts, ng = h->n_gpt[jat];
+      for (int ip = 0; ip < h->n_so_proj; ip++)
+        for (int jp = 0; jp < 6; jp++) {
+          int sp = spin_arr[jp], mp = m_arr[jp];
+          double pre = 0.0, pim = 0.0;
+          for (long g = 0; g < ng; g++) {
+            long ri = off + g, r = h->jxyz[ri] + h->ngrid * sp;
+            double prj = h->proj[ri * 5 + ip];
+            double yre = h->y1[ri * 6 + 2 * mp], yim = h->y1[ri * 6 + 2 * mp + 1];
+            pre += prj * (psi_tmp[r].re * yre + psi_tmp[r].im * yim);
+            pim += prj * (psi_tmp[r].im * yre - psi_tmp[r].re * yim);
+          }
+          pre *= h->dv;
+          pim *= h->dv;
+          for (int j = 0; j < 6; j++) {
+            int jt = jp * 6 + j;
+            double lre = h->LS[2 * jt], lim = h->LS[2 * jt + 1];
+            if (lre == 0.0 && lim == 0.0) continue;
+            int s = spin_arr[j], m = m_arr[j];
+            double plre = lre * pre - lim * pim, plim = lre * pim + lim * pre;
+            for (long g = 0; g < ng; g++) {
+              long ri = off + g, rp = h->jxyz[ri] + h->ngrid * s;
+              double prj = h->proj[ri * 5 + ip];
+              double yre = h->y1[ri * 6 + 2 * m], yim = h->y1[ri * 6 + 2 * m + 1];
+              double vre = prj * (yre * plre - yim * plim), vim = prj * (yre * plim + yim * plre);
+#pragma omp atomic
+              psi_out[rp].re += vre;
+#pragma omp atomic
+              psi_out[rp].im += vim;
+            }
+          }
+        }
+    }
+  }
+  if (h->nl == 1) {
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+    for (long jat = 0; jat < h->n_atoms; jat++) {
+      long off = jat * h->n_gridpts, ng = h->n_gpt[jat];
+      for (int ip = 0; ip < h->n_nl_proj; ip++)
+        for (long spin = 0; spin < h->nspin; spin++)
+          for (int m = 0; m < 3; m++) {
+            double pre = 0.0, pim = 0.0;
+            for (long g = 0; g < ng; g++) {
+              long ri = off + g, r = h->jxyz[ri] + h->ngrid * spin;
+              double nlp = h->nl_proj[ri * 5 + ip];
+              double yre = h->y1[ri * 6 + 2 * m], yim = h->y1[ri * 6 + 2 * m + 1];
+              pre += nlp * (psi_tmp[r].re * yre + psi_tmp[r].im * yim);
+              pim += nlp * (psi_tmp[r].im * yre - psi_tmp[r].re * yim);
+            }
+            int sgn = h->nl_sign[jat * 5 + ip];
+            pre *= sgn * h->dv;
+            pim *= sgn * h->dv;
+            for (long g = 0; g < ng; g++) {
+              long ri = off + g, rp = h->jxyz[ri] + h->ngrid * spin;
+              double nlp = h->nl_proj[ri * 5 + ip];
+              double yre = h->y1[ri * 6 + 2 * m], yim = h->y1[ri * 6 + 2 * m + 1];
+              double vre = nlp * (yre * pre - yim * pim), vim = nlp * (yre * pim + yim * pre);
+#pragma omp atomic
+              psi_out[rp].re += vre;
+#pragma omp atomic
+              psi_out[rp].im += vim;
+            }
+          }
+    }
+  }
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+  for (long j = 0; j < h->nspinngrid; j++) {
+    psi_out[j].re += (h->pot_local[j] * psi_tmp[j].re);
+    psi_out[j].im += (h->pot_local[j] * psi_tmp[j].im);
+  }
+}
+
+/* One Newton step after the H application: p_hnorm's combine
+ * (filter.c:402-407), psi <- dE_1 hpsi - (2 + zn + Vmin dE_1) psi, then the
+ * accumulation into every target (filter.c:318-359):
+ * block[jms*stlen ...] += an[jms, jc] psi.  Complex layout (stlen = 2 nsg). */
+void or_hnorm_acc_omp(or_cplx *psi, const or_cplx *hpsi, double zn, double Vmin, double dE_1, double *block,
+                      long ms, long nsg, const double *an, long ncheby, long jc, int nthreads)
+{
+  const double c0 = 2.0 + zn + Vmin * dE_1;
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+  for (long j = 0; j < nsg; j++) {
+    double re = dE_1 * hpsi[j].re - c0 * psi[j].re;
+    double im = dE_1 * hpsi[j].im - c0 * psi[j].im;
+    psi[j].re = re;
+    psi[j].im = im;
+    for (long jms = 0; jms < ms; jms++) {
+      const double *a = &an[2 * (ncheby * jms + jc)];
+      double *b = &block[jms * 2 * nsg + 2 * j];
+      b[0] += (a[0] * re - a[1] * im);
+      b[1] += (a[0] * im + a[1] * re);
+    }
+  }
+}

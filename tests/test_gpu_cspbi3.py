@@ -65,9 +65,11 @@ def run_path(J, precision, refine=False):
     sig, ev2 = ctx.sigma_dev(ptr, rank)
     if refine:
         sig_f = (1.0 / (2.0 * dt)) ** 0.5
-        ev, sig = ctx.refine_dev(ptr, rank, ev, targets.min() - 4 * sig_f, targets.max() + 4 * sig_f, ns * ms,
-                                 sigma=sig, sigma_max=1e-2)
-        rank = len(ev)
+        for _ in range(2):
+            ev2, sig2, _ = ctx.refine_dev(ptr, rank, ev, sig, targets.min() - 4 * sig_f, targets.max() + 4 * sig_f,
+                                          e_p=0.25, sigma_max=1e-2)
+            assert (sig2 <= sig).all()
+            ev, sig = ev2, sig2
     print("precision", precision, "refine", refine, "n", rank, "converged(1e-6)", int((sig < 1e-6).sum()),
           "converged(1e-4)", int((sig < 1e-4).sum()), flush=True)
     return ev, sig, rank

@@ -85,17 +85,18 @@ def test_fixed_axis_instances(grid):
     from test_gpu_parity import filter_inputs
     psi_rank, stlen, an, zn, Vmin, dE_1 = filter_inputs(s, 2, 2, 12)
     fref = g.filter(psi_rank, 2, 2, an, zn, Vmin, dE_1, precision=1)
-    # refinement: residuals, the k-space preconditioner pass (z lines), ortho, RR
+    # refinement: residuals, the k-space preconditioner pass (z lines), Grams, RR
     u, _, _ = g.ortho(random_states(s, 4, seed=10), 4)
-    ev0 = np.linspace(-0.5, 0.5, 4)
-    _, rref, _ = g.refine(u, 4, ev0, -1.0, 1.0, 8)
+    u, ev0 = g.rayleigh_ritz(u, 4)
+    sg0, _ = g.sigma(u, 4)
+    _, _, rref, _ = g.refine(u, 4, ev0, sg0, -1e3, 1e3, sigma_max=1e3)
     for lines in (1016, 1032, 1064):  # 1064: FP32 fixed instance, FP64 generic 64-line tiles
         g.set_fft_kernel(0, lines)
         out = g.apply_h(psi)
         assert rel(out, ref) < 1e-12, lines
         fout = g.filter(psi_rank, 2, 2, an, zn, Vmin, dE_1, precision=1)
         assert rel(fout, fref) < 1e-5, lines
-        _, r, _ = g.refine(u, 4, ev0, -1.0, 1.0, 8)
+        _, _, r, _ = g.refine(u, 4, ev0, sg0, -1e3, 1e3, sigma_max=1e3)
         assert np.abs(r - rref).max() < 1e-10 * np.abs(rref).max(), lines
     g.set_fft_kernel(-1, 0)
     g.close()
