@@ -1,13 +1,24 @@
 """Benchmark of the filter-diagonalization path on one MI355X.
 
-One step = one full pass of the path over the workload with the start states
-already in HBM: Newton filter of all start states at all energy targets,
-normalize_all, orthogonalisation (rank cut 1e-10), Rayleigh-Ritz, and sigma_E
-for every Ritz pair.  Metric: eigenpairs with sigma_E < 1e-4 Ha per second of
-that pass (headline: mixed precision), next to the plain FP64 pass, Hpsi/s of
-the filter and the HBM roofline of the dominant kernel.
+Default workload: the full-stack configuration, a 4.5 nm spin-orbit
+zincblende dot on a 256^3 grid, 16 ran2 start states x 16 energy targets,
+Newton length 2048.  Two passes of the path run with the start states in HBM:
 
-    python bench.py [--steps K] [--warmup W] [--config c1_so_96]
+* mixed + warm: the learned warm start (network trained beforehand, untimed,
+  on small dots) shapes the start states, the Newton filter runs in FP32
+  complex with FP64 coefficients and accumulators, then normalize_all, FP64
+  orthogonalisation (rank cut 1e-10), Rayleigh-Ritz, sigma_E and one FP64
+  refinement sweep;
+* plain FP64: ran2 start states, FP64 filter, normalize_all, ortho, RR, sigma_E.
+
+value = eigenpairs with sigma_E < 1e-4 Ha per second of the mixed + warm pass;
+the FP64 pass and the eigenvalue error of every FP64-converged state against
+the mixed pass are reported beside it.  A step (--steps / --warmup) is one
+Newton step of the mixed filter over the whole batch (H applied to every start
+state, the accumulation into every target): the timed steps give ms_per_step
+and H psi / s; the full passes give value.
+
+    python bench.py [--steps K] [--warmup W] [--config c4_so_256]
 """
 import argparse
 import json
@@ -30,6 +41,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def filter_guard(targets, dE, M):
+    """The reference's filter-width check (coeff.c:30-56): the Gaussian filter
+    of width sigma_f = 1/sqrt(2 dt), dt = (M / (2.5 dE))^2 (mod_filter.c:114),
+    must not exceed half the smallest spacing of consecutive targets."""
+    dt = (M / (2.5 * dE)) ** 2
+    sig_f = math.sqrt(1.0 / (2.0 * dt))
+    d = np.abs(np.diff(targets))
+    min_sp = float(d[d > 0].min()) if np.any(d > 0) else float("inf")
+    return dict(sigma_f=round(sig_f, 6), min_target_spacing=round(min_sp, 6), dE=round(float(dE), 6), M=int(M),
+                ok=bool(sig_f <= 0.5 * min_sp)), dt
+
+
 def setup(cfg_name, dev, ncheby=0):
     import torch
     from filterbse_amd import api, dots
@@ -39,9 +62,10 @@ def setup(cfg_name, dev, ncheby=0):
         cfg["ncheby"] = ncheby
     t0 = time.perf_counter()
     job = dots.build_dot(cfg["radius_nm"], cfg["n_req"], cfg["dx"], so=cfg["so"], nl=cfg["nl"],
-                         builder=cfg.get("builder", "host"))
-    log("dot: %d core atoms, %d ligands, grid %dx%dx%d, %.1fs" % (
-        job["n_core"], len(job["syms"]) - job["n_core"], job["nx"], job["ny"], job["nz"], time.perf_counter() - t0))
+                         ke_max=cfg.get("ke_max", 20.0), builder=cfg.get("builder", "host"))
+    log("dot: %d core atoms, %d ligands, grid %dx%dx%d, KEmax %.1f, %.1fs" % (
+        job["n_core"], len(job["syms"]) - job["n_core"], job["nx"], job["ny"], job["nz"], cfg.get("ke_max", 20.0),
+        time.perf_counter() - t0))
     ctx = api.Context(job["nx"], job["ny"], job["nz"], job["nspin"], job["is_complex"], job["dv"], job["ksqr"],
                       job["pot_local"], job["proj"])
     t0 = time.perf_counter()
@@ -51,9 +75,8 @@ def setup(cfg_name, dev, ncheby=0):
     t_range = time.perf_counter() - t0
     Emin_raw, Emax_raw, Vmin, dE = rng_out[:4]
     M, ms, n_starts = cfg["ncheby"], cfg["ms"], cfg["n_starts"]
-    dt = (M / (2.5 * dE)) ** 2  # mod_filter.c:114
-    sig_f = math.sqrt(1.0 / (2.0 * dt))
     targets = api.ene_targets(*cfg["targets"])
+    guard, dt = filter_guard(targets, dE, M)
     zn = api.leja_points(M)
     an = api.newton_coeff(zn, targets, Vmin, dE, dt)
     stlen = (job["is_complex"] + 1) * job["nspin"] * job["nx"] * job["ny"] * job["nz"]
@@ -62,61 +85,72 @@ def setup(cfg_name, dev, ncheby=0):
     starts_dev = torch.from_numpy(starts).to(dev).view(n_starts, stlen)
     del starts
     psi_rank = torch.empty(n_starts * ms * stlen, dtype=torch.float64, device=dev)
-    log("energy range [%.6g, %.6g] (%d + %d iterations, %.2fs); Vmin %.6g dE %.6g; sigma_filter %.4g" % (
-        Emin_raw, Emax_raw, rng_out[4], rng_out[5], t_range, Vmin, dE, sig_f))
+    log("energy range [%.6g, %.6g] (%d + %d iterations, %.2fs); Vmin %.6g dE %.6g; filter guard %s" % (
+        Emin_raw, Emax_raw, rng_out[4], rng_out[5], t_range, Vmin, dE, guard))
     return dict(cfg=cfg, job=job, ctx=ctx, M=M, ms=ms, n_starts=n_starts, stlen=stlen, an=an, zn=zn, Vmin=Vmin,
                 dE=dE, targets=targets, starts_dev=starts_dev, psi_rank=psi_rank, range=rng_out, t_range=t_range,
-                sig_f=sig_f)
+                sig_f=guard["sigma_f"], guard=guard)
 
 
-def one_pass(S, precision, stage_times=None, refine=False, profile=False):
-    """One pass of the path.  profile: per-kernel HIP events (on the library's
-    stream) of this pass's filter are left in S["prof"]."""
+def warm_start_network(S):
+    """Trains the warm-start network (untimed set-up: small training dots,
+    plain FP64 filters, Adam on the GPU) and keeps the dot's V_loc on the device."""
     import torch
+    from filterbse_amd import warmstart as WS
+    t0 = time.perf_counter()
+    mlp, info = WS.train_network(log=log)
+    info["train_s"] = round(time.perf_counter() - t0, 2)
+    S["mlp"], S["mlp_info"] = mlp, info
+    S["d_vloc"] = torch.from_numpy(S["job"]["pot_local"]).to(S["psi_rank"].device)
+    log("warm-start network trained: %s" % info)
+
+
+def one_pass(S, precision, warm=False, refine=0, profile=False):
+    """One pass of the path from the ran2 start states (warm: reshaped by the
+    warm-start network inside the pass).  Returns (eval, sigma, rank, stage
+    seconds [start states, filter, normalize+ortho, RR, sigma, refine])."""
+    import torch
+    from filterbse_amd import warmstart as WS
     ctx = S["ctx"]
     n = S["n_starts"] * S["ms"]
-    S["psi_rank"].view(S["n_starts"], S["ms"], S["stlen"])[:, 0, :].copy_(S["starts_dev"])
+    blk = S["psi_rank"].view(S["n_starts"], S["ms"], S["stlen"])
     torch.cuda.synchronize()
+    t = [time.perf_counter()]
+    blk[:, 0, :].copy_(S["starts_dev"])
+    if warm:
+        WS.warm_starts(ctx, S["mlp"], S["job"], S["d_vloc"].data_ptr(), blk[:, 0, :], ms=S["ms"])
+    torch.cuda.synchronize()
+    t.append(time.perf_counter())
     ptr = S["psi_rank"].data_ptr()
     free_gb = torch.cuda.mem_get_info()[0] / 1e9
-    held_gb = [round(b / 1e9, 1) for b in ctx.held_bytes()]
-    t = [time.perf_counter()]
     if profile:
         ctx.profile(True)
     ctx.filter_dev(ptr, S["n_starts"], S["ms"], S["an"], S["zn"], S["Vmin"], 4.0 / S["dE"], precision)
-    # accumulation window depth the filter took, and the HBM free before it
-    S.setdefault("windows", []).append(dict(precision=int(precision), profiled=bool(profile), K=ctx.last_window(),
-                                            free_gb=round(free_gb, 1), held_gb=held_gb))
     if profile:
         S["prof"] = {k: ctx.profile_read(k) for k in range(5)}
         ctx.profile(False)
+    S.setdefault("windows", []).append(dict(precision=int(precision), K=ctx.last_window(), free_gb=round(free_gb, 1)))
     t.append(time.perf_counter())
-    mem = lambda: round(torch.cuda.mem_get_info()[0] / 1e9, 1)  # noqa: E731
-    free_after = dict(filter=mem())
     ctx.normalize_all_dev(ptr, n)
     rank, sv = ctx.ortho_dev(ptr, n)
     t.append(time.perf_counter())
-    free_after["ortho"] = mem()
     ev = ctx.rayleigh_ritz_dev(ptr, rank)
     t.append(time.perf_counter())
-    free_after["rr"] = mem()
-    sig, ev2 = ctx.sigma_dev(ptr, rank)
-    free_after["sigma"] = mem()
-    if refine:
-        # one FP64 residual-refinement sweep over the Ritz pairs near the
-        # targets that are within two decades of the threshold
-        w = 4.0 * S["sig_f"]
-        ev_r, sig_r = ctx.refine_dev(ptr, rank, ev, S["targets"].min() - w, S["targets"].max() + w, n,
-                                     sigma=sig, sigma_max=100.0 * RESID)
-        if len(ev_r):
-            ev, sig, rank = ev_r, sig_r, len(ev_r)
-        free_after["refine"] = mem()
+    sig, _ = ctx.sigma_dev(ptr, rank)
     t.append(time.perf_counter())
-    # HBM free after each stage (GB): what the next pass's filter can use
-    S["windows"][-1]["free_after_gb"] = free_after
-    if stage_times is not None:
-        stage_times.append(np.diff(t))
-    return ev, sig, rank
+    S["refine_log"] = []
+    for k in range(refine):
+        # FP64 refinement over the Ritz pairs within 4 filter widths of the targets
+        w = 4.0 * S["sig_f"]
+        ev2, sig2, nrep = ctx.refine_dev(ptr, rank, ev, sig, S["targets"].min() - w, S["targets"].max() + w,
+                                         e_p=S["cfg"].get("refine_ep", 0.25), sigma_max=1e-2)
+        S["refine_log"].append(dict(replaced=int(nrep), converged_before=int((sig < RESID).sum()),
+                                    converged_after=int((sig2 < RESID).sum()),
+                                    best_before=float(sig.min()), best_after=float(sig2.min())))
+        ev, sig = ev2, sig2
+    torch.cuda.synchronize()
+    t.append(time.perf_counter())
+    return ev, sig, rank, np.diff(t)
 
 
 def converged(ev, sig):
@@ -124,32 +158,47 @@ def converged(ev, sig):
     return ev[ok], sig[ok]
 
 
-def match_error(ev_a, ev_b):
-    """Largest distance from each converged eigenvalue of a to the nearest of b."""
-    if len(ev_a) == 0 or len(ev_b) == 0:
-        return None
-    return float(max(np.min(np.abs(ev_b - x)) for x in ev_a))
+def per_state_errors(ev64, sig64, ev_m, sig_m):
+    """For every FP64-converged state: its eigenvalue, the nearest eigenvalue
+    of the mixed pass, |difference| and both residuals."""
+    out = []
+    for i in np.argsort(ev64):
+        if sig64[i] >= RESID:
+            continue
+        j = int(np.argmin(np.abs(ev_m - ev64[i])))
+        out.append(dict(e_fp64=float(ev64[i]), e_mixed=float(ev_m[j]), abs_err=float(abs(ev_m[j] - ev64[i])),
+                        sigma_fp64=float(sig64[i]), sigma_mixed=float(sig_m[j])))
+    return out
 
 
-def cpu_baseline(S, budget_s=15.0):
-    """The oracle's H-psi (a direct restatement of p_hamiltonian, one core) on
-    the same Hamiltonian, over a bounded number of applications."""
-    from oracle import oracle as O
-    job = S["job"]
-    h = O.Ham(job["nx"], job["ny"], job["nz"], job["nspin"], job["is_complex"], job["dv"], job["ksqr"],
-              job["pot_local"], job["proj"])
-    psi = S["starts_dev"][0].cpu().numpy()
-    n = 0
+def newton_steps(S, n, precision):
+    """n Newton steps of the filter over the whole batch (a filter call of
+    Newton length n + 1 on the ran2 start states); returns seconds."""
+    import torch
+    from filterbse_amd import api
+    M = n + 1
+    zn = api.leja_points(M)
+    guard, dt = filter_guard(S["targets"], S["dE"], M)
+    an = api.newton_coeff(zn, S["targets"], S["Vmin"], S["dE"], dt)
+    S["psi_rank"].view(S["n_starts"], S["ms"], S["stlen"])[:, 0, :].copy_(S["starts_dev"])
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    while True:
-        h.apply(psi)
-        n += 1
-        el = time.perf_counter() - t0
-        if el > budget_s or n >= 50:
-            break
-    return dict(value=n / el, unit="Hψ/s", cores=1, kind="port",
-                sample="%d oracle H-psi applications (T+V_loc+SO+NL, one state) on the %s Hamiltonian, %.1f s"
-                       % (n, S["cfg_name"], el))
+    S["ctx"].filter_dev(S["psi_rank"].data_ptr(), S["n_starts"], S["ms"], an, zn, S["Vmin"], 4.0 / S["dE"],
+                        precision)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(S, budget_s=20.0):
+    """The oracle's threaded Newton step (oracle/cpu_newton.py: scipy FFT
+    pair, OpenMP projectors / V_loc / accumulation) on the same workload, one
+    start state, on the host cores this process may use."""
+    from oracle import cpu_newton
+    job = S["job"]
+    start = S["starts_dev"][0].cpu().numpy()
+    out = cpu_newton.baseline(job, S["ms"], S["an"], S["zn"], S["Vmin"], 4.0 / S["dE"], start, budget_s=budget_s)
+    out["sample"] += " (%s)" % S["cfg_name"]
+    return out
 
 
 def measured_traffic(cfg_name, kname):
@@ -175,7 +224,8 @@ def traffic_sample(cfg_name, dev, M):
     from filterbse_amd import api, dots
 
     cfg = dict(dots.CONFIGS[cfg_name])
-    job = dots.build_dot(cfg["radius_nm"], cfg["n_req"], cfg["dx"], so=cfg["so"], nl=cfg["nl"])
+    job = dots.build_dot(cfg["radius_nm"], cfg["n_req"], cfg["dx"], so=cfg["so"], nl=cfg["nl"],
+                         ke_max=cfg.get("ke_max", 20.0), builder=cfg.get("builder", "host"))
     ctx = api.Context(job["nx"], job["ny"], job["nz"], job["nspin"], job["is_complex"], job["dv"], job["ksqr"],
                       job["pot_local"], job["proj"])
     ngrid = job["nx"] * job["ny"] * job["nz"]
@@ -302,15 +352,6 @@ def mlp_sample(args, dev):
     print(json.dumps(line), flush=True)
 
 
-# Batched uneven dots: 32 spin-orbit dots with seeded radii uniform in
-# 1.0-3.0 nm at dx = 1.0 bohr (the reference grid rule needs ~52^3 ... ~120^3;
-# each axis is rounded up to a size with prime factors 2, 3, 5), 32 starts x 8
-# targets each, Newton length 1024, mixed precision; at most 8 dots per
-# fbse_filter_many call (progress is logged between calls).
-C2 = dict(n_dots=32, r_lo=1.0, r_hi=3.0, dx=1.0, n_starts=32, ms=8, ncheby=1024,
-          targets=(4, 4, -0.32, -0.22, -0.14, -0.04), seed=2, group_max=8)
-
-
 def bench_uneven(args, dev):
     """One step = the filters of all dots (fbse_filter_many: every dot on its
     own stream, launches interleaved) plus normalize / ortho / RR / sigma_E of
@@ -319,7 +360,7 @@ def bench_uneven(args, dev):
     import torch
     from filterbse_amd import api, dots
 
-    cfg = C2
+    cfg = dots.UNEVEN
     rng = np.random.default_rng(cfg["seed"])
     radii = rng.uniform(cfg["r_lo"], cfg["r_hi"], cfg["n_dots"])
     D = []
@@ -429,11 +470,13 @@ def bench_uneven(args, dev):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20, help="timed Newton steps of the mixed filter")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed Newton steps before them")
     ap.add_argument("--window", type=int, default=0,
                     help="accumulation window depth K (0 = as deep as 60%% of the free HBM allows)")
-    ap.add_argument("--config", default="c1_so_96")
+    ap.add_argument("--config", default="c4_so_256")
+    ap.add_argument("--no-warm", action="store_true", help="mixed pass from the plain ran2 start states")
+    ap.add_argument("--refine", type=int, default=1, help="FP64 refinement sweeps in the mixed pass")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--traffic-sample", type=int, default=0, metavar="M",
@@ -468,124 +511,144 @@ def main():
         return
     S = setup(args.config, dev, args.ncheby)
     S["cfg_name"] = args.config
+    cfg, job, ctx = S["cfg"], S["job"], S["ctx"]
     if args.window:
-        S["ctx"].set_window(args.window)
+        ctx.set_window(args.window)
     mixed, fp64 = api.FBSE_PREC_MIXED, api.FBSE_PREC_FP64
+    warm = not args.no_warm
+    if warm:
+        warm_start_network(S)
 
-    for _ in range(args.warmup):
-        log("warmup pass (mixed)")
-        one_pass(S, mixed, refine=True)
-    log("timed mixed pass%s" % ("es" if args.steps > 1 else ""))
-    stages = []
-    ctx = S["ctx"]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        ev, sig, rank = one_pass(S, mixed, stages, refine=True)
-    torch.cuda.synchronize()
-    t_mixed = (time.perf_counter() - t0) / args.steps
-    # per-kernel HIP events of one more (untimed) pass: the library then runs
-    # the projection pass on the main stream, so every launch is timed alone
-    log("profiled mixed pass (untimed)")
-    one_pass(S, mixed, None, refine=True, profile=True)
-    prof = S["prof"]
-    fft_path, fft_inst = ctx.fft_choice(mixed)
+    # steps: Newton steps of the mixed filter over the whole batch
+    if args.warmup:
+        log("warmup: %d Newton steps (mixed)" % args.warmup)
+        newton_steps(S, args.warmup, mixed)
+    log("timed: %d Newton steps (mixed)" % args.steps)
+    t_steps = newton_steps(S, args.steps, mixed)
+    ms_per_step = 1e3 * t_steps / args.steps
+
+    # the headline passes
+    log("mixed%s pass" % ("+warm" if warm else ""))
+    ev, sig, rank, st = one_pass(S, mixed, warm=warm, refine=args.refine)
+    t_mixed = float(st.sum())
+    refine_log = S["refine_log"]
     ev_c, sig_c = converged(ev, sig)
-    st = np.mean(stages, axis=0)
-    log("mixed: %.3fs/pass (filter %.3f ortho %.3f RR %.3f refine+sigma %.3f), n %d, converged %d"
-        % (t_mixed, *st, rank, len(ev_c)))
-
+    log("mixed: %.3fs (start states %.3f filter %.3f ortho %.3f RR %.3f sigma %.3f refine %.3f), rank %d, "
+        "converged %d, refine %s" % (t_mixed, *st, rank, len(ev_c), refine_log))
     fp = None
     if not args.no_fp64:
-        fst = []
         log("fp64 pass")
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        ev64, sig64, rank64 = one_pass(S, fp64, fst)
-        torch.cuda.synchronize()
-        t64 = time.perf_counter() - t0
+        ev64, sig64, rank64, st64 = one_pass(S, fp64)
+        t64 = float(st64.sum())
         ev64_c, _ = converged(ev64, sig64)
-        fp = dict(seconds_per_pass=t64, eigenpairs_per_s=len(ev64_c) / t64, converged=int(len(ev64_c)),
-                  rank=int(rank64), stage_s=[round(float(x), 4) for x in fst[0]],
-                  hpsi_per_s=S["n_starts"] * (S["M"] - 1) / fst[0][0],
-                  max_eig_err_mixed_vs_fp64=match_error(ev_c, ev64_c))
-        log("fp64: %.3fs/pass, rank %d, converged %d, max |e_mixed - e_fp64| %s"
-            % (t64, rank64, len(ev64_c), fp["max_eig_err_mixed_vs_fp64"]))
-
-    job = S["job"]
-    ngrid = job["nx"] * job["ny"] * job["nz"]
-    nsg = job["nspin"] * ngrid
-    n_items = S["n_starts"] if job["is_complex"] else (S["n_starts"] + 1) // 2
-    elem = 8  # complex float
-    grid_bytes = n_items * nsg * elem  # one pass over the batch of spinor grids
-    pj = job["proj"]
-    chained = pj is None or ((pj["n_so_proj"] if pj["so"] else 0) <= 1 and (pj["n_nl_proj"] if pj["nl"] else 0) <= 1)
-    if fft_path == 1:
-        e = {1: 12, 2: 16, 3: 32}.get(fft_inst, 12)
-        # instance 4: the fixed-schedule persistent instance of a cubic grid
-        kfmt = ("fft_fx_f32_n%d<%%d>" % job["nx"]) if fft_inst == 4 else ("fft_tile_f32_e%d<%%d>" % e)
-        # dominant kernel: the combine pass.  Algorithmic bytes per launch:
-        # the z-slab output and psi_{j-1} read, psi_j written, V_loc read
-        # once, and (chained) the next step's forward xy transform written
-        launches, ms_tot = prof[1]
-        bytes_per_launch = (4.0 if chained else 3.0) * grid_bytes + job["nspin"] * ngrid * 4
-        kname = kfmt % 2
-        kdesc = ("xy-plane backward FFT, V_loc, Newton combine, projector expansion%s"
-                 % (", next step's forward xy FFT" if chained else ""))
-        zl, zms = prof[0]
-        zslab = dict(kernel=kfmt % 3, desc="xz-slab forward z FFT, k^2, backward z FFT",
-                     avg_launch_ms=round(zms / max(zl, 1), 4), bytes_per_launch=2.0 * grid_bytes,
-                     achieved=round(2.0 * grid_bytes / (zms / max(zl, 1) * 1e-3) / 1e9, 1))
-        zslab["frac"] = round(zslab["achieved"] / HBM_PEAK_GBS, 4)
-        zslab["traffic"] = measured_traffic(args.config, zslab["kernel"])
+        errs = per_state_errors(ev64, sig64, ev, sig)
+        fp = dict(seconds_per_pass=round(t64, 3), eigenpairs_per_s=len(ev64_c) / t64, converged=int(len(ev64_c)),
+                  rank=int(rank64), stage_s=[round(float(x), 4) for x in st64],
+                  hpsi_per_s=round(S["n_starts"] * (S["M"] - 1) / st64[1], 1),
+                  converged_eigenvalues=[round(float(x), 9) for x in np.sort(ev64_c)])
+        fp["max_eig_err_mixed_vs_fp64"] = max([e["abs_err"] for e in errs]) if errs else None
+        log("fp64: %.3fs/pass, rank %d, converged %d, per-state errors %s" % (t64, rank64, len(ev64_c), errs))
     else:
-        launches, ms_tot = prof[0]
-        bytes_per_launch = 2.0 * grid_bytes
-        kname = "per-axis FFT passes (axis_fx_f32_n%d_l* where the axis length has a fixed instance, else fft_axis_kernel<float>)" % job["nx"]
-        kdesc = "x/y forward, z kinetic, y backward passes"
-        zslab = None
-    avg_ms = ms_tot / max(launches, 1)
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=measured_traffic(args.config, kname),
-                    kernel=kname, desc=kdesc, avg_launch_ms=round(avg_ms, 4), bytes_per_launch=bytes_per_launch,
-                    launches=launches, measured="HIP events on the library stream, filter of one profiled pass after the timed ones",
-                    zslab=zslab)
+        errs = []
+
+    # per-kernel HIP events over a short profiled mixed filter (its launches
+    # are those of the full filter; every launch on the library stream)
+    Mp = max(3 * (max(ctx.last_window(), 1) + 1) + 2, 64)
+    Sp = dict(S)
+    Sp["M"] = Mp
+    Sp["zn"] = api.leja_points(Mp)
+    _, dtp = filter_guard(S["targets"], S["dE"], Mp)
+    Sp["an"] = api.newton_coeff(Sp["zn"], S["targets"], S["Vmin"], S["dE"], dtp)
+    blk = S["psi_rank"].view(S["n_starts"], S["ms"], S["stlen"])
+    blk[:, 0, :].copy_(S["starts_dev"])
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    ctx.filter_dev(S["psi_rank"].data_ptr(), S["n_starts"], S["ms"], Sp["an"], Sp["zn"], S["Vmin"], 4.0 / S["dE"],
+                   mixed)
+    prof = {k: ctx.profile_read(k) for k in range(5)}
+    ctx.profile(False)
+    K_prof = ctx.last_window()
+    fft_path, fft_inst = ctx.fft_choice(mixed)
+    steps_p = Mp - 1
+    ngrid = job["nx"] * job["ny"] * job["nz"]
+    s_ = job["nspin"]
+    n_items = S["n_starts"] if job["is_complex"] else (S["n_starts"] + 1) // 2
+    # algorithmic bytes (FP32 complex psi, FP32 V_loc): B_Hpsi per state and
+    # the windowed accumulation per state and step (FP64 accumulators)
+    b_hpsi = 2 * s_ * ngrid * 8 + ngrid * 4
+    b_acc = 2 * S["ms"] * s_ * ngrid * 16 / max(K_prof, 1) + s_ * ngrid * 8
+    t_h = (prof[0][1] + prof[1][1] + prof[2][1]) / steps_p          # ms per Newton step, H psi kernels
+    t_all = prof[4][1] / steps_p
+    ach = b_hpsi * n_items / (t_h * 1e-3) / 1e9
+    whole = (b_hpsi + b_acc) * n_items / (t_all * 1e-3) / 1e9
+    # dominant kernel on its own bytes: one FFT sweep reads and writes the batch
+    l0, m0 = prof[0]
+    own_bytes = 2.0 * n_items * s_ * ngrid * 8
+    own = own_bytes / (m0 / max(l0, 1) * 1e-3) / 1e9
+    if fft_path == 1:
+        kname = ("fft_fx_f32_n%d<*>" % job["nx"]) if fft_inst == 4 else "fft_tile_f32_e*<*>"
+    else:
+        kname = "axis_fx_f32_n%d_l*" % job["nx"] if job["nx"] in (160, 256) else "fft_axis_kernel<float>"
+    roofline = dict(
+        bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
+        traffic=measured_traffic(args.config, "step"),
+        kernel="H psi of one Newton step: %s FFT passes + combine pass + projector passes" % kname,
+        algorithmic="B_Hpsi = 2 s N b_psi + N b_V = %d bytes per state (b_psi 8, b_V 4) x %d states" % (
+            b_hpsi, n_items),
+        avg_ms_per_step=round(t_h, 4), profiled_steps=steps_p,
+        kernel_own=dict(kernel=kname + " (one FFT sweep)", bytes_per_launch=own_bytes,
+                        avg_launch_ms=round(m0 / max(l0, 1), 4), achieved=round(own, 1),
+                        frac=round(own / HBM_PEAK_GBS, 4), traffic=measured_traffic(args.config, "fft_sweep")),
+        whole_filter=dict(bytes_per_step=round((b_hpsi + b_acc) * n_items), window_K=K_prof,
+                          avg_ms_per_step=round(t_all, 4), achieved=round(whole, 1),
+                          frac=round(whole / HBM_PEAK_GBS, 4)),
+        measured="HIP events on the library stream over a %d-step profiled mixed filter after the passes" % Mp)
     kernels = {name: dict(launches=prof[k][0], total_ms=round(prof[k][1], 3))
                for k, name in enumerate(["fft_passes", "x_backward_newton_combine", "projectors",
                                          "window_accumulate", "all"])}
-
     cpu = None
     if not args.no_cpu_baseline:
+        log("cpu baseline")
         cpu = cpu_baseline(S)
-    cfg = S["cfg"]
-    hpsi = S["n_starts"] * (S["M"] - 1) / st[0]
+        log("cpu baseline: %s" % cpu)
+    hpsi_pass = S["n_starts"] * (S["M"] - 1) / st[1]
+    fp64_eps = fp["eigenpairs_per_s"] if fp else None
     line = {
         "metric": METRIC,
-        "value": round(len(ev_c) / t_mixed, 4),
+        "value": round(len(ev_c) / t_mixed, 5),
         "unit": "eigenpairs/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * t_mixed, 2),
+        "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-complex filter, f64 ortho/RR/sigma",
-        "data": "synthetic (seeded zincblende dot, ran2 start states)",
+        "dtype": "f32-complex filter; f64 coefficients, accumulators, ortho/RR/sigma/refine",
+        "data": "synthetic (seeded zincblende dot, ran2 start states%s)" % (", learned warm start" if warm else ""),
         "config": {
-            "workload": "%s: %.1f nm dot, %d atoms + %d ligands, %s, grid %dx%dx%d (dx %.3g bohr), %d starts x %d "
-                        "targets, Newton length %d" % (
+            "workload": "%s: %.1f nm dot, %d atoms + %d ligands, %s, grid %dx%dx%d (dx %.3g bohr), KEmax %.1f Ha, "
+                        "%d starts x %d targets %s, Newton length %d" % (
                             args.config, cfg["radius_nm"], job["n_core"], len(job["syms"]) - job["n_core"],
                             "SO+NL spinors" if cfg["so"] else "real states", job["nx"], job["ny"], job["nz"],
-                            cfg["dx"], S["n_starts"], S["ms"], S["M"]),
+                            cfg["dx"], cfg.get("ke_max", 20.0), S["n_starts"], S["ms"],
+                            "(setTargets %d %d, VB %.4f..%.4f, CB %.4f..%.4f)" % tuple(cfg["targets"]), S["M"]),
             "parallelism": "single GPU; start states batched in one filter",
+            "step": "one Newton step of the mixed filter over the batch (H on every start state + accumulation)",
         },
         "converged": int(len(ev_c)),
         "rank": int(rank),
-        "hpsi_per_s": round(hpsi, 1),
+        "pass_s": round(t_mixed, 3),
         "stage_s": [round(float(x), 4) for x in st],
+        "hpsi_per_s": round(hpsi_pass, 1),
+        "hpsi_per_s_steps": round(S["n_starts"] / (ms_per_step * 1e-3), 1),
+        "converged_eigenvalues": [round(float(x), 9) for x in np.sort(ev_c)],
+        "refine": refine_log,
         "fp64": fp,
+        "mixed_warm_over_fp64": (round((len(ev_c) / t_mixed) / fp64_eps, 3) if fp64_eps else None),
+        "per_state_error_vs_fp64": errs,
+        "filter_guard": S["guard"],
+        "warm_start": S.get("mlp_info"),
         "energy_range_s": round(S["t_range"], 3),
         "roofline": roofline,
         "kernels_ms": kernels,

@@ -1683,19 +1683,49 @@ __global__ void set_diag_kernel(double *M, long ld, long n, const double *d, int
   }
 }
 
-// One guarded FP64 refinement sweep (Davidson step with a kinetic-energy
-// preconditioner) over the Ritz pairs of a preceding Rayleigh-Ritz:
+// G[b] (nt x nt, column-major) = dv (M2 - 2 theta_b M) + theta_b^2 I for the
+// raw Rayleigh-Ritz matrix M = Q^H H Q and M2 = (H Q)^H (H Q) of a dv-orthonormal
+// basis Q: the matrix of ||(H - theta_b) Q y||^2 (dv-weighted) over unit y.
+__global__ void refined_matrix_kernel(const double *M, const double *M2, const double *theta, long nt, long nb,
+                                      double dv, int cplx, double *G) {
+  const long per = nt * nt, tot = per * nb;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < tot; idx += (long)gridDim.x * blockDim.x) {
+    const long b = idx / per, e = idx - b * per;
+    const long i = e % nt, j = e / nt;
+    const double th = theta[b];
+    if (cplx) {
+      const double2 m = ((const double2 *)M)[e], m2 = ((const double2 *)M2)[e];
+      double re = dv * (m2.x - 2.0 * th * m.x), im = dv * (m2.y - 2.0 * th * m.y);
+      if (i == j) {
+        re += th * th;
+        im = 0.0;
+      }
+      ((double2 *)G)[idx] = make_double2(re, im);
+    } else {
+      double v = dv * (M2[e] - 2.0 * th * M[e]);
+      if (i == j) v += th * th;
+      G[idx] = v;
+    }
+  }
+}
+
+// One guarded FP64 refinement sweep over the Ritz pairs of a preceding
+// Rayleigh-Ritz (refined Ritz vectors over a Davidson-extended basis):
 //   1. the k selected pairs (eval in [e_lo, e_hi], sigma < sigma_max, lowest
 //      sigma first; k limited by memory) get residuals r = H u - theta u and
 //      corrections w = P r, P = 1 / (1 + 0.5|k|^2 / e_p) in k-space;
-//   2. the corrections are orthogonalised against all n_ritz Ritz vectors
-//      (twice) and among themselves (rank cut), then H w is formed;
-//   3. Rayleigh-Ritz over [U, W]: U^H H U = diag(theta) (U are Ritz vectors),
-//      U^H H W and W^H H W from the matrix-core Grams;
-//   4. a new pair whose coefficient on the old Ritz vector u_i carries more
-//      than half its weight (so the match is one to one) replaces u_i only
-//      when its own residual is lower than u_i's.
-// Every pair stays where it was; none gets a larger residual.
+//   2. the corrections are orthogonalised against all n Ritz vectors (twice)
+//      and among themselves (rank cut), giving Q = [U, W];
+//   3. A1 = Q^H H Q and A2 = (H Q)^H (H Q) on the matrix-core Grams;
+//   4. for each selected pair (clusters of equal theta together) the unit
+//      vector of span(Q) with the least ||(H - theta) v|| (lowest eigenvectors
+//      of A2 - 2 theta A1 + theta^2); since u itself lies in span(Q), that is
+//      never above the pair's sigma, also where the spectrum is too dense for
+//      interior Rayleigh-Ritz to help;
+//   5. a candidate that would duplicate another Ritz vector or an earlier
+//      candidate (overlap above one half) is dropped, and a pair is replaced
+//      only when its new vector's own residual is lower.
+// Every pair stays in its slot; none gets a larger residual.
 static int refine_impl(fbse_ctx *c, double *d_psi, long n, const double *eval, const double *sigma,
                        double sigma_max, double e_lo, double e_hi, double e_p, double *eval_out, double *sigma_out,
                        long *n_replaced) {
@@ -1721,15 +1751,13 @@ static int refine_impl(fbse_ctx *c, double *d_psi, long n, const double *eval, c
   const long k = std::max(1L, std::min((long)sel.size(), kfit));
   sel.resize(k);
   int rc = FBSE_OK;
-  double *X = nullptr, *Y = nullptr, *d_th = nullptr, *Cm = nullptr, *M = nullptr, *d_w = nullptr;
+  double *X = nullptr, *Y = nullptr, *d_th = nullptr, *Cm = nullptr, *M = nullptr, *d_w = nullptr, *M2 = nullptr;
+  double *T1 = nullptr, *T2 = nullptr, *G = nullptr, *Gw = nullptr, *Ge = nullptr;
+  int *Ginfo = nullptr;
   auto cleanup = [&]() {
     hipStreamSynchronize(c->stream);
-    hipFree(X);
-    hipFree(Y);
-    hipFree(d_th);
-    hipFree(Cm);
-    hipFree(M);
-    hipFree(d_w);
+    for (double *p : {X, Y, d_th, Cm, M, d_w, M2, T1, T2, G, Gw, Ge}) hipFree(p);
+    hipFree(Ginfo);
   };
   if (big_malloc(c, (void **)&X, (size_t)k * sl * 8) != hipSuccess ||
       big_malloc(c, (void **)&Y, (size_t)k * sl * 8) != hipSuccess) {
@@ -1798,10 +1826,14 @@ static int refine_impl(fbse_ctx *c, double *d_psi, long n, const double *eval, c
     REF_HIP(hipMemcpyAsync(Cm, z.data(), z.size() * 8, hipMemcpyHostToDevice, c->stream));
     REF_RC(la.rotate(X, k, Cm, k, k2));
   }
-  // 3. H W, and the Rayleigh-Ritz matrix of [U, W] (order nt = n + k2)
+  // 3. H W; with Q = [U, W] (order nt = n + k2) the matrices A1 = Q^H H Q
+  // (U^H H U = diag(theta): U are Ritz vectors) and A2 = (H Q)^H (H Q), raw
+  // (unweighted) sums: U^H H^2 U and (H U)^H (H W) from H U and H^2 U formed
+  // chunk by chunk, (H W)^H (H W) directly.
   REF_RC(apply_h_dev_impl(c, X, Y, k2));
   const long nt = n + k2;
   REF_HIP(hipMalloc(&M, (size_t)nt * nt * es));
+  REF_HIP(hipMalloc(&M2, (size_t)nt * nt * es));
   REF_RC(fill_zero(c, M, (size_t)nt * nt * es));
   for (long i = 0; i < n; ++i) th[i] = eval[i] / dv;
   REF_HIP(hipMemcpyAsync(d_th, th.data(), n * 8, hipMemcpyHostToDevice, c->stream));
@@ -1811,46 +1843,139 @@ static int refine_impl(fbse_ctx *c, double *d_psi, long n, const double *eval, c
   REF_RC(la.gram(X, k2, Y, k2, M + ((size_t)n * nt + n) * cw, nt));
   hipLaunchKernelGGL(mirror_upper_kernel, dim3(256), dim3(256), 0, c->stream, M, nt, cw - 1);
   REF_HIP(hipGetLastError());
-  hipFree(d_w);
-  d_w = nullptr;
-  REF_HIP(hipMalloc(&d_w, (size_t)nt * 8));
-  REF_RC(la.eigh(M, nt, d_w));
-  std::vector<double> mu(nt), Yv((size_t)nt * nt * cw);
-  REF_HIP(hipMemcpyAsync(mu.data(), d_w, nt * 8, hipMemcpyDeviceToHost, c->stream));
-  REF_HIP(hipMemcpyAsync(Yv.data(), M, Yv.size() * 8, hipMemcpyDeviceToHost, c->stream));
-  REF_HIP(hipStreamSynchronize(c->stream));
-  // 4. one-to-one matches: new pair j whose weight on old Ritz vector i > 1/2
-  std::vector<char> is_sel(n, 0);
-  for (long j = 0; j < k; ++j) is_sel[sel[j]] = 1;
-  std::vector<long> cand_j, cand_i;
-  for (long j = 0; j < nt && (long)cand_j.size() < k; ++j) {
-    long best = -1;
-    double bw = 0.0;
-    for (long i = 0; i < n; ++i) {
-      const double *y = &Yv[((size_t)j * nt + i) * cw];
-      const double w2 = cw == 2 ? y[0] * y[0] + y[1] * y[1] : y[0] * y[0];
-      if (w2 > bw) {
-        bw = w2;
-        best = i;
+  REF_RC(la.gram(Y, k2, Y, k2, M2 + ((size_t)n * nt + n) * cw, nt));
+  {
+    const long cch = std::max(1L, std::min(n, states_that_fit(c, (size_t)2 * sl * 8, 0.7)));
+    if (big_malloc(c, (void **)&T1, (size_t)cch * sl * 8) != hipSuccess ||
+        big_malloc(c, (void **)&T2, (size_t)cch * sl * 8) != hipSuccess) {
+      cleanup();
+      return set_err(c, FBSE_ERR_NOMEM, "refine: two blocks of %ld states for H U", cch);
+    }
+    for (long a0 = 0; a0 < n; a0 += cch) {
+      const long cc = std::min(cch, n - a0);
+      REF_RC(apply_h_dev_impl(c, d_psi + a0 * sl, T1, cc));
+      REF_RC(apply_h_dev_impl(c, T1, T2, cc));
+      REF_RC(la.gram(d_psi, n, T2, cc, M2 + (size_t)a0 * nt * cw, nt));
+      REF_RC(la.gram(T1, cc, Y, k2, M2 + ((size_t)n * nt + a0) * cw, nt));
+    }
+    REF_HIP(hipStreamSynchronize(c->stream));
+    hipFree(T1);
+    hipFree(T2);
+    T1 = T2 = nullptr;
+  }
+  hipLaunchKernelGGL(mirror_upper_kernel, dim3(256), dim3(256), 0, c->stream, M2, nt, cw - 1);
+  REF_HIP(hipGetLastError());
+  // 4. refined Ritz vectors: for the selected pairs, grouped into clusters of
+  // equal theta (Kramers pairs), the s lowest eigenvectors y of
+  // G(theta) = A2 - 2 theta A1 + theta^2 (dv-weighted: the smallest
+  // ||(H - theta) Q y||^2 over unit Q y).  A Ritz vector u_i is itself a unit
+  // vector of span(Q), so the lowest value is at most sigma_i^2.
+  std::vector<long> order(sel.begin(), sel.end());
+  std::sort(order.begin(), order.end(), [&](long a, long b) { return eval[a] < eval[b]; });
+  // clusters: consecutive values closer than the larger of the two residuals
+  // (at most 1e-3 Ha): the Ritz values of one degenerate level, split by the
+  // noise of the mixed-precision filter, refine together, so that each gets
+  // its own vector of the level instead of a copy of its neighbour's
+  std::vector<std::vector<long>> clusters;
+  for (long i : order) {
+    if (!clusters.empty()) {
+      const long j = clusters.back().back();
+      const double tol = std::max(1e-9 * std::max(1.0, fabs(eval[i])), std::min(std::max(sigma[i], sigma[j]), 1e-3));
+      if (fabs(eval[i] - eval[j]) <= tol) {
+        clusters.back().push_back(i);
+        continue;
       }
     }
-    if (best >= 0 && bw > 0.5 && is_sel[best]) {
-      cand_j.push_back(j);
-      cand_i.push_back(best);
+    clusters.push_back({i});
+  }
+  const long nb = (long)clusters.size();
+  std::vector<double> cth(nb);
+  for (long b = 0; b < nb; ++b) {
+    double sum = 0.0;
+    for (long i : clusters[b]) sum += eval[i];
+    cth[b] = sum / (double)clusters[b].size();
+    std::sort(clusters[b].begin(), clusters[b].end(), [&](long a, long bb) { return sigma[a] < sigma[bb]; });
+  }
+  REF_HIP(hipMalloc(&G, (size_t)nb * nt * nt * es));
+  REF_HIP(hipMalloc(&Gw, (size_t)nb * nt * 8));
+  REF_HIP(hipMalloc(&Ge, (size_t)nb * nt * 8));
+  REF_HIP(hipMalloc(&Ginfo, (size_t)nb * sizeof(int)));
+  REF_HIP(hipMemcpyAsync(d_th, cth.data(), nb * 8, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(refined_matrix_kernel, dim3(256), dim3(256), 0, c->stream, (const double *)M, (const double *)M2,
+                     (const double *)d_th, nt, nb, dv, cw - 1, G);
+  REF_HIP(hipGetLastError());
+  {
+    rocblas_status st;
+    if (cw == 2)
+      st = rocsolver_zheevd_strided_batched(c->blas, rocblas_evect_original, rocblas_fill_upper, (int)nt,
+                                            (rocblas_double_complex *)G, (int)nt, (rocblas_stride)(nt * nt), Gw,
+                                            (rocblas_stride)nt, Ge, (rocblas_stride)nt, Ginfo, (int)nb);
+    else
+      st = rocsolver_dsyevd_strided_batched(c->blas, rocblas_evect_original, rocblas_fill_upper, (int)nt, G, (int)nt,
+                                            (rocblas_stride)(nt * nt), Gw, (rocblas_stride)nt, Ge,
+                                            (rocblas_stride)nt, Ginfo, (int)nb);
+    if (st != rocblas_status_success) {
+      cleanup();
+      return set_err(c, FBSE_ERR_SOLVER, "refine: batched eigensolver status %d", (int)st);
     }
   }
-  const long nc = (long)cand_j.size();
+  std::vector<double> lam_b((size_t)nb * nt), vec_b((size_t)nb * nt * nt * cw);
+  std::vector<int> info(nb);
+  REF_HIP(hipMemcpyAsync(lam_b.data(), Gw, lam_b.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  REF_HIP(hipMemcpyAsync(vec_b.data(), G, vec_b.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  REF_HIP(hipMemcpyAsync(info.data(), Ginfo, nb * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  REF_HIP(hipStreamSynchronize(c->stream));
+  // candidates: rank r of cluster b (its r-th lowest sigma slot) gets the r-th
+  // lowest refined vector when that one's predicted residual is lower; a
+  // candidate that carries more than half its weight on another Ritz vector,
+  // or on a candidate taken before it, would duplicate that state: rejected
+  struct Cand {
+    long slot;
+    double pred;
+    const double *y;
+  };
+  std::vector<Cand> cands;
+  for (long b = 0; b < nb; ++b) {
+    if (info[b] != 0) continue;
+    for (size_t r = 0; r < clusters[b].size(); ++r) {
+      const double lm = std::max(0.0, lam_b[(size_t)b * nt + r]);
+      const long slot = clusters[b][r];
+      if (sqrt(lm) < sigma[slot]) cands.push_back({slot, sqrt(lm), &vec_b[((size_t)b * nt + r) * nt * cw]});
+    }
+  }
+  std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.pred < b.pred; });
+  auto w2 = [&](const double *y, long l) { return cw == 2 ? y[2 * l] * y[2 * l] + y[2 * l + 1] * y[2 * l + 1] : y[l] * y[l]; };
+  std::vector<Cand> acc;
+  for (const Cand &cd : cands) {
+    bool ok = true;
+    for (long l = 0; l < n && ok; ++l)
+      if (l != cd.slot && w2(cd.y, l) > 0.5) ok = false;
+    for (const Cand &a : acc) {
+      if (!ok) break;
+      double re = 0.0, im = 0.0;
+      for (long l = 0; l < nt; ++l) {
+        if (cw == 2) {
+          re += a.y[2 * l] * cd.y[2 * l] + a.y[2 * l + 1] * cd.y[2 * l + 1];
+          im += a.y[2 * l] * cd.y[2 * l + 1] - a.y[2 * l + 1] * cd.y[2 * l];
+        } else {
+          re += a.y[l] * cd.y[l];
+        }
+      }
+      if (re * re + im * im > 0.5) ok = false;
+    }
+    if (ok) acc.push_back(cd);
+  }
+  const long nc = (long)acc.size();
   if (nc == 0) {
     cleanup();
     return FBSE_OK;
   }
-  // new vectors [U, W] y_j into Y: U (n x nc block of y) then += W (k2 x nc)
+  // the accepted vectors Q y = U y_top + W y_bottom into Y (H W is not needed any more)
   {
     std::vector<double> yt((size_t)n * nc * cw), yb((size_t)k2 * nc * cw);
     for (long q2 = 0; q2 < nc; ++q2) {
-      const double *col = &Yv[(size_t)cand_j[q2] * nt * cw];
-      memcpy(&yt[(size_t)q2 * n * cw], col, (size_t)n * cw * 8);
-      memcpy(&yb[(size_t)q2 * k2 * cw], col + (size_t)n * cw, (size_t)k2 * cw * 8);
+      memcpy(&yt[(size_t)q2 * n * cw], acc[q2].y, (size_t)n * cw * 8);
+      memcpy(&yb[(size_t)q2 * k2 * cw], acc[q2].y + (size_t)n * cw, (size_t)k2 * cw * 8);
     }
     double *d_yt = Cm;  // n x k >= n x nc
     REF_HIP(hipMemcpyAsync(d_yt, yt.data(), yt.size() * 8, hipMemcpyHostToDevice, c->stream));
@@ -1858,13 +1983,14 @@ static int refine_impl(fbse_ctx *c, double *d_psi, long n, const double *eval, c
     REF_HIP(hipStreamSynchronize(c->stream));
     REF_HIP(hipMemcpyAsync(d_yt, yb.data(), yb.size() * 8, hipMemcpyHostToDevice, c->stream));
     REF_RC(la.product(X, k2, d_yt, k2, nc, Y, true));
+    REF_HIP(hipStreamSynchronize(c->stream));
   }
   REF_RC(normalize_dev_impl(c, Y, nc));
   std::vector<double> ev2(nc), sg2(nc);
   REF_RC(energies_dev_impl(c, Y, nc, ev2.data(), sg2.data()));
   long nrep = 0;
   for (long q2 = 0; q2 < nc; ++q2) {
-    const long i = cand_i[q2];
+    const long i = acc[q2].slot;
     if (std::isfinite(sg2[q2]) && sg2[q2] < sigma[i]) {
       REF_HIP(hipMemcpyAsync(d_psi + i * sl, Y + q2 * sl, sl * 8, hipMemcpyDeviceToDevice, c->stream));
       eval_out[i] = ev2[q2];

@@ -397,7 +397,22 @@ CONFIGS = {
     "c3_so_160": dict(radius_nm=3.5, n_req=(160, 160, 160), dx=0.89, so=True, nl=True, n_starts=8, ms=16,
                       ncheby=2048, targets=(10, 6, -0.31, -0.24, -0.15, -0.06), builder="gpu"),
     # 4.5 nm dot, 256^3 (dx 0.71: the grid rule keeps 256), 256 filtered states
-    # (16 starts x 16 targets), Newton length 2048
+    # (16 starts x 16 targets), Newton length 2048.  KEmax = 5 Ha (the input
+    # key KEmax, read.c:385-387) gives dE = 7.54 Ha and a filter width
+    # sigma_f = 2.5 dE / (sqrt(2) M) = 0.0065 Ha; the targets are spaced
+    # 0.0134 Ha >= 2 sigma_f, as the reference's filter-width check asks
+    # (coeff.c:30-56), 8 from the valence-band edge (-0.248 Ha) down, 8 from
+    # the middle of the gap across the conduction-band edge (-0.161 Ha)
+    # (set_ene_targets, init.c:401-478: VB targets end at VBmax, CB targets
+    # start at CBmin, spacing (max - min) / n)
     "c4_so_256": dict(radius_nm=4.5, n_req=(256, 256, 256), dx=0.71, so=True, nl=True, n_starts=16, ms=16,
-                      ncheby=2048, targets=(10, 6, -0.31, -0.24, -0.15, -0.06), builder="gpu"),
+                      ncheby=2048, ke_max=5.0, targets=(8, 8, -0.3134, -0.2062, -0.19, -0.0828), builder="gpu"),
 }
+
+# Batched uneven dots: 32 spin-orbit dots with seeded radii uniform in
+# 1.0-3.0 nm at dx = 1.0 bohr (the reference grid rule needs ~52^3 ... ~120^3;
+# each axis is rounded up to a size with prime factors 2, 3, 5), 32 starts x 8
+# targets each, Newton length 1024, mixed precision.
+UNEVEN = dict(n_dots=32, r_lo=1.0, r_hi=3.0, dx=1.0, n_starts=32, ms=8, ncheby=1024,
+              targets=(4, 4, -0.32, -0.22, -0.14, -0.04), seed=2, group_max=8)
+

@@ -120,8 +120,9 @@ def train(mlp, xs, ts, steps=600, batch=65536, lr=2e-3, seed=0, log=None):
 SMOOTH_EP = 0.01
 
 
-def warm_starts(ctx, mlp, job, d_vloc, starts_dev, channel=2, smooth_ep=SMOOTH_EP):
-    """Start states (device tensor n_starts x stlen), low-passed when
+def warm_starts(ctx, mlp, job, d_vloc, starts_dev, channel=2, smooth_ep=SMOOTH_EP, ms=1):
+    """Start states (device tensor n_starts x stlen, or the start-state view
+    [:, 0, :] of a psi_rank block with ms targets per start), low-passed when
     smooth_ep > 0, times the predicted band-edge density's square root,
     renormalised (in place, returned)."""
     import torch
@@ -130,5 +131,52 @@ def warm_starts(ctx, mlp, job, d_vloc, starts_dev, channel=2, smooth_ep=SMOOTH_E
     y = torch.empty((ng, N_OUT), dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
     mlp.forward_dev(x.data_ptr(), ng, y.data_ptr())
-    ctx.warm_start_dev(y.data_ptr(), N_OUT, channel, starts_dev.data_ptr(), starts_dev.shape[0], 1, smooth_ep)
+    ctx.warm_start_dev(y.data_ptr(), N_OUT, channel, starts_dev.data_ptr(), starts_dev.shape[0], ms, smooth_ep)
     return starts_dev
+
+
+# Training dots: spin-orbit zincblende dots of 1.0-1.3 nm on a 48^3 grid at
+# dx = 1.0 bohr (seeds 11-14), 16 ran2 starts x 16 targets straddling the gap,
+# Newton length 2048, plain FP64 filter.
+TRAIN_DOTS = ((1.0, 11), (1.1, 12), (1.2, 13), (1.3, 14))
+TRAIN_TARGETS = (8, 8, -0.32, -0.22, -0.14, -0.04)
+
+
+def train_network(width=128, steps=800, seed=1, log=None):
+    """Trains the warm-start network on the band-edge states the plain FP64
+    filter converges on the training dots, on the GPU (fbse_mlp_*).  Returns
+    (mlp, info) with the per-dot converged counts and the loss history."""
+    import torch
+    from . import dots
+    mlp = api.Mlp(N_IN, width, N_OUT, seed=seed)
+    tg = api.ene_targets(*TRAIN_TARGETS)
+    lo, hi = tg.min() - 0.05, tg.max() + 0.05
+    xs, ts, info = [], [], []
+    for r, sd in TRAIN_DOTS:
+        job = dots.build_dot(r, (48, 48, 48), 1.0, so=True, nl=True, seed=sd)
+        ctx = api.Context(job["nx"], job["ny"], job["nz"], job["nspin"], job["is_complex"], job["dv"], job["ksqr"],
+                          job["pot_local"], job["proj"])
+        p0, p1 = api.energy_range_start_states(job["nx"], job["ny"], job["nz"], job["nspin"], job["is_complex"],
+                                               job["dv"])
+        rng = ctx.energy_range(p0, p1, max_iter=1500)
+        st, _ = api.init_filter_states(16, 1, -5, job["nx"], job["ny"], job["nz"], job["nspin"], job["is_complex"],
+                                       job["dv"])
+        st = torch.from_numpy(st).cuda().view(16, -1)
+        psi, ev, sig, rank = filter_pass(ctx, job, st, 16, 2048, tg, rng[2], rng[3])
+        vb, cb = edge_sets(ev, sig, lo, hi)
+        info.append(dict(radius_nm=r, converged=int((sig < 1e-4).sum()), vb=len(vb), cb=len(cb)))
+        if log:
+            log("warm-start training dot %.1f nm: %d converged, %d + %d band-edge states" % (
+                r, info[-1]["converged"], len(vb), len(cb)))
+        if len(vb) and len(cb):
+            ts.append(band_edge_targets(ctx, job, psi, vb, cb))
+            d_v = torch.from_numpy(job["pot_local"]).cuda()
+            xs.append(dot_features(mlp, job, d_v.data_ptr()))
+            del d_v
+        del psi, st
+        ctx.close()
+    if not xs:
+        raise RuntimeError("warm-start training: no training dot converged band-edge states")
+    losses = train(mlp, xs, ts, steps=steps, log=log)
+    return mlp, dict(dots=info, loss_first=float(losses[0]), loss_last=float(losses[-1]), steps=steps)
+

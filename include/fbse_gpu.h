@@ -160,13 +160,14 @@ int fbse_rotate_dev(fbse_ctx *ctx, double *d_a, long nin, const double *v, long 
  * eval and sigma (fbse_sigma).  The pairs with eval in [e_lo, e_hi] and
  * sigma < sigma_max (lowest sigma first, as many as the device memory holds
  * two blocks of) get residuals r = H u - e u, preconditioned in k-space
- * (r / (1 + 0.5|k|^2 / e_p); e_p <= 0 selects 1 Ha), orthogonalised against
- * all n_ritz Ritz vectors; Rayleigh-Ritz over [Ritz vectors, corrections]
- * then gives new pairs, and a new pair replaces the Ritz pair it overlaps by
- * more than one half only when its residual is lower.  On return psitot holds
- * n_ritz pairs in the same slots, eval_out / sigma_out (n_ritz-long host
- * arrays) their values, *n_replaced how many changed: no pair is dropped and
- * none gets a larger residual. */
+ * (r / (1 + 0.5|k|^2 / e_p); e_p <= 0 selects 1 Ha) and orthogonalised
+ * against all n_ritz Ritz vectors.  Over that extended basis each selected
+ * pair gets its refined vector, the unit vector with the least
+ * ||H v - e v|| (degenerate pairs together), which replaces the pair only
+ * when its own residual is lower and it duplicates no other state.  On return
+ * psitot holds n_ritz pairs in the same slots, eval_out / sigma_out (n_ritz-
+ * long host arrays) their values, *n_replaced how many changed: no pair is
+ * dropped and none gets a larger residual. */
 int fbse_refine(fbse_ctx *ctx, double *psitot, long n_ritz, const double *eval, const double *sigma,
                 double sigma_max, double e_lo, double e_hi, double e_p, double *eval_out, double *sigma_out,
                 long *n_replaced);
