@@ -352,6 +352,59 @@ def mlp_sample(args, dev):
     print(json.dumps(line), flush=True)
 
 
+def gemm_sample(args, dev):
+    """The FP64 matrix-core subspace products (gemm_kernels.hip) at the
+    workloads' shapes: a block of n random complex states of the workload's
+    grid (n = its filtered states), the overlap dv A^H A (fbse_overlap_dev:
+    the general kernel over all 64 x 64 tiles, split-K over row slabs) and the
+    rotation A V in place (fbse_rotate_dev); FLOPs 8 m n^2 each (complex
+    multiply-add = 8 real operations), against the dense FP64 peak of
+    78.6 TFLOP/s."""
+    import torch
+    from filterbse_amd import api, dots
+    shapes = {"c1_so_96": (96, 2, 1024), "c4_so_256": (256, 2, 256)}
+    out = []
+    for name in args.gemm_sample.split(","):
+        g, nspin, n = shapes[name]
+        ng = g ** 3
+        ctx = api.Context(g, g, g, nspin, 1, 1.0, np.zeros(ng), np.zeros(nspin * ng), None)
+        m = nspin * ng
+        a = torch.randn(n * 2 * m, dtype=torch.float64, device=dev)
+        a /= math.sqrt(2.0 * m)
+        V = np.linalg.qr(np.random.default_rng(1).standard_normal((n, n)))[0].astype(np.complex128)
+        torch.cuda.synchronize()
+        ctx.overlap_dev(a.data_ptr(), n, a.data_ptr(), n)  # warm-up
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.overlap_dev(a.data_ptr(), n, a.data_ptr(), n)
+        t_g = (time.perf_counter() - t0) / reps
+        ctx.rotate_dev(a.data_ptr(), V)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.rotate_dev(a.data_ptr(), V)
+        ctx.synchronize()
+        t_r = (time.perf_counter() - t0) / reps
+        fl = 8.0 * m * n * n
+        out.append(dict(shape=name, rows=m, states=n, gram_ms=round(1e3 * t_g, 3),
+                        gram_tflops=round(fl / t_g / 1e12, 2), rotate_ms=round(1e3 * t_r, 3),
+                        rotate_tflops=round(fl / t_r / 1e12, 2)))
+        log("gemm sample %s: %s" % (name, out[-1]))
+        del a
+        ctx.close()
+        torch.cuda.empty_cache()
+    peak = 78.6
+    best = max(o["gram_tflops"] for o in out)
+    line = dict(metric="FP64 matrix-core subspace products (Gram A^H A, rotation A V)", value=best,
+                unit="TFLOP/s", n_gpus=1, higher_is_better=True, dtype="f64 (complex)", data="synthetic",
+                config=dict(workload="random complex state blocks of the c1 / c4 shapes"), shapes=out,
+                mfma=dict(bound="mfma", peak=peak, unit="TFLOP/s",
+                          frac={o["shape"]: dict(gram=round(o["gram_tflops"] / peak, 4),
+                                                 rotate=round(o["rotate_tflops"] / peak, 4)) for o in out}))
+    print(json.dumps(line), flush=True)
+
+
 def bench_uneven(args, dev):
     """One step = the filters of all dots (fbse_filter_many: every dot on its
     own stream, launches interleaved) plus normalize / ortho / RR / sigma_E of
@@ -473,7 +526,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20, help="timed Newton steps of the mixed filter")
     ap.add_argument("--warmup", type=int, default=5, help="untimed Newton steps before them")
     ap.add_argument("--window", type=int, default=0,
-                    help="accumulation window depth K (0 = as deep as 60%% of the free HBM allows)")
+                    help="accumulation window depth K (0 = as deep as 75%% of the free HBM allows)")
     ap.add_argument("--config", default="c4_so_256")
     ap.add_argument("--no-warm", action="store_true", help="mixed pass from the plain ran2 start states")
     ap.add_argument("--refine", type=int, default=1, help="FP64 refinement sweeps in the mixed pass")
@@ -486,6 +539,8 @@ def main():
                     help="Newton length override (short filters for counter runs of the other stages)")
     ap.add_argument("--mlp-sample", type=int, default=0, metavar="N",
                     help="only measure the warm-start network's MFMA kernels on N grid points, then exit")
+    ap.add_argument("--gemm-sample", default="", metavar="SHAPES",
+                    help="only time the FP64 matrix-core Gram and rotation at c1_so_96,c4_so_256 shapes, then exit")
     ap.add_argument("--io-sample", type=int, default=0, metavar="N",
                     help="only measure state-file streaming of N FP64 spinor states of 96^3, then exit")
     args = ap.parse_args()
@@ -502,6 +557,9 @@ def main():
         return
     if args.io_sample:
         io_sample(args, dev)
+        return
+    if args.gemm_sample:
+        gemm_sample(args, dev)
         return
     if args.mlp_sample:
         mlp_sample(args, dev)

@@ -449,8 +449,8 @@ extern "C" int fbse_create(const fbse_grid_desc *g, const fbse_proj_desc *pd, in
             break;
           }
         }
-    // a capped 1D entry would make the tables wrong; require no cap on the axes
-    for (long i = 0; sep && i < nx; ++i) sep = kx[i] < kmax || i == 0 || kmax == 0.0;
+    // (an axis entry at the cap is exact too: any sum with it reaches the cap;
+    // the check above covers every point either way)
     c->ksqr_sep = sep ? 1 : 0;
     c->kcap = kmax;
     if ((rc = upload(c, (void **)&c->d_kx, kx.data(), nx * 8)) || (rc = upload(c, (void **)&c->d_ky, ky.data(), ny * 8)) ||
@@ -1043,7 +1043,7 @@ template <typename T> struct FilterRun {
   }
 };
 
-// window depth: forced, or as deep as 60% of free memory allows for all the
+// window depth: forced, or as deep as 75% of free memory allows for all the
 // runs' rings together (<= kMaxWindow); the rest covers the FP32 staging
 // slot, the autotuners' scratch and the projection coefficients.  The rings
 // stay within kMaxRingBytes: a 165 GB FP64 ring (K = 89 at 96^3) made the
@@ -1056,7 +1056,7 @@ static int window_depth(int forced, size_t ring_bytes_per_slot, size_t held) {
   (void)hipMemGetInfo(&fr, &tot);
   // rings reuse the contexts' scratch buffers: what they hold counts as free
   fr += held;
-  const double room = std::min(0.6 * (double)fr, kMaxRingBytes);
+  const double room = std::min(0.75 * (double)fr, kMaxRingBytes);
   long kk = (long)(room / (double)ring_bytes_per_slot) - 2;
   return (int)std::max(1L, std::min((long)kMaxWindow, kk));
 }
@@ -1717,11 +1717,14 @@ __global__ void refined_matrix_kernel(const double *M, const double *M2, const d
 //   2. the corrections are orthogonalised against all n Ritz vectors (twice)
 //      and among themselves (rank cut), giving Q = [U, W];
 //   3. A1 = Q^H H Q and A2 = (H Q)^H (H Q) on the matrix-core Grams;
-//   4. for each selected pair (clusters of equal theta together) the unit
-//      vector of span(Q) with the least ||(H - theta) v|| (lowest eigenvectors
-//      of A2 - 2 theta A1 + theta^2); since u itself lies in span(Q), that is
-//      never above the pair's sigma, also where the spectrum is too dense for
-//      interior Rayleigh-Ritz to help;
+//   4. for each selected pair two candidates: its Rayleigh-Ritz vector over
+//      span(Q) (when that carries more than half its weight on the pair), and
+//      its refined vector, the unit vector with the least ||(H - theta) v||
+//      (lowest eigenvectors of A2 - 2 theta A1 + theta^2, near-degenerate
+//      pairs together, theta updated twice to the refined vectors' Rayleigh
+//      quotient); since u itself lies in span(Q), the refined residual is never
+//      above the pair's sigma, also where the spectrum is too dense for
+//      interior Rayleigh-Ritz to help; the lower predicted residual wins;
 //   5. a candidate that would duplicate another Ritz vector or an earlier
 //      candidate (overlap above one half) is dropped, and a pair is replaced
 //      only when its new vector's own residual is lower.
@@ -1865,17 +1868,44 @@ static int refine_impl(fbse_ctx *c, double *d_psi, long n, const double *eval, c
   }
   hipLaunchKernelGGL(mirror_upper_kernel, dim3(256), dim3(256), 0, c->stream, M2, nt, cw - 1);
   REF_HIP(hipGetLastError());
-  // 4. refined Ritz vectors: for the selected pairs, grouped into clusters of
-  // equal theta (Kramers pairs), the s lowest eigenvectors y of
-  // G(theta) = A2 - 2 theta A1 + theta^2 (dv-weighted: the smallest
-  // ||(H - theta) Q y||^2 over unit Q y).  A Ritz vector u_i is itself a unit
-  // vector of span(Q), so the lowest value is at most sigma_i^2.
-  std::vector<long> order(sel.begin(), sel.end());
-  std::sort(order.begin(), order.end(), [&](long a, long b) { return eval[a] < eval[b]; });
+  // 4. candidates for every selected pair, from two constructions over span(Q):
+  //  (a) Rayleigh-Ritz: the eigenvectors y of A1 whose weight on the pair's
+  //      Ritz vector exceeds one half (one to one), predicted residual^2
+  //      y^H A2 y - mu^2;
+  //  (b) refined vectors: the s lowest eigenvectors of
+  //      G(theta) = A2 - 2 theta A1 + theta^2 for a cluster of s pairs with
+  //      nearly equal theta (the least ||(H - theta) Q y|| over unit y; the
+  //      pair's own Ritz vector is one such y, so the lowest value never
+  //      exceeds its sigma^2), theta updated twice to the Rayleigh quotient of
+  //      the cluster's refined vectors.
+  //  Each pair takes the candidate with the lower predicted residual.
+  const long nt2 = nt * nt;
+  std::vector<double> A1h(nt2 * cw), A2h(nt2 * cw);
+  REF_HIP(hipMemcpyAsync(A1h.data(), M, A1h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  REF_HIP(hipMemcpyAsync(A2h.data(), M2, A2h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  REF_HIP(hipStreamSynchronize(c->stream));
+  for (auto &v : A1h) v *= dv;  // A1, A2 in the dv-weighted metric (energies, energies^2)
+  for (auto &v : A2h) v *= dv;
+  // y^H A y = sum_ij conj(y_i) A_ij y_j (real for Hermitian A), A column-major on the host
+  auto quad = [&](const std::vector<double> &A, const double *y) {
+    double acc = 0.0;
+    for (long j = 0; j < nt; ++j) {
+      const double yr = y[j * cw], yi = cw == 2 ? y[j * cw + 1] : 0.0;
+      for (long i = 0; i < nt; ++i) {
+        const double ar = A[(j * nt + i) * cw], ai = cw == 2 ? A[(j * nt + i) * cw + 1] : 0.0;
+        const double xr = y[i * cw], xi = cw == 2 ? y[i * cw + 1] : 0.0;
+        const double pr = ar * yr - ai * yi, pi = ar * yi + ai * yr;  // A_ij y_j
+        acc += xr * pr + xi * pi;                                     // Re(conj(y_i) A_ij y_j)
+      }
+    }
+    return acc;
+  };
   // clusters: consecutive values closer than the larger of the two residuals
   // (at most 1e-3 Ha): the Ritz values of one degenerate level, split by the
   // noise of the mixed-precision filter, refine together, so that each gets
   // its own vector of the level instead of a copy of its neighbour's
+  std::vector<long> order(sel.begin(), sel.end());
+  std::sort(order.begin(), order.end(), [&](long a, long b) { return eval[a] < eval[b]; });
   std::vector<std::vector<long>> clusters;
   for (long i : order) {
     if (!clusters.empty()) {
@@ -1896,55 +1926,97 @@ static int refine_impl(fbse_ctx *c, double *d_psi, long n, const double *eval, c
     cth[b] = sum / (double)clusters[b].size();
     std::sort(clusters[b].begin(), clusters[b].end(), [&](long a, long bb) { return sigma[a] < sigma[bb]; });
   }
-  REF_HIP(hipMalloc(&G, (size_t)nb * nt * nt * es));
+  REF_HIP(hipMalloc(&G, (size_t)nb * nt2 * es));
   REF_HIP(hipMalloc(&Gw, (size_t)nb * nt * 8));
   REF_HIP(hipMalloc(&Ge, (size_t)nb * nt * 8));
   REF_HIP(hipMalloc(&Ginfo, (size_t)nb * sizeof(int)));
-  REF_HIP(hipMemcpyAsync(d_th, cth.data(), nb * 8, hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(refined_matrix_kernel, dim3(256), dim3(256), 0, c->stream, (const double *)M, (const double *)M2,
-                     (const double *)d_th, nt, nb, dv, cw - 1, G);
-  REF_HIP(hipGetLastError());
-  {
+  std::vector<double> lam_b((size_t)nb * nt), vec_b((size_t)nb * nt2 * cw);
+  std::vector<int> info(nb);
+  auto refined_solve = [&]() -> int {
+    REF_HIP(hipMemcpyAsync(d_th, cth.data(), nb * 8, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(refined_matrix_kernel, dim3(256), dim3(256), 0, c->stream, (const double *)M,
+                       (const double *)M2, (const double *)d_th, nt, nb, dv, cw - 1, G);
+    REF_HIP(hipGetLastError());
     rocblas_status st;
     if (cw == 2)
       st = rocsolver_zheevd_strided_batched(c->blas, rocblas_evect_original, rocblas_fill_upper, (int)nt,
-                                            (rocblas_double_complex *)G, (int)nt, (rocblas_stride)(nt * nt), Gw,
+                                            (rocblas_double_complex *)G, (int)nt, (rocblas_stride)nt2, Gw,
                                             (rocblas_stride)nt, Ge, (rocblas_stride)nt, Ginfo, (int)nb);
     else
       st = rocsolver_dsyevd_strided_batched(c->blas, rocblas_evect_original, rocblas_fill_upper, (int)nt, G, (int)nt,
-                                            (rocblas_stride)(nt * nt), Gw, (rocblas_stride)nt, Ge,
-                                            (rocblas_stride)nt, Ginfo, (int)nb);
+                                            (rocblas_stride)nt2, Gw, (rocblas_stride)nt, Ge, (rocblas_stride)nt,
+                                            Ginfo, (int)nb);
     if (st != rocblas_status_success) {
+      rc = set_err(c, FBSE_ERR_SOLVER, "refine: batched eigensolver status %d", (int)st);
       cleanup();
-      return set_err(c, FBSE_ERR_SOLVER, "refine: batched eigensolver status %d", (int)st);
+      return rc;
+    }
+    REF_HIP(hipMemcpyAsync(lam_b.data(), Gw, lam_b.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    REF_HIP(hipMemcpyAsync(vec_b.data(), G, vec_b.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    REF_HIP(hipMemcpyAsync(info.data(), Ginfo, nb * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    REF_HIP(hipStreamSynchronize(c->stream));
+    return FBSE_OK;
+  };
+  for (int round = 0; round < 3; ++round) {
+    if ((rc = refined_solve())) return rc;
+    if (round == 2) break;
+    for (long b = 0; b < nb; ++b) {  // theta <- mean Rayleigh quotient of the cluster's refined vectors
+      if (info[b] != 0) continue;
+      double sum = 0.0;
+      const long sz = (long)clusters[b].size();
+      for (long r = 0; r < sz; ++r) sum += quad(A1h, &vec_b[((size_t)b * nt + r) * nt * cw]);
+      cth[b] = sum / (double)sz;
     }
   }
-  std::vector<double> lam_b((size_t)nb * nt), vec_b((size_t)nb * nt * nt * cw);
-  std::vector<int> info(nb);
-  REF_HIP(hipMemcpyAsync(lam_b.data(), Gw, lam_b.size() * 8, hipMemcpyDeviceToHost, c->stream));
-  REF_HIP(hipMemcpyAsync(vec_b.data(), G, vec_b.size() * 8, hipMemcpyDeviceToHost, c->stream));
-  REF_HIP(hipMemcpyAsync(info.data(), Ginfo, nb * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  // Rayleigh-Ritz eigenvectors of A1 (the raw M, in place) and their predicted residuals
+  std::vector<double> mu(nt), Yrr(nt2 * cw);
+  hipFree(d_w);
+  d_w = nullptr;
+  REF_HIP(hipMalloc(&d_w, (size_t)nt * 8));
+  REF_RC(la.eigh(M, nt, d_w));
+  REF_HIP(hipMemcpyAsync(mu.data(), d_w, nt * 8, hipMemcpyDeviceToHost, c->stream));
+  REF_HIP(hipMemcpyAsync(Yrr.data(), M, Yrr.size() * 8, hipMemcpyDeviceToHost, c->stream));
   REF_HIP(hipStreamSynchronize(c->stream));
-  // candidates: rank r of cluster b (its r-th lowest sigma slot) gets the r-th
-  // lowest refined vector when that one's predicted residual is lower; a
-  // candidate that carries more than half its weight on another Ritz vector,
-  // or on a candidate taken before it, would duplicate that state: rejected
+  std::vector<char> is_sel(n, 0);
+  for (long j = 0; j < k; ++j) is_sel[sel[j]] = 1;
+  auto w2 = [&](const double *y, long l) {
+    return cw == 2 ? y[2 * l] * y[2 * l] + y[2 * l + 1] * y[2 * l + 1] : y[l] * y[l];
+  };
   struct Cand {
     long slot;
     double pred;
     const double *y;
   };
-  std::vector<Cand> cands;
+  std::vector<Cand> best(n, Cand{-1, 1e300, nullptr});
+  for (long j = 0; j < nt; ++j) {
+    const double *y = &Yrr[(size_t)j * nt * cw];
+    long bi = -1;
+    double bw = 0.0;
+    for (long i = 0; i < n; ++i)
+      if (w2(y, i) > bw) {
+        bw = w2(y, i);
+        bi = i;
+      }
+    if (bi < 0 || bw <= 0.5 || !is_sel[bi]) continue;
+    const double m = mu[j] * dv;
+    const double pred = sqrt(std::max(0.0, quad(A2h, y) - m * m));
+    if (pred < sigma[bi] && pred < best[bi].pred) best[bi] = Cand{bi, pred, y};
+  }
   for (long b = 0; b < nb; ++b) {
     if (info[b] != 0) continue;
     for (size_t r = 0; r < clusters[b].size(); ++r) {
-      const double lm = std::max(0.0, lam_b[(size_t)b * nt + r]);
       const long slot = clusters[b][r];
-      if (sqrt(lm) < sigma[slot]) cands.push_back({slot, sqrt(lm), &vec_b[((size_t)b * nt + r) * nt * cw]});
+      const double pred = sqrt(std::max(0.0, lam_b[(size_t)b * nt + r]));
+      if (pred < sigma[slot] && pred < best[slot].pred)
+        best[slot] = Cand{slot, pred, &vec_b[((size_t)b * nt + r) * nt * cw]};
     }
   }
+  std::vector<Cand> cands;
+  for (long i = 0; i < n; ++i)
+    if (best[i].slot >= 0) cands.push_back(best[i]);
+  // a candidate that carries more than half its weight on another Ritz
+  // vector, or on a candidate taken before it, would duplicate that state
   std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.pred < b.pred; });
-  auto w2 = [&](const double *y, long l) { return cw == 2 ? y[2 * l] * y[2 * l] + y[2 * l + 1] * y[2 * l + 1] : y[l] * y[l]; };
   std::vector<Cand> acc;
   for (const Cand &cd : cands) {
     bool ok = true;

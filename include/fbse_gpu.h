@@ -193,7 +193,7 @@ int fbse_profile_read(fbse_ctx *ctx, int kind, long *launches, double *total_ms)
 
 /* Window depth K of the blocked Newton accumulation, FFT tile widths etc. are
  * chosen by the built-in autotuner at the first call; this forces K
- * (1..127; 0 = auto: as deep as 60% of the free HBM allows, within 120 GB
+ * (1..127; 0 = auto: as deep as 75% of the free HBM allows, within 120 GB
  * of ring and at most 127). */
 int fbse_set_window(fbse_ctx *ctx, int window);
 
