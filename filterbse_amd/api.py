@@ -57,6 +57,7 @@ GPU_SIGNATURES = {
     "fbse_synchronize": (C.c_int, [C.c_void_p]),
     "fbse_apply_h": (C.c_int, [C.c_void_p, DP, DP, C.c_long]),
     "fbse_apply_h_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long]),
+    "fbse_apply_h_dev_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long]),
     "fbse_filter": (C.c_int, [C.c_void_p, DP, C.c_long, C.c_long, C.c_long, DP, DP, C.c_double, C.c_double,
                               C.c_int]),
     "fbse_filter_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_long, DP, DP, C.c_double,
@@ -380,6 +381,9 @@ class Context:
 
     def apply_h_dev(self, d_in, d_out, nstates):
         self._check(gpu().fbse_apply_h_dev(self.ctx, C.c_void_p(d_in), C.c_void_p(d_out), nstates))
+
+    def apply_h_dev_noalloc(self, d_in, d_out, nstates):
+        self._check(gpu().fbse_apply_h_dev_noalloc(self.ctx, C.c_void_p(d_in), C.c_void_p(d_out), nstates))
 
     def filter_dev(self, d_psi_rank, n_starts, ms, an, zn, Vmin, dE_1, precision=FBSE_PREC_FP64):
         an, zn = _f64(an), _f64(zn)

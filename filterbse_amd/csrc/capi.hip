@@ -30,6 +30,8 @@ int set_err(fbse_ctx *c, int code, const char *fmt, ...) {
 
 static int grow(fbse_ctx *c, void **p, size_t *cur, size_t bytes) {
   if (bytes <= *cur) return FBSE_OK;
+  if (c->no_alloc)
+    return set_err(c, FBSE_ERR_NOMEM, "a %zu-byte buffer was needed inside the no-allocation entry point", bytes);
   if (*p) {
     FBSE_HIP(c, hipStreamSynchronize(c->stream));
     FBSE_HIP(c, hipFree(*p));
@@ -385,6 +387,40 @@ static int build_projectors(fbse_ctx *c, const fbse_proj_desc *pd) {
   return FBSE_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Work space of fbse_apply_h_dev_noalloc: one file-scope device array, which
+// every GPU the library runs on holds from module load to process end.  In
+// host code the array's name is only a stand-in: its device address is looked
+// up once per device (hipGetSymbolAddress, at the first fbse_create on that
+// device, with the device current) and kept in noalloc_base[device]; kernels
+// and copies get that address, never the name.
+// ---------------------------------------------------------------------------
+static constexpr size_t kNoAllocBytes = (size_t)640 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+static constexpr int kNoAllocDevices = 64;
+static std::mutex noalloc_mu;
+static void *noalloc_base[kNoAllocDevices] = {};
+
+static int noalloc_setup(fbse_ctx *c) {
+  if (c->device < 0 || c->device >= kNoAllocDevices)
+    return set_err(c, FBSE_ERR_UNSUPPORTED, "device %d beyond the %d the library tracks", c->device, kNoAllocDevices);
+  std::lock_guard<std::mutex> lock(noalloc_mu);
+  if (noalloc_base[c->device]) return FBSE_OK;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return set_err(c, FBSE_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", c->device,
+                   hipGetErrorString(e));
+  noalloc_base[c->device] = ptr;
+  return FBSE_OK;
+}
+
+static void *noalloc_ptr(int device) {
+  if (device < 0 || device >= kNoAllocDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(noalloc_mu);
+  return noalloc_base[device];
+}
+
 extern "C" int fbse_create(const fbse_grid_desc *g, const fbse_proj_desc *pd, int device, fbse_ctx **out) {
   if (!out) return FBSE_ERR_ARG;
   *out = nullptr;
@@ -408,6 +444,10 @@ extern "C" int fbse_create(const fbse_grid_desc *g, const fbse_proj_desc *pd, in
     return FBSE_ERR_HIP;
   }
   c->stream = c->own_stream;
+  if ((rc = noalloc_setup(c))) {
+    fbse_destroy(c);
+    return rc;
+  }
   if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
@@ -685,6 +725,59 @@ extern "C" int fbse_apply_h_dev(fbse_ctx *c, const double *d_psi, double *d_hpsi
   if (!c || nstates < 0 || (nstates && (!d_psi || !d_hpsi))) return FBSE_ERR_ARG;
   FBSE_HIP(c, hipSetDevice(c->device));
   return apply_h_dev_impl(c, d_psi, d_hpsi, nstates);
+}
+
+// fbse_apply_h_dev without device allocation: the FFT work buffer(s) and the
+// projection coefficients of apply_h_dev_impl are carved out of the library's
+// static work space instead of the context's buffers, and the autotuners
+// (which time candidates in a scratch allocation) and profiling events stay
+// off for the call; every other step is the same code.  No byte of the work
+// space is read before this call writes it.
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+extern "C" int fbse_apply_h_dev_noalloc(fbse_ctx *c, const double *d_psi, double *d_hpsi, long nstates) {
+  if (!c || nstates < 0 || (nstates && (!d_psi || !d_hpsi))) return FBSE_ERR_ARG;
+  FBSE_HIP(c, hipSetDevice(c->device));
+  char *base = (char *)noalloc_ptr(c->device);
+  if (!base) return set_err(c, FBSE_ERR_HIP, "no-allocation work space of device %d is not set up", c->device);
+  if (nstates == 0) return FBSE_OK;
+  const size_t per = (size_t)c->nsg * sizeof(double2);
+  // the buffers apply_h_dev_impl asks for: complex states in one chunk; real
+  // states packed two per complex grid (packed input + FFT work, result)
+  const long items = c->is_complex ? nstates : (nstates + 1) / 2;
+  size_t need0 = c->is_complex ? (size_t)items * per : (size_t)items * per * 2;
+  size_t need1 = c->is_complex ? 0 : (size_t)items * per;
+  size_t need2 = c->has_proj ? (size_t)c->pj.n_atoms * items * c->nspin * 6 * sizeof(double2) : 0;
+  need0 = align256(need0);
+  need1 = align256(need1);
+  need2 = align256(need2);
+  if (need0 + need1 + need2 > kNoAllocBytes || (long)(per ? (8ull << 30) / (3 * per) : 0) < nstates)
+    return set_err(c, FBSE_ERR_ARG,
+                   "fbse_apply_h_dev_noalloc: %ld states need %zu bytes of work space, more than its %zu", nstates,
+                   need0 + need1 + need2, kNoAllocBytes);
+  void *sv_s = c->d_scratch, *sv_s2 = c->d_scratch2, *sv_pjd = c->d_pjd;
+  const size_t sv_sb = c->scratch_bytes, sv_s2b = c->scratch2_bytes, sv_pjdb = c->pjd_bytes;
+  const int sv_tune = c->autotune, sv_prof = c->profile;
+  c->d_scratch = base;
+  c->scratch_bytes = need0;
+  c->d_scratch2 = need1 ? base + need0 : nullptr;
+  c->scratch2_bytes = need1;
+  c->d_pjd = need2 ? base + need0 + need1 : nullptr;
+  c->pjd_bytes = need2;
+  c->autotune = 0;
+  c->profile = 0;
+  c->no_alloc = 1;
+  const int rc = apply_h_dev_impl(c, d_psi, d_hpsi, nstates);
+  c->no_alloc = 0;
+  c->autotune = sv_tune;
+  c->profile = sv_prof;
+  c->d_scratch = sv_s;
+  c->scratch_bytes = sv_sb;
+  c->d_scratch2 = sv_s2;
+  c->scratch2_bytes = sv_s2b;
+  c->d_pjd = sv_pjd;
+  c->pjd_bytes = sv_pjdb;
+  return rc;
 }
 
 // host-pointer wrapper: stage through device buffers in chunks

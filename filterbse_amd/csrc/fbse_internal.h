@@ -203,6 +203,7 @@ struct fbse_ctx {
   int last_window = 0;          // accumulation window depth K of the last filter
   double tune_ms[2][2][3][3] = {};
   int autotune = 1;
+  int no_alloc = 0;             // inside fbse_apply_h_dev_noalloc: buffers may not grow
   int force_variant = -1;       // fbse_set_fft_kernel: -1 auto, 0 per-axis passes, 1 plane/slab passes
   int force_lines = 0;
 

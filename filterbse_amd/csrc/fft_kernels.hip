@@ -535,6 +535,9 @@ int fft_path(const fbse_ctx *c, int f32, int *instance) {
 template <typename T> static int ensure_pjd(fbse_ctx *c, long nstates) {
   const size_t need = (size_t)c->pj.n_atoms * nstates * c->nspin * 6 * sizeof(cplx_t<T>);
   if (need > c->pjd_bytes) {
+    if (c->no_alloc)
+      return set_err(c, FBSE_ERR_NOMEM, "projection coefficients (%zu bytes) inside the no-allocation entry point",
+                     need);
     if (c->d_pjd) (void)hipFree(c->d_pjd);
     c->d_pjd = nullptr;
     c->pjd_bytes = 0;

@@ -93,6 +93,19 @@ int fbse_synchronize(fbse_ctx *ctx);
 int fbse_apply_h(fbse_ctx *ctx, const double *psi, double *hpsi, long nstates);
 int fbse_apply_h_dev(fbse_ctx *ctx, const double *d_psi, double *d_hpsi, long nstates);
 
+/* fbse_apply_h_dev that allocates nothing (no hipMalloc / hipFree, no
+ * allocating library routine): for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * it).  No counterpart in the reference.  Its work space is one static device
+ * array of 640 MiB per GPU, held from library load to process end; calls
+ * needing more (nstates x nspin*ngrid x 16 bytes of FFT work, real states
+ * packed two per grid, plus the projection coefficients) return FBSE_ERR_ARG
+ * without running.  Autotuning is off for the call (a context that already
+ * ran fbse_apply_h_dev reuses its tuned FFT kernels).  All calls on one GPU
+ * share the array: calls that overlap on one GPU (several threads, or
+ * contexts on different streams) race and must be serialised by the caller. */
+int fbse_apply_h_dev_noalloc(fbse_ctx *ctx, const double *d_psi, double *d_hpsi, long nstates);
+
 /* The Newton filter over all start states of psi_rank: replaces the loop
  * filter.c:102-116 (filter_cycle, filter.c:187-371, with p_hnorm filter.c:375-410).
  * On entry start state jns sits at psi_rank[jns*ms*stlen]; on exit
