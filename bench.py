@@ -122,6 +122,7 @@ def one_pass(S, precision, warm=False, refine=0, profile=False):
     torch.cuda.synchronize()
     t.append(time.perf_counter())
     ptr = S["psi_rank"].data_ptr()
+    torch.cuda.empty_cache()  # memory torch's allocator caches is HBM the filter's ring cannot use
     free_gb = torch.cuda.mem_get_info()[0] / 1e9
     if profile:
         ctx.profile(True)

@@ -1231,7 +1231,6 @@ static int filter_many_impl(int n, fbse_ctx *const *ctx, double *const *psi, con
 extern "C" int fbse_filter_dev(fbse_ctx *c, double *d_psi_rank, long n_starts, long ms, long M, const double *an,
                                const double *zn, double Vmin, double dE_1, int precision) {
   if (!c || n_starts < 0 || ms < 1 || M < 1 || !an || (M > 1 && !zn)) return FBSE_ERR_ARG;
-  if (ms > 16) return set_err(c, FBSE_ERR_UNSUPPORTED, "ms = %ld > 16 energy targets per start", ms);
   if (n_starts == 0) return FBSE_OK;
   FBSE_HIP(c, hipSetDevice(c->device));
   if (precision == FBSE_PREC_FP64) return filter_impl<double>(c, d_psi_rank, n_starts, ms, M, an, zn, Vmin, dE_1);
@@ -1251,7 +1250,6 @@ extern "C" int fbse_filter_many(int n_dots, fbse_ctx *const *ctx, double *const 
     for (int e = 0; e < d; ++e)
       if (ctx[e] == ctx[d]) return set_err(ctx[0], FBSE_ERR_ARG, "dots %d and %d share a context", e, d);
   }
-  if (ms > 16) return set_err(ctx[0], FBSE_ERR_UNSUPPORTED, "ms = %ld > 16 energy targets per start", ms);
   FBSE_HIP(ctx[0], hipSetDevice(ctx[0]->device));
   if (precision == FBSE_PREC_FP64)
     return filter_many_impl<double>(n_dots, ctx, d_psi_rank, n_starts, ms, M, an, zn, Vmin, dE_1);

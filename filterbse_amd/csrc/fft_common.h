@@ -38,15 +38,25 @@ template <typename C, typename T> __device__ __forceinline__ C cmul_i(C a, T s) 
 // contribution of one projector record to its point:
 // Y_m (proj d_so[m] + nl_proj d_nl[m]) summed over m (phamiltonian.c:140-444)
 template <typename T>
-__device__ __forceinline__ cplx_t<T> record_term(const T *v, const cplx_t<T> *dd) {
-  // the record (8 values) and the coefficients (6 complex) are 16-byte
-  // aligned: read them as 16-byte vectors
-  T r[8];
-  cplx_t<T> d[6];
+__device__ __forceinline__ void load_record(const T *v, T (&r)[8]) {
+  // the record (8 values) is 16-byte aligned: read it as 16-byte vectors
   if constexpr (sizeof(T) == 4) {
     const float4 *v4 = reinterpret_cast<const float4 *>(v);
     const float4 p0 = v4[0], p1 = v4[1];
     r[0] = p0.x, r[1] = p0.y, r[2] = p0.z, r[3] = p0.w, r[4] = p1.x, r[5] = p1.y, r[6] = p1.z, r[7] = p1.w;
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) r[q] = v[q];
+  }
+}
+
+// One projector record's term of the expansion, sum_m Y_m (w_so d[m] + w_nl d[3+m]),
+// for a record r[8] = (w_so, w_nl, Re/Im Y_m) and the 6 coefficients dd
+// (16-byte aligned).
+template <typename T>
+__device__ __forceinline__ cplx_t<T> record_term_r(const T (&r)[8], const cplx_t<T> *dd) {
+  cplx_t<T> d[6];
+  if constexpr (sizeof(T) == 4) {
     const float4 *d4 = reinterpret_cast<const float4 *>(dd);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -55,8 +65,6 @@ __device__ __forceinline__ cplx_t<T> record_term(const T *v, const cplx_t<T> *dd
       d[2 * q + 1] = cplx_t<T>{t.z, t.w};
     }
   } else {
-#pragma unroll
-    for (int q = 0; q < 8; ++q) r[q] = v[q];
 #pragma unroll
     for (int q = 0; q < 6; ++q) d[q] = dd[q];
   }
@@ -71,6 +79,13 @@ __device__ __forceinline__ cplx_t<T> record_term(const T *v, const cplx_t<T> *dd
     acc.y += yx * sy + yy * sx;
   }
   return acc;
+}
+
+template <typename T>
+__device__ __forceinline__ cplx_t<T> record_term(const T *v, const cplx_t<T> *dd) {
+  T r[8];
+  load_record<T>(v, r);
+  return record_term_r<T>(r, dd);
 }
 
 // Butterfly of radix R on values x[0..R-1] (already twiddled); sgn = -1 forward.

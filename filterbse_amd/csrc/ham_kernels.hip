@@ -1,6 +1,8 @@
 // ham_kernels.hip -- projector (SO / NL) application, windowed Newton
 // accumulation and per-state reductions for gfx950.
 #include "fbse_internal.h"
+
+#include <algorithm>
 #include "fft_common.h"
 
 namespace fbse {
@@ -324,20 +326,28 @@ int proj_project(fbse_ctx *c, const cplx_t<T> *psi_in, long nstates, double a1, 
 // pass's rounds; the records of one rank touch distinct points, and ranks run
 // as separate launches in order, so every point sums its atoms in index order.
 template <typename T>
-__global__ void proj_expand_rank_kernel(const int *rec_off, const int *rec_pa, const T *rec_val, int ranks, int k,
-                                        const cplx_t<T> *d, int natoms, cplx_t<T> *out, long item_stride, long nx,
-                                        long nxny) {
+__global__ __launch_bounds__(256) void proj_expand_rank_kernel(const int *rec_off, const int *rec_pa,
+                                                               const T *rec_val, int ranks, int k,
+                                                               const cplx_t<T> *d, int natoms, cplx_t<T> *out,
+                                                               long item_stride, long nx, long nxny, int nitems) {
+  // one thread per record of rank k in plane z, every batch item in turn:
+  // the record is read once, not once per item
   const int z = blockIdx.y;
-  const long item = blockIdx.z;
   const int e = rec_off[z * ranks + k] + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (e >= rec_off[z * ranks + k + 1]) return;
   const int2 q = reinterpret_cast<const int2 *>(rec_pa)[e];
-  const cplx_t<T> t = record_term<T>(rec_val + (long)e * 8, d + (item * natoms + q.y) * 6);
-  cplx_t<T> *o = out + item * item_stride + z * nxny + (long)(q.x >> 16) * nx + (q.x & 0xffff);
-  cplx_t<T> v = *o;
-  v.x += t.x;
-  v.y += t.y;
-  *o = v;
+  T r[8];
+  load_record<T>(rec_val + (long)e * 8, r);
+  cplx_t<T> *o = out + z * nxny + (long)(q.x >> 16) * nx + (q.x & 0xffff);
+  const cplx_t<T> *dd = d + (long)q.y * 6;
+#pragma unroll 4
+  for (int item = 0; item < nitems; ++item) {
+    const cplx_t<T> t = record_term_r<T>(r, dd + (long)item * natoms * 6);
+    cplx_t<T> v = o[item * item_stride];
+    v.x += t.x;
+    v.y += t.y;
+    o[item * item_stride] = v;
+  }
 }
 
 template <typename T>
@@ -353,9 +363,9 @@ int proj_expand(fbse_ctx *c, const cplx_t<T> *d, cplx_t<T> *out, long nstates) {
   for (int k = 0; k < c->pt_ranks; ++k) {
     const int seg = c->rec_seg_max[k];
     if (seg == 0) continue;
-    const dim3 g((unsigned)((seg + 255) / 256), (unsigned)c->nz, (unsigned)nb);
+    const dim3 g((unsigned)((seg + 255) / 256), (unsigned)c->nz);
     hipLaunchKernelGGL((proj_expand_rank_kernel<T>), g, dim3(256), 0, c->stream, c->d_rec_off, c->d_rec_pa, val,
-                       c->pt_ranks, k, d, c->pj.n_atoms, out, c->ngrid, c->nx, c->nx * c->ny);
+                       c->pt_ranks, k, d, c->pj.n_atoms, out, c->ngrid, c->nx, c->nx * c->ny, (int)nb);
   }
   FBSE_HIP(c, hipGetLastError());
   prof_end(c, 2, ev);
@@ -570,14 +580,15 @@ constexpr int kAccGroup = 4;  // ring slots loaded per group in accumulate_kerne
 template <typename T, bool REAL_A>
 __global__ __launch_bounds__(256) void accumulate_kernel(const cplx_t<T> *__restrict__ ring, int ring_slots,
                                                          long slot_stride, long j_lo, int nj,
-                                                         const double *__restrict__ an, int ms,
+                                                         const double *__restrict__ an, int ms, int t0, int mc,
                                                          double *__restrict__ psi_rank, long n_starts, long n_items,
                                                          int packed, long nsg, const long *__restrict__ jlo_dev) {
   const long item = blockIdx.y;  // complex: start state; packed: start pair
   if (jlo_dev) j_lo = *jlo_dev;
   const long slot0 = j_lo % ring_slots;
   // REAL_A: anT[(j_lo + j)*ms + t]; else anT[((j_lo + j)*ms + t)*2 + {0,1}]
-  const double *a_win = an + (REAL_A ? 1 : 2) * j_lo * ms;
+  // this launch accumulates targets t0 .. t0 + mc - 1 (mc <= kMaxMs) of the ms
+  const double *a_win = an + (REAL_A ? 1 : 2) * (j_lo * ms + t0);
   for (long g = blockIdx.x * (long)blockDim.x + threadIdx.x; g < nsg; g += (long)gridDim.x * blockDim.x) {
     const cplx_t<T> *src = ring + item * nsg + g;
     double2 acc[kMaxMs];
@@ -607,7 +618,7 @@ __global__ __launch_bounds__(256) void accumulate_kernel(const cplx_t<T> *__rest
           const double *aj = a_win + (REAL_A ? 1 : 2) * (long)(j0 + u) * ms;
 #pragma unroll
           for (int t = 0; t < kMaxMs; ++t) {
-            if (t < ms) {
+            if (t < mc) {
               const double ar = REAL_A ? aj[t] : aj[2 * t];
               if (REAL_A) {
                 acc[t].x += ar * vx;
@@ -628,8 +639,8 @@ __global__ __launch_bounds__(256) void accumulate_kernel(const cplx_t<T> *__rest
       const long stlen = 2 * nsg;
 #pragma unroll
       for (int t = 0; t < kMaxMs; ++t) {
-        if (t < ms) {
-          double2 *dst = reinterpret_cast<double2 *>(psi_rank + (item * ms + t) * stlen + 2 * g);
+        if (t < mc) {
+          double2 *dst = reinterpret_cast<double2 *>(psi_rank + (item * ms + t0 + t) * stlen + 2 * g);
           double2 o = *dst;
           o.x += acc[t].x;
           o.y += acc[t].y;
@@ -640,9 +651,9 @@ __global__ __launch_bounds__(256) void accumulate_kernel(const cplx_t<T> *__rest
       const long stlen = nsg;
 #pragma unroll
       for (int t = 0; t < kMaxMs; ++t) {
-        if (t < ms) {
-          psi_rank[((2 * item) * ms + t) * stlen + g] += acc[t].x;
-          if (2 * item + 1 < n_starts) psi_rank[((2 * item + 1) * ms + t) * stlen + g] += acc[t].y;
+        if (t < mc) {
+          psi_rank[((2 * item) * ms + t0 + t) * stlen + g] += acc[t].x;
+          if (2 * item + 1 < n_starts) psi_rank[((2 * item + 1) * ms + t0 + t) * stlen + g] += acc[t].y;
         }
       }
     }
@@ -655,19 +666,24 @@ int accumulate_window(fbse_ctx *c, const cplx_t<T> *ring, int ring_slots, long s
                       int packed, int real_coef, const long *jlo_dev) {
   const int nj = (int)(j_hi - j_lo);
   if (nj <= 0) return FBSE_OK;
-  if (nj > kMaxWin || ms > kMaxMs)
-    return set_err(c, FBSE_ERR_UNSUPPORTED, "window %d > %d or ms %ld > %d", nj, kMaxWin, ms, kMaxMs);
+  if (nj > kMaxWin) return set_err(c, FBSE_ERR_UNSUPPORTED, "window %d > %d", nj, kMaxWin);
   hipEvent_t ev = nullptr;
   prof_begin(c, 3, &ev);
   long blocks = (c->nsg + 255) / 256;
   if (blocks > 4096) blocks = 4096;
   const dim3 grid((unsigned)blocks, (unsigned)n_items);
-  if (real_coef || packed)
-    hipLaunchKernelGGL((accumulate_kernel<T, true>), grid, dim3(256), 0, c->stream, ring, ring_slots, slot_stride,
-                       j_lo, nj, d_an, (int)ms, psi_rank, n_starts, n_items, packed, c->nsg, jlo_dev);
-  else
-    hipLaunchKernelGGL((accumulate_kernel<T, false>), grid, dim3(256), 0, c->stream, ring, ring_slots, slot_stride,
-                       j_lo, nj, d_an, (int)ms, psi_rank, n_starts, n_items, packed, c->nsg, jlo_dev);
+  // up to kMaxMs targets per launch (their accumulators live in registers);
+  // more targets per start take further launches over the same window
+  for (long t0 = 0; t0 < ms; t0 += kMaxMs) {
+    const int mc = (int)std::min<long>(kMaxMs, ms - t0);
+    if (real_coef || packed)
+      hipLaunchKernelGGL((accumulate_kernel<T, true>), grid, dim3(256), 0, c->stream, ring, ring_slots, slot_stride,
+                         j_lo, nj, d_an, (int)ms, (int)t0, mc, psi_rank, n_starts, n_items, packed, c->nsg, jlo_dev);
+    else
+      hipLaunchKernelGGL((accumulate_kernel<T, false>), grid, dim3(256), 0, c->stream, ring, ring_slots,
+                         slot_stride, j_lo, nj, d_an, (int)ms, (int)t0, mc, psi_rank, n_starts, n_items, packed,
+                         c->nsg, jlo_dev);
+  }
   FBSE_HIP(c, hipGetLastError());
   prof_end(c, 3, ev);
   return FBSE_OK;

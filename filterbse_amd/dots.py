@@ -393,9 +393,12 @@ CONFIGS = {
                        ncheby=512, targets=(4, 4, -0.31, -0.24, -0.15, -0.06)),
     "c1_so_96": dict(radius_nm=2.0, n_req=(96, 96, 96), dx=0.9, so=True, nl=True, n_starts=64, ms=16,
                      ncheby=1024, targets=(10, 6, -0.31, -0.24, -0.15, -0.06)),
-    # 3.5 nm dot, 160^3, 128 filtered states (8 starts x 16 targets), Newton length 2048
+    # 3.5 nm dot, 160^3, 128 filtered states (8 starts x 16 targets), Newton
+    # length 2048; KEmax 5 Ha and targets placed as for c4 around this dot's
+    # band edges (-0.2490 / -0.1593 Ha), spaced 0.0134 Ha >= 2 sigma_f
     "c3_so_160": dict(radius_nm=3.5, n_req=(160, 160, 160), dx=0.89, so=True, nl=True, n_starts=8, ms=16,
-                      ncheby=2048, targets=(10, 6, -0.31, -0.24, -0.15, -0.06), builder="gpu"),
+                      ncheby=2048, ke_max=5.0, targets=(8, 8, -0.31416, -0.20696, -0.18832, -0.08112),
+                      builder="gpu"),
     # 4.5 nm dot, 256^3 (dx 0.71: the grid rule keeps 256), 256 filtered states
     # (16 starts x 16 targets), Newton length 2048.  KEmax = 5 Ha (the input
     # key KEmax, read.c:385-387) gives dE = 7.54 Ha and a filter width

@@ -54,6 +54,21 @@ def filter_inputs(s, n_starts, ms, M):
     return psi_rank, stlen, an, zn, Vmin, 4.0 / dE
 
 
+@pytest.mark.parametrize("precision,tol", [(0, 1e-10), (1, 3e-5)])
+def test_filter_more_than_16_targets(precision, tol):
+    """ms = 21 energy targets per start (the accumulation takes them in
+    launches of at most 16): FP64 and mixed filters against the oracle."""
+    s = small_system(**VARIANTS["spinor_so_nl_1p"])
+    h, g = oracle_ham(s), gpu_context(s)
+    n_starts, ms, M = 2, 21, 30
+    psi_rank, stlen, an, zn, Vmin, dE_1 = filter_inputs(s, n_starts, ms, M)
+    ref = h.filter_cycles(psi_rank, n_starts, ms, an, zn, Vmin, dE_1)
+    g.set_window(4)
+    out = g.filter(psi_rank, n_starts, ms, an, zn, Vmin, dE_1, precision=precision)
+    g.close()
+    assert rel(out, ref) < tol
+
+
 @pytest.mark.parametrize("window", [3, 0])
 def test_filter_fp64(system, window):
     name, s, h, g = system
