@@ -426,7 +426,8 @@ def bench_uneven(args, dev):
     for i, r in enumerate(radii):
         # the grid the reference's rule needs, rounded up to FFT-friendly axes
         n_req = dots.friendly_grid(float(r), cfg["dx"], seed=100 + i)
-        job = dots.build_dot(float(r), n_req, cfg["dx"], so=True, nl=True, seed=100 + i, builder="gpu")
+        job = dots.build_dot(float(r), n_req, cfg["dx"], so=True, nl=True, seed=100 + i,
+                             ke_max=cfg.get("ke_max", 20.0), builder="gpu")
         ctx = api.Context(job["nx"], job["ny"], job["nz"], job["nspin"], job["is_complex"], job["dv"], job["ksqr"],
                           job["pot_local"], job["proj"])
         t0 = time.perf_counter()

@@ -415,7 +415,11 @@ CONFIGS = {
 # Batched uneven dots: 32 spin-orbit dots with seeded radii uniform in
 # 1.0-3.0 nm at dx = 1.0 bohr (the reference grid rule needs ~52^3 ... ~120^3;
 # each axis is rounded up to a size with prime factors 2, 3, 5), 32 starts x 8
-# targets each, Newton length 1024, mixed precision.
-UNEVEN = dict(n_dots=32, r_lo=1.0, r_hi=3.0, dx=1.0, n_starts=32, ms=8, ncheby=1024,
-              targets=(4, 4, -0.32, -0.22, -0.14, -0.04), seed=2, group_max=8)
+# targets each, Newton length 1024, mixed precision.  KEmax 5 Ha (dE about
+# 7.5 Ha, sigma_f = 0.013 Ha at M = 1024) and targets spaced 0.027 Ha >=
+# 2 sigma_f (the reference's filter-width check, coeff.c:30-56): 4 from the
+# gap above the valence-band edge down, 4 from the gap below the
+# conduction-band edge up.
+UNEVEN = dict(n_dots=32, r_lo=1.0, r_hi=3.0, dx=1.0, n_starts=32, ms=8, ncheby=1024, ke_max=5.0,
+              targets=(4, 4, -0.318, -0.21, -0.183, -0.075), seed=2, group_max=8)
 
