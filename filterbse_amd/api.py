@@ -85,6 +85,7 @@ GPU_SIGNATURES = {
     "fbse_set_fft_kernel": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "fbse_fft_choice": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "fbse_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "fbse_set_axis_fusion": (C.c_int, [C.c_void_p, C.c_int]),
     "fbse_build_local_pot": (C.c_int, [C.c_int, C.c_long, DP, DP, DP, C.c_long, C.c_long, C.c_long, DP,
                                        C.POINTER(C.c_int), C.c_long, DP, DP, LP, DP, C.c_long, C.c_long, DP]),
     "fbse_count_nl_gridpts": (C.c_long, [C.c_int, DP, C.c_long, DP, DP, DP, C.c_long, C.c_long, C.c_long,
@@ -450,6 +451,11 @@ class Context:
     def set_graph(self, enable):
         """Newton-window hipGraph replay on (1, default) or off (0)."""
         self._check(gpu().fbse_set_graph(self.ctx, int(enable)))
+
+    def set_axis_fusion(self, enable):
+        """Fused per-axis combine (projector expansion and the next forward x
+        transform inside the x combine pass) on (1, default) or off (0)."""
+        self._check(gpu().fbse_set_axis_fusion(self.ctx, int(enable)))
 
     def trim(self):
         """Release the context's work buffers (allocated again on demand)."""

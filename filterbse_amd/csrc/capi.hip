@@ -377,8 +377,19 @@ static int build_projectors(fbse_ctx *c, const fbse_proj_desc *pd) {
         v[3 + 2 * m] = y[2 * (m * npts + idx) + 1];
       }
     }
+    // row offsets inside each bucket (records of a bucket are in point order,
+    // so their rows ascend)
+    std::vector<int> yoff((size_t)c->nz * nr * (c->ny + 1));
+    for (size_t b = 0; b + 1 < roff.size(); ++b) {
+      int e = roff[b];
+      for (long y = 0; y <= c->ny; ++y) {
+        while (e < roff[b + 1] && (pa[2 * (size_t)e] >> 16) < y) ++e;
+        yoff[b * (c->ny + 1) + y] = e;
+      }
+    }
     auto pef = cast_vec<float>(pe.data(), (long)pe.size());
-    if ((rc = upload(c, (void **)&c->d_rec_off, roff.data(), roff.size() * sizeof(int))) ||
+    if ((rc = upload(c, (void **)&c->d_rec_yoff, yoff.data(), yoff.size() * sizeof(int))) ||
+        (rc = upload(c, (void **)&c->d_rec_off, roff.data(), roff.size() * sizeof(int))) ||
         (rc = upload(c, (void **)&c->d_rec_pa, pa.data(), pa.size() * sizeof(int))) ||
         (rc = upload(c, (void **)&c->d_rec_val, pe.data(), pe.size() * 8)) ||
         (rc = upload(c, (void **)&c->d_rec_val_f, pef.data(), pef.size() * 4)))
@@ -558,7 +569,7 @@ extern "C" void fbse_destroy(fbse_ctx *c) {
                   c->d_ksqr_f, c->d_scratch, c->d_scratch2, c->d_red, c->pj.a_off, c->pj.a_cnt, c->pj.jxyz,
                   c->pj.sign, c->pj.proj, c->pj.nlproj, c->pj.y1, c->pj.ls, c->pj_f.proj, c->pj_f.nlproj,
                   c->pj_f.y1, c->pj_f.ls, c->pj.color_atoms, c->pj.spatial_atoms, c->d_step, c->d_rec_off, c->d_rec_pa,
-                  c->d_rec_val, c->d_rec_val_f, c->d_pjd, c->d_gram};
+                  c->d_rec_val, c->d_rec_val_f, c->d_pjd, c->d_gram, c->d_rec_yoff};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (int a = 0; a < 3; ++a) {
@@ -628,6 +639,11 @@ extern "C" int fbse_set_slab_split(fbse_ctx *c, int split) {
 extern "C" int fbse_set_graph(fbse_ctx *c, int enable) {
   if (!c || enable < 0 || enable > 1) return FBSE_ERR_ARG;
   c->use_graph = enable;
+  return FBSE_OK;
+}
+extern "C" int fbse_set_axis_fusion(fbse_ctx *c, int enable) {
+  if (!c || !(enable == 0 || enable == 1 || enable == 16 || enable == 32 || enable == 64)) return FBSE_ERR_ARG;
+  c->axis_fuse = enable;
   return FBSE_OK;
 }
 extern "C" int fbse_trim(fbse_ctx *c) {

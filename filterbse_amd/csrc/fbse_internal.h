@@ -67,6 +67,23 @@ template <typename T> struct AxisArgs {
   // preconditioner (z pass): g = pa / (1 + k2 * pb)
   T pa, pb;
   long ntiles;             // tiles of the launch (fixed-schedule instances loop over them)
+  // combine of the fixed instances with the projector expansion of the tile's
+  // rows (rec_yoff != nullptr; ny a multiple of the tile's lines): in rounds
+  // k < pt_ranks, the records of plane z, rank k and rows y0 .. y0+L-1 are
+  // rec_yoff[(z*pt_ranks + k)*(ny+1) + y0] .. rec_yoff[.. + y0 + L] - 1 (record
+  // layout of fbse_ctx::d_rec_pa / d_rec_val), the coefficients of batch item
+  // i pjd[(i*pj_natoms + atom)*6 + r]
+  const int *rec_yoff, *rec_pa;
+  const T *rec_val;
+  const cplx_t<T> *pjd;
+  int pj_natoms, pt_ranks, ny, nz;
+  // combine: when set, the forward x transform of the output rows is also
+  // written here (the first sweep of the next Newton step)
+  cplx_t<T> *fwd_dst;
+  // combine: tile order in which the tiles of one row block (all batch items)
+  // follow each other on one XCD (work-group b runs on XCD b mod 8), so the
+  // row block's records are fetched into one L2 once
+  int xcd_order;
 };
 
 enum PassKind { kPassFwd = 0, kPassBwd = 1, kPassKin = 2, kPassBwdCombine = 3, kPassPrec = 4 };
@@ -159,10 +176,20 @@ struct fbse_ctx {
   double *d_rec_val = nullptr;
   float *d_rec_val_f = nullptr;
   int pt_ranks = 0;
+  // per (plane z, rank k, row y): the first record of bucket (z, k) whose row
+  // is >= y, at d_rec_yoff[(z*pt_ranks + k)*(ny+1) + y] (y = ny: the bucket's
+  // end); the per-axis combine finds a tile's records with it
+  int *d_rec_yoff = nullptr;
   std::vector<int> rec_seg_max;  // per rank: the most records one plane holds
   void *d_pjd = nullptr;        // mixed projection coefficients (column x atom x 6), grown on demand
   size_t pjd_bytes = 0;
   int fuse_proj = 1;            // expansion inside the plane passes when the tables allow it
+  // per-axis passes on axes of length 160 / 256: the combine adds the
+  // projector expansion and writes the next step's forward x transform
+  // (fbse_set_axis_fusion); ax_pjd / ax_fwd carry one combine's operands
+  int axis_fuse = 1;
+  const void *ax_pjd = nullptr;
+  void *ax_fwd = nullptr;
 
   // scratch (grown on demand)
   void *d_scratch = nullptr;

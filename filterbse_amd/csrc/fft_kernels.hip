@@ -389,6 +389,26 @@ static int fft_pass_lines(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src,
   }
   A.lines = lines;
   A.ntiles = ntiles;
+  A.ny = (int)c->ny;
+  A.nz = (int)c->nz;
+  if (kind == kPassBwdCombine && (c->ax_pjd || c->ax_fwd)) {
+    // the fused combine: fixed instances only, tiles within one xy plane
+    if (!fx || c->ny % lines != 0)
+      return set_err(c, FBSE_ERR_UNSUPPORTED, "fused combine needs a fixed instance with ny %% lines == 0");
+    if (c->ax_pjd) {
+      A.rec_yoff = c->d_rec_yoff;
+      A.rec_pa = c->d_rec_pa;
+      if constexpr (sizeof(T) == 8) A.rec_val = c->d_rec_val;
+      else A.rec_val = c->d_rec_val_f;
+      A.pjd = (const cplx_t<T> *)c->ax_pjd;
+      A.pj_natoms = c->pj.n_atoms;
+      A.pt_ranks = c->pt_ranks;
+    }
+    A.fwd_dst = (cplx_t<T> *)c->ax_fwd;
+  }
+  // the combine's tiles of one row block for all batch items on one XCD: its
+  // V_loc rows (and expansion records) are then fetched into one L2 once
+  if (kind == kPassBwdCombine && fx && c->ny % lines == 0) A.xcd_order = (c->ny * c->nz / lines) % 8 == 0 ? 1 : 0;
   if (fx) {
     hipEvent_t ev = nullptr;
     const int pk = (kind == kPassBwdCombine) ? 1 : 0;
@@ -620,10 +640,60 @@ int apply_h_cplx(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *w
   return FBSE_OK;
 }
 
+// Per-axis passes with the fused combine: the x combine pass (fixed 16-line
+// instance) adds the projector expansion and writes the forward x transform
+// of its output into the work buffer, where the next Newton step's y pass
+// starts (first = true: this step's forward x pass runs first).  For axis
+// lengths with a fixed instance (x) and ny a multiple of 16, with no
+// projectors or projectors the record layout covers, when the per-axis tile
+// heights are automatic or forced to the fixed 16-line instance.
+// lines per tile of the fused combine: fbse_set_axis_fusion's 16, 32 or 64
+// (64: FP32 only), else 16
+template <typename T> static int fused_lines(const fbse_ctx *c) {
+  const int f = c->axis_fuse;
+  return (f == 32 || (f == 64 && sizeof(T) == 4)) ? f : 16;
+}
+template <typename T> static bool axis_fused(const fbse_ctx *c) {
+  // (the combine keeps 2 * pt_ranks rank-segment bounds in its 256 + 16
+  // element k^2 tables, unused by the combine)
+  return c->axis_fuse && axis_fx_length((int)c->nx) && c->ny % fused_lines<T>(c) == 0 &&
+         (!c->has_proj || (proj_fusable(c) && c->pt_ranks <= 128)) &&
+         (c->force_lines == 0 || c->force_lines == 1000 + fused_lines<T>(c));
+}
+
+template <typename T>
+static int apply_h_axes_chain(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *work, long nstates,
+                              T a1, T a0, bool first) {
+  const long nb = nstates * c->nspin;
+  cplx_t<T> *d = nullptr;
+  bool joined = true;
+  int rc;
+  if (c->has_proj) {
+    if ((rc = ensure_pjd<T>(c, nstates))) return rc;
+    d = (cplx_t<T> *)c->d_pjd;
+    if ((rc = project_forked<T>(c, psi, nstates, (double)a1, d, &joined))) return rc;
+  }
+  rc = first ? fft_pass<T>(c, 0, kPassFwd, psi, work, nb, nullptr, 0, 0) : FBSE_OK;
+  if (!rc) rc = fft_pass<T>(c, 1, kPassFwd, work, work, nb, nullptr, 0, 0);
+  if (!rc) rc = fft_pass<T>(c, 2, kPassKin, work, work, nb, nullptr, 0, 0);
+  if (!rc) rc = fft_pass<T>(c, 1, kPassBwd, work, work, nb, nullptr, 0, 0);
+  if (!joined) (void)hipStreamWaitEvent(c->stream, c->ev_join, 0);  // the projections are done
+  if (rc) return rc;
+  c->ax_pjd = d;
+  c->ax_fwd = work;
+  rc = fft_pass_lines<T>(c, 0, kPassBwdCombine, work, out, nb, psi, a1, a0, 1000 + fused_lines<T>(c));
+  c->ax_pjd = nullptr;
+  c->ax_fwd = nullptr;
+  return rc;
+}
+
 template <typename T>
 int apply_h_chain(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *work, long nstates, T a1, T a0,
                   bool first) {
-  if (!use_tiles<T>(c)) return apply_h_cplx<T>(c, psi, out, work, nstates, a1, a0);
+  if (!use_tiles<T>(c)) {
+    if (axis_fused<T>(c)) return apply_h_axes_chain<T>(c, psi, out, work, nstates, a1, a0, first);
+    return apply_h_cplx<T>(c, psi, out, work, nstates, a1, a0);
+  }
   const cplx_t<T> *d = nullptr;
   hipEvent_t join = nullptr;
   int rc;

@@ -750,7 +750,7 @@ FBSE_FX_F64(FBSE_FX_K64)
 // passes, and each work-group loops over tiles.  x tiles are L consecutive
 // rows; y and z tiles are L consecutive inner indices (each axis point a
 // contiguous 8*L- or 16*L-byte segment).
-template <typename T, int KIND, int N, int R1, int R2, int L, int NT, bool XAX>
+template <typename T, int KIND, int N, int R1, int R2, int L, int NT, bool XAX, int EXP_GROUP>
 __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
   using C = cplx_t<T>;
   constexpr int LS = N | 1, E = N * L / NT;
@@ -775,7 +775,16 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
     long base, i0 = 0, line0 = 0;
     int nl;
     if (XAX) {
-      line0 = t * L;
+      long tt = t;
+      if (KIND == kPassBwdCombine && A.xcd_order) {
+        // t -> XCD t mod 8, place s = t / 8 in that XCD's sequence of (row
+        // block, batch item) pairs, row blocks xcd, xcd + 8, ... (the host
+        // sets xcd_order only when the row blocks per item are a multiple of 8)
+        const long rows = (long)A.ny * A.nz, nbat = A.total_lines / rows;
+        const long s = t >> 3, q = s / nbat;
+        tt = (s - q * nbat) * (rows / L) + (t & 7) + 8 * q;
+      }
+      line0 = tt * L;
       base = line0 * N;
       const long rem = A.total_lines - line0;
       nl = rem < L ? (int)rem : L;
@@ -844,6 +853,8 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
     if (KIND == kPassBwdCombine) {  // x axis: out = a1*(T psi + V psi_in) + a0*psi_in
       const T a0 = A.a0_tab ? (T)A.a0_tab[*A.step_base + A.step_off] : A.a0;
       const C *pin = A.psi_in + base;
+      const bool expand = A.rec_yoff != nullptr;
+      const bool keep = expand || A.fwd_dst != nullptr;
 #pragma unroll
       for (int u = 0; u < E; ++u) {
         const int e = tid + u * NT;
@@ -856,7 +867,80 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
           C o;
           o.x = A.a1 * (r.x + vv * q.x) + a0 * q.x;
           o.y = A.a1 * (r.y + vv * q.y) + a0 * q.y;
-          dst[e] = o;
+          if (keep) buf[l * LS + p] = o;
+          if (!expand) dst[e] = o;
+        }
+      }
+      if (expand) {
+        // projector expansion of the tile's rows (one plane z, rows y0 ..
+        // y0+L-1 of batch item `item`) in rounds per rank: the records of one
+        // rank touch distinct points, so every point adds its atoms in index
+        // order, as the separate expansion pass does.  The rank segments of
+        // the tile (seg_start[k], seg_len[k], in the k^2 tables of the LDS,
+        // which the combine does not use) are read once.  Inside a round a
+        // thread takes kG records at a time and issues all their loads before
+        // it adds any term: a record's loads wait on its position and atom,
+        // and a loop of one record per step would wait once per record.
+        constexpr int kG = EXP_GROUP;
+        const long rows = (long)A.ny * A.nz;
+        const long item = line0 / rows, zy = line0 - item * rows;
+        const int z = (int)(zy / A.ny), y0 = (int)(zy - (long)z * A.ny);
+        const C *dcol = A.pjd + item * A.pj_natoms * 6;
+        const int2 *pa = reinterpret_cast<const int2 *>(A.rec_pa);
+        const int nr = A.pt_ranks;
+        int *seg_start = reinterpret_cast<int *>(kzl), *seg_len = seg_start + nr;
+        if (tid < nr) {
+          const int *ro = A.rec_yoff + ((long)z * nr + tid) * (A.ny + 1) + y0;
+          const int s0 = ro[0];
+          seg_start[tid] = s0;
+          seg_len[tid] = ro[L] - s0;
+        }
+        __syncthreads();  // also orders the combine's LDS writes before the rounds
+        for (int k = 0; k < nr; ++k) {
+          const int s0 = seg_start[k], n0 = seg_len[k];
+          if (n0 == 0) continue;
+          for (int b0 = 0; b0 < n0; b0 += kG * NT) {
+            int2 q[kG];
+            C tv[kG];
+#pragma unroll
+            for (int g = 0; g < kG; ++g) {
+              const int i = b0 + g * NT + tid;
+              q[g] = i < n0 ? pa[s0 + i] : int2{-1, 0};
+            }
+#pragma unroll
+            for (int g = 0; g < kG; ++g)
+              if (q[g].x >= 0)
+                tv[g] = record_term<T>(A.rec_val + (long)(s0 + b0 + g * NT + tid) * 8, dcol + (long)q[g].y * 6);
+#pragma unroll
+            for (int g = 0; g < kG; ++g) {
+              if (q[g].x >= 0) {
+                const int pos = ((q[g].x >> 16) - y0) * LS + (q[g].x & 0xffff);
+                C o = buf[pos];
+                o.x += tv[g].x;
+                o.y += tv[g].y;
+                buf[pos] = o;
+              }
+            }
+          }
+          __syncthreads();
+        }
+#pragma unroll 4
+        for (int u = 0; u < E; ++u) {
+          const int e = tid + u * NT;
+          const int l = e / N, p = e % N;
+          if (l < nl) dst[e] = buf[l * LS + p];
+        }
+      }
+      if (A.fwd_dst) {
+        // the next Newton step's forward x transform of these rows
+        __syncthreads();
+        fx_fft<T, N, R1, R2, L, 1, LS, NT, false>(tw_off);
+        C *fd = A.fwd_dst + base;
+#pragma unroll 4
+        for (int u = 0; u < E; ++u) {
+          const int e = tid + u * NT;
+          const int l = e / N, p = e % N;
+          if (l < nl) fd[e] = buf[l * LS + p];
         }
       }
     } else {
@@ -869,23 +953,25 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
   }
 }
 
-#define FBSE_AXIS_FX(NAME, T, N, R1, R2, L, NT, WAVES)                                                      \
+// G: expansion records a thread loads at a time in the fused combine (3 for
+// FP32 at 256: the most that fits the 128 VGPRs of four waves per SIMD)
+#define FBSE_AXIS_FX(NAME, T, N, R1, R2, L, NT, WAVES, G)                                                   \
   template <int KIND, bool XAX>                                                                          \
   __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WAVES))) void NAME(AxisArgs<T> A) {  \
-    axis_fx_body<T, KIND, N, R1, R2, L, NT, XAX>(A);                                                     \
+    axis_fx_body<T, KIND, N, R1, R2, L, NT, XAX, G>(A);                                                  \
   }
-FBSE_AXIS_FX(axis_fx_f32_n256_l16, float, 256, 16, 16, 16, 256, 4)
-FBSE_AXIS_FX(axis_fx_f32_n256_l32, float, 256, 16, 16, 32, 512, 4)
-FBSE_AXIS_FX(axis_fx_f32_n160_l16, float, 160, 16, 10, 16, 256, 4)
-FBSE_AXIS_FX(axis_fx_f32_n160_l32, float, 160, 16, 10, 32, 512, 4)
+FBSE_AXIS_FX(axis_fx_f32_n256_l16, float, 256, 16, 16, 16, 256, 4, 3)
+FBSE_AXIS_FX(axis_fx_f32_n256_l32, float, 256, 16, 16, 32, 512, 4, 3)
+FBSE_AXIS_FX(axis_fx_f32_n160_l16, float, 160, 16, 10, 16, 256, 4, 4)
+FBSE_AXIS_FX(axis_fx_f32_n160_l32, float, 160, 16, 10, 32, 512, 4, 4)
 // 64 lines: the y and z tiles read 512-byte segments per axis point (FP32
 // only; one work-group per CU)
-FBSE_AXIS_FX(axis_fx_f32_n256_l64, float, 256, 16, 16, 64, 1024, 4)
-FBSE_AXIS_FX(axis_fx_f32_n160_l64, float, 160, 16, 10, 64, 1024, 4)
-FBSE_AXIS_FX(axis_fx_f64_n256_l16, double, 256, 16, 16, 16, 256, 2)
-FBSE_AXIS_FX(axis_fx_f64_n256_l32, double, 256, 16, 16, 32, 512, 2)
-FBSE_AXIS_FX(axis_fx_f64_n160_l16, double, 160, 16, 10, 16, 256, 2)
-FBSE_AXIS_FX(axis_fx_f64_n160_l32, double, 160, 16, 10, 32, 512, 2)
+FBSE_AXIS_FX(axis_fx_f32_n256_l64, float, 256, 16, 16, 64, 1024, 4, 3)
+FBSE_AXIS_FX(axis_fx_f32_n160_l64, float, 160, 16, 10, 64, 1024, 4, 4)
+FBSE_AXIS_FX(axis_fx_f64_n256_l16, double, 256, 16, 16, 16, 256, 2, 4)
+FBSE_AXIS_FX(axis_fx_f64_n256_l32, double, 256, 16, 16, 32, 512, 2, 4)
+FBSE_AXIS_FX(axis_fx_f64_n160_l16, double, 160, 16, 10, 16, 256, 2, 4)
+FBSE_AXIS_FX(axis_fx_f64_n160_l32, double, 160, 16, 10, 32, 512, 2, 4)
 #undef FBSE_AXIS_FX
 
 bool axis_fx_length(int n) { return n == 160 || n == 256; }

@@ -240,6 +240,16 @@ int fbse_set_slab_split(fbse_ctx *ctx, int split);
  * identical.  No counterpart in the reference. */
 int fbse_set_graph(fbse_ctx *ctx, int enable);
 
+/* Per-axis passes on an x axis of length 160 or 256 with ny a multiple of 16
+ * (default on): the Newton filter's x combine pass (backward x transform,
+ * V_loc, Newton shift) also adds the projector expansion of its rows and
+ * writes the forward x transform of its output, which the next Newton step
+ * starts from; 0 runs the expansion and the forward x transform as passes of
+ * their own.  Results are identical.  No counterpart in the reference (its
+ * kinetic and projector terms are separate loops, hamiltonian.c:62-99,
+ * phamiltonian.c:140-444). */
+int fbse_set_axis_fusion(fbse_ctx *ctx, int enable);
+
 /* Releases the context's work buffers (the filter's ring of Newton terms,
  * FFT work space); they are allocated again on demand.  For callers that keep
  * many contexts alive (one per dot) and filter them in turns. */

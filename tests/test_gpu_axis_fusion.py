@@ -1,0 +1,50 @@
+"""The fused per-axis combine (fbse_set_axis_fusion): on x axes of length 160
+and 256 the Newton filter's x combine pass also adds the projector expansion
+of its rows and writes the forward x transform the next step starts from.
+With every per-axis pass forced to the fixed 16-line instance, the fused and
+the separate passes run the same arithmetic in the same order, so the
+filtered states must be bit-identical; the FP64 filter must also match the
+oracle.  Windows are forced short so that graph replays carry the chained
+forward transform from one window into the next.  Grids: 160 x 32 x 48 (the
+combine's row blocks per item, 96, a multiple of 8: XCD tile order on) and
+256 x 16 x 20 (20 row blocks: plain order), spinors with SO + NL projectors
+and packed real states with NL projectors."""
+import numpy as np
+import pytest
+
+from systems import gpu_context, oracle_ham, small_system
+from test_gpu_parity import filter_inputs
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "spinor_160x32x48": dict(n=(160, 32, 48), dx=0.5, nspin=2, is_complex=1, so=1, nl=1, natoms=6, n_so_proj=1,
+                             n_nl_proj=1),
+    "spinor_256x16x20": dict(n=(256, 16, 20), dx=0.4, nspin=2, is_complex=1, so=1, nl=1, natoms=5, n_so_proj=1,
+                             n_nl_proj=1),
+    "real_256x16x20": dict(n=(256, 16, 20), dx=0.4, nspin=1, is_complex=0, so=0, nl=1, natoms=5, n_nl_proj=1),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fused_combine_identical(case, precision):
+    s = small_system(**CASES[case])
+    print("case axis fusion %s precision %d" % (case, precision), flush=True)
+    g = gpu_context(s)
+    g.set_fft_kernel(0, 1016)
+    g.set_window(4)
+    n_starts, ms, M = 3, 4, 40
+    psi_rank, stlen, an, zn, Vmin, dE_1 = filter_inputs(s, n_starts, ms, M)
+    g.set_axis_fusion(0)
+    sep = g.filter(psi_rank, n_starts, ms, an, zn, Vmin, dE_1, precision=precision)
+    g.set_axis_fusion(1)
+    fused = g.filter(psi_rank, n_starts, ms, an, zn, Vmin, dE_1, precision=precision)
+    assert g.graph_replays() > 0
+    assert np.array_equal(fused, sep)
+    if precision == 0:
+        ref = oracle_ham(s).filter_cycles(psi_rank, n_starts, ms, an, zn, Vmin, dE_1)
+        err = np.abs(fused - ref).max() / np.abs(ref).max()
+        print("fused FP64 filter vs oracle: %.2e" % err)
+        assert err < 1e-10
+    g.close()
