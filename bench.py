@@ -449,8 +449,11 @@ def bench_uneven(args, dev):
     # groups that fit: psi_rank (FP64) + about four FP32 ring slots per dot
     free, _ = torch.cuda.mem_get_info()
     budget = 0.55 * free
+    # dots in grid-size order, so that dots of one size share a filter_many
+    # call and with it the grouped FFT launches of their size
     groups, cur, used = [], [], 0.0
-    for k, d in enumerate(D):
+    for k in sorted(range(len(D)), key=lambda i: (D[i]["n"], i)):
+        d = D[k]
         need = ns * ms * d["stlen"] * 8 + 4 * ns * d["stlen"] * 4
         if cur and (used + need > budget or len(cur) == cfg["group_max"]):
             groups.append(cur)
@@ -477,6 +480,8 @@ def bench_uneven(args, dev):
                         [D[k]["an"] for k in grp], [zn] * len(grp), [D[k]["Vmin"] for k in grp],
                         [4.0 / D[k]["dE"] for k in grp], mixed)
         t_filter += time.perf_counter() - tf
+        for k in grp:
+            D[k]["launch_group"] = D[k]["ctx"].last_group()
         for k, p in zip(grp, psis):
             c = D[k]["ctx"]
             c.normalize_all_dev(p.data_ptr(), ns * ms)
@@ -510,14 +515,17 @@ def bench_uneven(args, dev):
                                "each, Newton length %d" % (len(D), min(radii), max(radii),
                                                            "%d^3..%d^3" % (min(d["n"][0] for d in D),
                                                                           max(d["n"][0] for d in D)), ns, ms, M),
-                   "parallelism": "single GPU; dots interleaved over per-dot streams, %d group(s)" % len(groups)},
+                   "parallelism": "single GPU; %d filter_many call(s); dots of one grid size share grouped FFT "
+                                  "launches (%d dots in %d launch groups), others interleaved over per-dot streams" % (
+                                      len(groups), sum(1 for d in D if d["launch_group"] > 1),
+                                      round(sum(1.0 / d["launch_group"] for d in D if d["launch_group"] > 1)))},
         "converged": conv_total,
         "hpsi_per_s": round(hpsi, 1),
         "grid_points_per_s": round(pts, 1),
         "filter_s": round(t_filter, 3),
         "energy_range_s": round(t_range, 2),
-        "dots": [dict(r_nm=round(d["r"], 3), grid=d["n"], atoms=d["atoms"], rank=d["rank"], converged=d["conv"])
-                 for d in D],
+        "dots": [dict(r_nm=round(d["r"], 3), grid=d["n"], atoms=d["atoms"], rank=d["rank"], converged=d["conv"],
+                      launch_group=d["launch_group"]) for d in D],
     }
     print(json.dumps(line), flush=True)
 

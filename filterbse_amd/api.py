@@ -86,6 +86,7 @@ GPU_SIGNATURES = {
     "fbse_fft_choice": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "fbse_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "fbse_set_axis_fusion": (C.c_int, [C.c_void_p, C.c_int]),
+    "fbse_last_group": (C.c_int, [C.c_void_p]),
     "fbse_build_local_pot": (C.c_int, [C.c_int, C.c_long, DP, DP, DP, C.c_long, C.c_long, C.c_long, DP,
                                        C.POINTER(C.c_int), C.c_long, DP, DP, LP, DP, C.c_long, C.c_long, DP]),
     "fbse_count_nl_gridpts": (C.c_long, [C.c_int, DP, C.c_long, DP, DP, DP, C.c_long, C.c_long, C.c_long,
@@ -505,6 +506,11 @@ class Context:
     def last_window(self):
         """Accumulation window depth of the last filter call."""
         return gpu().fbse_last_window(self.ctx)
+
+    def last_group(self):
+        """Dots that shared grouped FFT launches with this context in its last
+        filter_many call (1: its own launches)."""
+        return gpu().fbse_last_group(self.ctx)
 
     def graph_replays(self):
         """hipGraph replays of the last filter call."""

@@ -656,6 +656,7 @@ extern "C" int fbse_trim(fbse_ctx *c) {
 }
 extern "C" long fbse_graph_replays(const fbse_ctx *c) { return c ? c->graph_replays : -1; }
 extern "C" int fbse_last_window(const fbse_ctx *c) { return c ? c->last_window : -1; }
+extern "C" int fbse_last_group(const fbse_ctx *c) { return c ? c->last_group : -1; }
 extern "C" int fbse_held_bytes(fbse_ctx *c, size_t *out) {
   if (!c || !out) return FBSE_ERR_ARG;
   out[0] = c->scratch_bytes;
@@ -1075,14 +1076,14 @@ template <typename T> struct FilterRun {
     return FBSE_OK;
   }
 
-  // Newton step j (eager launches), with the window's accumulation when due
-  int step() {
-    const C *in = ring + (size_t)((j - 1) % R) * slot_elems;
-    C *out = ring + (size_t)(j % R) * slot_elems;
-    const double a0 = -(2.0 + zn[j - 1] + c2);
-    int r = apply_h_chain<T>(c, in, out, work, n_items, (T)dE_1, (T)a0, j == 1);
-    if (r) return r;
+  // operands of Newton step j: input and output ring slots, Newton shift
+  const C *step_in() const { return ring + (size_t)((j - 1) % R) * slot_elems; }
+  C *step_out() const { return ring + (size_t)(j % R) * slot_elems; }
+  double step_a0() const { return -(2.0 + zn[j - 1] + c2); }
+  // after step j's H application: the window's accumulation when due, next j
+  int finish_step() {
     if (j + 1 - acc_lo == R || j == M - 1) {
+      int r;
       if ((r = accumulate_window<T>(c, ring, R, slot_elems, acc_lo, j + 1, d_an, ms, M, d_psi_rank, n_starts,
                                     n_items, packed, real_an)))
         return r;
@@ -1091,33 +1092,58 @@ template <typename T> struct FilterRun {
     ++j;
     return FBSE_OK;
   }
+  // Newton step j (eager launches), with the window's accumulation when due
+  int step() {
+    int r = apply_h_chain<T>(c, step_in(), step_out(), work, n_items, (T)dE_1, (T)step_a0(), j == 1);
+    return r ? r : finish_step();
+  }
   bool in_window0() const { return j < R && j < M; }
+
+  // window capture in pieces (a group of runs captures them into one graph):
+  // the device step counter before the capture, the Newton shifts read from
+  // the device table for in-window step i, the window's accumulation and
+  // counter update at the end of it
+  int capture_prologue() {
+    int rc;
+    // counter = R (start of window 1), set by kernels so the stream order holds
+    if ((rc = fill_zero(c, c->d_step, sizeof(long))) || (rc = step_advance(c, c->d_step, R))) return rc;
+    return FBSE_OK;
+  }
+  void capture_step(int i) {
+    c->capturing = 1;
+    c->dyn_a0_tab = d_a0;
+    c->dyn_step_base = c->d_step;
+    c->dyn_step_off = i;
+  }
+  const C *window_in(int i) const { return ring + (size_t)((i + R - 1) % R) * slot_elems; }
+  C *window_out(int i) const { return ring + (size_t)i * slot_elems; }
+  int capture_epilogue() {
+    c->dyn_a0_tab = nullptr;
+    c->dyn_step_base = nullptr;
+    int rc = accumulate_window<T>(c, ring, R, slot_elems, R, 2 * R, d_an, ms, M, d_psi_rank, n_starts, n_items,
+                                  packed, real_an, c->d_step);
+    return rc ? rc : step_advance(c, c->d_step, R);
+  }
+  void capture_done() {
+    c->capturing = 0;
+    c->dyn_a0_tab = nullptr;
+    c->dyn_step_base = nullptr;
+  }
 
   // capture one full window (call after window 0)
   int capture() {
     if (!graph) return FBSE_OK;
     int rc;
-    // counter = R (start of window 1), set by kernels so the stream order holds
-    if ((rc = fill_zero(c, c->d_step, sizeof(long))) || (rc = step_advance(c, c->d_step, R))) return rc;
+    if ((rc = capture_prologue())) return rc;
     FBSE_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int crc = FBSE_OK;
-    c->capturing = 1;
-    c->dyn_a0_tab = d_a0;
-    c->dyn_step_base = c->d_step;
     for (int i = 0; i < R && !crc; ++i) {
-      c->dyn_step_off = i;
-      const C *in = ring + (size_t)((i + R - 1) % R) * slot_elems;
-      C *out = ring + (size_t)i * slot_elems;
-      crc = apply_h_chain<T>(c, in, out, work, n_items, (T)dE_1, (T)0, false);
+      capture_step(i);
+      crc = apply_h_chain<T>(c, window_in(i), window_out(i), work, n_items, (T)dE_1, (T)0, false);
     }
-    c->dyn_a0_tab = nullptr;
-    c->dyn_step_base = nullptr;
-    if (!crc)
-      crc = accumulate_window<T>(c, ring, R, slot_elems, R, 2 * R, d_an, ms, M, d_psi_rank, n_starts, n_items,
-                                 packed, real_an, c->d_step);
-    if (!crc) crc = step_advance(c, c->d_step, R);
+    if (!crc) crc = capture_epilogue();
     hipError_t ee = hipStreamEndCapture(c->stream, &g);
-    c->capturing = 0;
+    capture_done();
     hipError_t ei = (crc || ee != hipSuccess) ? hipSuccess : hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
     if (crc) return crc;
     if (ee != hipSuccess || ei != hipSuccess)
@@ -1126,14 +1152,18 @@ template <typename T> struct FilterRun {
     return FBSE_OK;
   }
   bool replays_left() const { return graph && ge && w_next <= n_full; }
-  int replay() {
-    FBSE_HIP(c, hipGraphLaunch(ge, c->stream));
+  // bookkeeping of one replayed window (the graph's or the group's)
+  void replayed() {
     ++c->graph_replays;
     ++w_next;
     if (w_next > n_full) {  // all full windows queued: the tail continues eagerly
       j = (n_full + 1) * R;
       acc_lo = j;
     }
+  }
+  int replay() {
+    FBSE_HIP(c, hipGraphLaunch(ge, c->stream));
+    replayed();
     return FBSE_OK;
   }
   bool tail_left() const { return j < M; }
@@ -1187,6 +1217,148 @@ static int filter_impl(fbse_ctx *c, double *d_psi_rank, long n_starts, long ms, 
 // Several dots at once: every run on its own context's stream, launches
 // interleaved step by step and window by window so that the dots' kernels
 // overlap on the device's hardware queues.
+// Scheduling units of fbse_filter_many: one dot's run, or a group of runs of
+// one grid size whose FFT passes are grouped launches.
+struct ManyRunner {
+  virtual ~ManyRunner() {}
+  virtual int dot() const = 0;  // index of the (first) dot, for error reports
+  virtual bool in_window0() const = 0;
+  virtual int step() = 0;
+  virtual int capture() = 0;
+  virtual bool replays_left() const = 0;
+  virtual int replay() = 0;
+  virtual bool tail_left() const = 0;
+  virtual int finish() = 0;
+};
+
+template <typename T> struct SingleRunner : ManyRunner {
+  FilterRun<T> *r;
+  int d;
+  SingleRunner(FilterRun<T> *r_, int d_) : r(r_), d(d_) { r->c->last_group = 1; }
+  int dot() const override { return d; }
+  bool in_window0() const override { return r->in_window0(); }
+  int step() override { return r->step(); }
+  int capture() override { return r->capture(); }
+  bool replays_left() const override { return r->replays_left(); }
+  int replay() override { return r->replay(); }
+  bool tail_left() const override { return r->tail_left(); }
+  int finish() override { return r->finish(); }
+};
+
+// Dots of one grid size on the FP32 fixed plane/slab instance: every Newton
+// step issues the dots' projections and then ONE launch per FFT pass for all
+// of them (apply_h_group), on the first dot's stream, which the other dots'
+// streams join for the duration; a full window of all dots is captured into
+// one hipGraph.  The dots share M, the window and the step counter position,
+// so they advance in lockstep.  Results equal the dots' separate runs.
+struct GroupRunner : ManyRunner {
+  std::vector<FilterRun<float> *> m;
+  std::vector<int> idx;
+  std::vector<hipStream_t> saved;
+  hipStream_t gs = nullptr;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  bool graph = false;
+  GroupRunner(std::vector<FilterRun<float> *> runs, std::vector<int> dots) : m(std::move(runs)), idx(std::move(dots)) {}
+  ~GroupRunner() override {
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
+  }
+  fbse_ctx *c0() const { return m[0]->c; }
+  int dot() const override { return idx[0]; }
+  // the members' streams join the first one (after their prepare work)
+  int begin() {
+    gs = c0()->stream;
+    graph = true;
+    for (auto *r : m) {
+      graph = graph && r->graph;
+      r->c->last_group = (int)m.size();
+      saved.push_back(r->c->stream);
+      if (r->c->stream != gs) {
+        FBSE_HIP(c0(), hipEventRecord(r->c->ev_fork, r->c->stream));
+        FBSE_HIP(c0(), hipStreamWaitEvent(gs, r->c->ev_fork, 0));
+        r->c->stream = gs;
+      }
+    }
+    return FBSE_OK;
+  }
+  int apply(bool first, bool window, int i) {
+    const int n = (int)m.size();
+    fbse_ctx *cs[kTileGroupMax];
+    const float2 *in[kTileGroupMax];
+    float2 *out[kTileGroupMax], *work[kTileGroupMax];
+    long ns[kTileGroupMax];
+    float a1[kTileGroupMax], a0[kTileGroupMax];
+    for (int k = 0; k < n; ++k) {
+      FilterRun<float> *r = m[k];
+      cs[k] = r->c;
+      in[k] = window ? r->window_in(i) : r->step_in();
+      out[k] = window ? r->window_out(i) : r->step_out();
+      work[k] = r->work;
+      ns[k] = r->n_items;
+      a1[k] = (float)r->dE_1;
+      a0[k] = window ? 0.f : (float)r->step_a0();
+    }
+    return apply_h_group(n, cs, in, out, work, ns, a1, a0, first);
+  }
+  bool in_window0() const override { return m[0]->in_window0(); }
+  int step() override {
+    int rc = apply(m[0]->j == 1, false, 0);
+    for (size_t k = 0; k < m.size() && !rc; ++k) rc = m[k]->finish_step();
+    return rc;
+  }
+  int capture() override {
+    if (!graph) return FBSE_OK;
+    int rc;
+    for (auto *r : m)
+      if ((rc = r->capture_prologue())) return rc;
+    FBSE_HIP(c0(), hipStreamBeginCapture(gs, hipStreamCaptureModeThreadLocal));
+    int crc = FBSE_OK;
+    const int R = m[0]->R;
+    for (int i = 0; i < R && !crc; ++i) {
+      for (auto *r : m) r->capture_step(i);
+      crc = apply(false, true, i);
+    }
+    for (size_t k = 0; k < m.size() && !crc; ++k) crc = m[k]->capture_epilogue();
+    hipError_t ee = hipStreamEndCapture(gs, &g);
+    for (auto *r : m) r->capture_done();
+    hipError_t ei = (crc || ee != hipSuccess) ? hipSuccess : hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (crc) return crc;
+    if (ee != hipSuccess || ei != hipSuccess)
+      return set_err(c0(), FBSE_ERR_HIP, "grouped Newton window capture: %s",
+                     hipGetErrorString(ee != hipSuccess ? ee : ei));
+    return FBSE_OK;
+  }
+  bool replays_left() const override { return graph && ge && m[0]->w_next <= m[0]->n_full; }
+  int replay() override {
+    FBSE_HIP(c0(), hipGraphLaunch(ge, gs));
+    for (auto *r : m) r->replayed();
+    return FBSE_OK;
+  }
+  bool tail_left() const override { return m[0]->tail_left(); }
+  int finish() override {
+    int rc = FBSE_OK;
+    for (auto *r : m) {
+      int r2 = r->finish();
+      if (r2 && !rc) rc = r2;
+    }
+    // the members' own streams again, ordered after the group's work
+    for (size_t k = 0; k < m.size(); ++k) {
+      fbse_ctx *c = m[k]->c;
+      if (saved[k] != gs) {
+        c->stream = saved[k];
+        if (hipEventRecord(c->ev_fork, gs) != hipSuccess || hipStreamWaitEvent(c->stream, c->ev_fork, 0) != hipSuccess)
+          if (!rc) rc = set_err(c0(), FBSE_ERR_HIP, "grouped filter: stream join failed");
+      }
+    }
+    return rc;
+  }
+};
+
+// Dots that may share grouped launches: FP32 fixed plane/slab instance,
+// projectors absent or expanded in the plane passes.
+static bool groupable(const fbse_ctx *c) { return tile_groupable(c) && (!c->has_proj || proj_fusable(c)); }
+
 template <typename T>
 static int filter_many_impl(int n, fbse_ctx *const *ctx, double *const *psi, const long *n_starts, long ms, long M,
                             const double *const *an, const double *const *zn, const double *Vmin,
@@ -1210,34 +1382,59 @@ static int filter_many_impl(int n, fbse_ctx *const *ctx, double *const *psi, con
     }
   };
   for (int d = 0; d < n && !rc; ++d) fail(d, runs[d]->prepare(K));
+  // scheduling units: FP32 dots of one fixed-instance size in groups of up
+  // to kTileGroupMax (two or more), every other dot on its own
+  std::vector<std::unique_ptr<ManyRunner>> units;
+  std::vector<char> taken(n, 0);
+  if constexpr (sizeof(T) == 4) {
+    for (int d = 0; d < n && !rc; ++d) {
+      if (taken[d] || !groupable(ctx[d]) || ctx[d]->profile) continue;
+      std::vector<FilterRun<float> *> grp;
+      std::vector<int> di;
+      for (int e = d; e < n && (int)grp.size() < kTileGroupMax; ++e)
+        if (!taken[e] && groupable(ctx[e]) && !ctx[e]->profile && ctx[e]->nx == ctx[d]->nx && ctx[e]->ny == ctx[d]->ny &&
+            ctx[e]->nz == ctx[d]->nz && ctx[e]->use_graph == ctx[d]->use_graph) {
+          grp.push_back(runs[e].get());
+          di.push_back(e);
+        }
+      if (grp.size() < 2) continue;
+      for (int e : di) taken[e] = 1;
+      auto *gr = new GroupRunner(grp, di);
+      units.emplace_back(gr);
+      fail(d, gr->begin());
+    }
+  }
+  for (int d = 0; d < n; ++d)
+    if (!taken[d]) units.emplace_back(new SingleRunner<T>(runs[d].get(), d));
+  const int nu = (int)units.size();
   for (bool more = true; more && !rc;) {
     more = false;
-    for (int d = 0; d < n && !rc; ++d)
-      if (runs[d]->in_window0()) {
-        fail(d, runs[d]->step());
+    for (int u = 0; u < nu && !rc; ++u)
+      if (units[u]->in_window0()) {
+        fail(units[u]->dot(), units[u]->step());
         more = true;
       }
   }
-  for (int d = 0; d < n && !rc; ++d) fail(d, runs[d]->capture());
+  for (int u = 0; u < nu && !rc; ++u) fail(units[u]->dot(), units[u]->capture());
   for (bool more = true; more && !rc;) {
     more = false;
-    for (int d = 0; d < n && !rc; ++d)
-      if (runs[d]->replays_left()) {
-        fail(d, runs[d]->replay());
+    for (int u = 0; u < nu && !rc; ++u)
+      if (units[u]->replays_left()) {
+        fail(units[u]->dot(), units[u]->replay());
         more = true;
       }
   }
   for (bool more = true; more && !rc;) {
     more = false;
-    for (int d = 0; d < n && !rc; ++d)
-      if (runs[d]->tail_left()) {
-        fail(d, runs[d]->step());
+    for (int u = 0; u < nu && !rc; ++u)
+      if (units[u]->tail_left()) {
+        fail(units[u]->dot(), units[u]->step());
         more = true;
       }
   }
-  for (int d = 0; d < n; ++d) {
-    int r = runs[d]->finish();
-    if (r) fail(d, r);
+  for (int u = 0; u < nu; ++u) {
+    int r = units[u]->finish();
+    if (r) fail(units[u]->dot(), r);
   }
   if (rc && bad >= 0 && ctx[0] != ctx[bad])
     set_err(ctx[0], rc, "dot %d: %s", bad, ctx[bad]->err.c_str());

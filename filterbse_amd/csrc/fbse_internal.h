@@ -228,6 +228,7 @@ struct fbse_ctx {
   long *d_step = nullptr;       // device step counter of the replayed window
   long graph_replays = 0;       // replays in the last filter
   int last_window = 0;          // accumulation window depth K of the last filter
+  int last_group = 0;           // contexts that shared grouped FFT launches in the last fbse_filter_many (1: own)
   double tune_ms[2][2][3][3] = {};
   int autotune = 1;
   int no_alloc = 0;             // inside fbse_apply_h_dev_noalloc: buffers may not grow
@@ -285,6 +286,17 @@ template <typename T> bool tiles_fixed(const fbse_ctx *c);
 // instance, and the launch of one (lines = 16 or 32; A.ntiles tiles)
 bool axis_fx_length(int n);
 template <typename T> int axis_fx_pass(fbse_ctx *c, int kind, const AxisArgs<T> &A, int lines);
+// Grouped launches of the FP32 fixed-schedule plane/slab passes
+// (fft_tile_kernels.hip): one launch runs pass `kind` (kTileFwdXY, kTileKinZ,
+// kTileBwdXYCombine) for n_ctx <= kTileGroupMax contexts of one grid size,
+// each with its own operands, on cs[0]'s stream.  tile_groupable: the context
+// runs the plane/slab passes and has a fixed FP32 instance (not forced off).
+constexpr int kTileGroupMax = 8;
+enum TileKind { kTileFwdXY = 0, kTileBwdXY = 1, kTileBwdXYCombine = 2, kTileKinZ = 3, kTilePrecZ = 4 };
+bool tile_groupable(const fbse_ctx *c);
+int tile_group_pass(int n_ctx, fbse_ctx *const *cs, int kind, const float2 *const *src, float2 *const *dst,
+                    const long *nbatch, const float2 *const *psi_in, const float *a1, const float *a0,
+                    const float2 *const *pjd, float2 *const *fwd_dst);
 // pjd != nullptr: the combine pass also adds the projector expansion with the
 // coefficients of proj_project
 // fwd_next: the combine pass also leaves the forward xy transform of out in
@@ -316,6 +328,12 @@ int apply_h_cplx(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *w
 template <typename T>
 int apply_h_chain(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *work, long nstates, T a1, T a0,
                   bool first);
+// The same step for n <= kTileGroupMax contexts of one grid size on one
+// stream (fft_kernels.hip): projections per context, FFT passes as grouped
+// launches (tile_groupable contexts, FP32, projectors absent or expanded in
+// the plane passes).
+int apply_h_group(int n, fbse_ctx *const *cs, const float2 *const *psi, float2 *const *out, float2 *const *work,
+                  const long *nstates, const float *a1, const float *a0, bool first);
 
 // n contiguous states (reference layout, FP64): out = F^-1 [ F[r] / (N (1 + 0.5|k|^2 / e_p)) ]
 // (capi.hip; out may equal r)

@@ -722,6 +722,34 @@ int apply_h_chain(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *
   if (!chain) return proj_apply<T>(c, psi, out, nstates, (double)a1);
   return FBSE_OK;
 }
+// One Newton step's H application for a group of contexts of one grid size
+// that all run the FP32 fixed plane/slab instance (tile_groupable) with
+// projectors the plane passes expand (or none): the projections per context,
+// then the three passes as grouped launches on cs[0]'s stream (all contexts'
+// streams must be that stream).  Same chain as apply_h_chain: the combine
+// writes the forward xy transform of out into work for the next step.
+int apply_h_group(int n, fbse_ctx *const *cs, const float2 *const *psi, float2 *const *out, float2 *const *work,
+                  const long *nstates, const float *a1, const float *a0, bool first) {
+  const float2 *pjd[kTileGroupMax] = {};
+  long nb[kTileGroupMax];
+  int rc;
+  for (int i = 0; i < n; ++i) {
+    fbse_ctx *c = cs[i];
+    nb[i] = nstates[i] * c->nspin;
+    if (c->has_proj) {
+      if ((rc = ensure_pjd<float>(c, nstates[i]))) return rc;
+      pjd[i] = (const float2 *)c->d_pjd;
+      if ((rc = proj_project<float>(c, psi[i], nstates[i], (double)a1[i], (float2 *)c->d_pjd))) return rc;
+    }
+  }
+  if (first && (rc = tile_group_pass(n, cs, kTileFwdXY, psi, work, nb, nullptr, nullptr, nullptr, nullptr, nullptr)))
+    return rc;
+  const float2 *wsrc[kTileGroupMax];
+  for (int i = 0; i < n; ++i) wsrc[i] = work[i];
+  if ((rc = tile_group_pass(n, cs, kTileKinZ, wsrc, work, nb, nullptr, nullptr, nullptr, nullptr, nullptr))) return rc;
+  return tile_group_pass(n, cs, kTileBwdXYCombine, wsrc, out, nb, psi, a1, a0, pjd, work);
+}
+
 template int apply_h_chain<double>(fbse_ctx *, const double2 *, double2 *, double2 *, long, double, double, bool);
 template int apply_h_chain<float>(fbse_ctx *, const float2 *, float2 *, float2 *, long, float, float, bool);
 

@@ -35,7 +35,6 @@
 
 namespace fbse {
 
-enum TileKind { kTileFwdXY = 0, kTileBwdXY = 1, kTileBwdXYCombine = 2, kTileKinZ = 3, kTilePrecZ = 4 };
 
 // A thread stages ceil(E / R) butterflies of radix R per pass: a work-group
 // of NT threads covers tiles of up to NT * E elements.  E is a compile-time
@@ -741,6 +740,25 @@ FBSE_FX_F64(FBSE_FX_K64)
 #undef FBSE_FX_K64
 #undef FBSE_FX_KERNEL
 
+// Grouped launches of the FP32 fixed instances: one launch runs a pass for
+// up to kTileGroupMax contexts of one grid size (the dots of the batched
+// uneven-dot filter), blockIdx.y selecting the context's arguments and
+// blockIdx.x the persistent work-group of that context.  Each tile runs the
+// arithmetic of the single-context instance, so results are identical.
+template <typename T> struct TileGroup {
+  TileArgs<T> a[kTileGroupMax];
+};
+static_assert(sizeof(TileGroup<float>) <= 4096, "grouped tile arguments exceed the kernel argument space");
+#define FBSE_FX_GKERNEL(NAME, T, N, R1, R2, NT, WAVES)                                                    \
+  template <int KIND>                                                                                    \
+  __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WAVES))) void NAME(TileGroup<T> G) { \
+    fx_body<T, KIND, N, R1, R2, NT, 0, 0>(G.a[blockIdx.y]);                                              \
+  }
+#define FBSE_FX_G32(N, R1, R2, NT, W) FBSE_FX_GKERNEL(fft_fx_f32_n##N##_grp, float, N, R1, R2, NT, W)
+FBSE_FX_F32(FBSE_FX_G32)
+#undef FBSE_FX_G32
+#undef FBSE_FX_GKERNEL
+
 // ---------------------------------------------------------------------------
 // Fixed-schedule per-axis passes (axis lengths 160 = 16*10 and 256 = 16*16)
 // ---------------------------------------------------------------------------
@@ -1252,7 +1270,7 @@ template <typename T, int KIND> static int fx_grid(fbse_ctx *c, int n, long ntil
 template <typename T>
 static int tile_pass_v(fbse_ctx *c, int v, int kind, const cplx_t<T> *src, cplx_t<T> *dst, long nbatch,
                        const cplx_t<T> *psi_in, T a1, T a0, const cplx_t<T> *pjd = nullptr,
-                       cplx_t<T> *fwd_dst = nullptr, int zsplit = 1) {
+                       cplx_t<T> *fwd_dst = nullptr, int zsplit = 1, TileArgs<T> *fill = nullptr) {
   TileArgs<T> A{};
   A.src = src;
   A.dst = dst;
@@ -1319,6 +1337,10 @@ static int tile_pass_v(fbse_ctx *c, int v, int kind, const cplx_t<T> *src, cplx_
     const int n = fixed_n<T>(c);
     if (!n || A.col_split != 1 || kind == kTileBwdXY || kind == kTilePrecZ)
       return set_err(c, FBSE_ERR_UNSUPPORTED, "no fixed FFT instance for this pass");
+    if (fill) {  // arguments for a grouped launch
+      *fill = A;
+      return FBSE_OK;
+    }
     if (ntiles <= 0) return FBSE_OK;
     int g;
     switch (kind) {
@@ -1517,6 +1539,94 @@ static int tile_pass(fbse_ctx *c, int kind, const cplx_t<T> *src, cplx_t<T> *dst
     return tile_pass_v<T>(c, v, kind, src, dst, nbatch, psi_in, a1, a0, pjd, fwd_dst, zs);
   }
   return tile_pass_v<T>(c, v, kind, src, dst, nbatch, psi_in, a1, a0, pjd, fwd_dst);
+}
+
+// Work-groups per context of a grouped fixed-instance launch: the resident
+// work-groups of the instance shared out over the contexts.
+template <int KIND> static int fx_group_grid(fbse_ctx *c, int n, int n_ctx) {
+  static int cache[64][kFxMaxN / 2 + 1][5] = {};
+  const int d = c->device, ni = n / 2;
+  if (d < 0 || d >= 64 || n % 2 || n > kFxMaxN) return 0;
+  int &res = cache[d][ni][KIND];
+  if (res == 0) {
+    const void *fn = nullptr;
+#define FBSE_FX_GFN(N, R1, R2, NT, W) \
+  if (n == N) fn = (const void *)fft_fx_f32_n##N##_grp<KIND>;
+    FBSE_FX_F32(FBSE_FX_GFN)
+#undef FBSE_FX_GFN
+    int per_cu = 0, ncu = 0;
+    if (!fn || hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, fx_threads<float>(n), tile_lds_bytes<float>(n, n)) !=
+            hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, d) != hipSuccess) {
+      (void)hipGetLastError();
+      per_cu = 0;
+    }
+    res = per_cu > 0 && ncu > 0 ? per_cu * ncu : -1;
+  }
+  return res < 0 ? 0 : std::max(1, res / n_ctx);
+}
+
+bool tile_groupable(const fbse_ctx *c) {
+  int nv;
+  tile_variants<float>(&nv);
+  const bool forced = c->force_variant == 1 && c->force_lines > 0;
+  return c->force_variant != 0 && tiles_supported<float>(c) && fixed_n<float>(c) > 0 &&
+         (!forced || c->force_lines == nv + 1);
+}
+
+int tile_group_pass(int n_ctx, fbse_ctx *const *cs, int kind, const float2 *const *src, float2 *const *dst,
+                    const long *nbatch, const float2 *const *psi_in, const float *a1, const float *a0,
+                    const float2 *const *pjd, float2 *const *fwd_dst) {
+  fbse_ctx *c0 = cs[0];
+  if (n_ctx < 1 || n_ctx > kTileGroupMax) return set_err(c0, FBSE_ERR_ARG, "%d contexts in one grouped pass", n_ctx);
+  if (!(kind == kTileFwdXY || kind == kTileKinZ || kind == kTileBwdXYCombine))
+    return set_err(c0, FBSE_ERR_UNSUPPORTED, "no grouped instance of pass %d", kind);
+  const int n = fixed_n<float>(c0);
+  int nv;
+  tile_variants<float>(&nv);
+  TileGroup<float> G{};
+  long most = 0;
+  int rc;
+  for (int i = 0; i < n_ctx; ++i) {
+    if (fixed_n<float>(cs[i]) != n || n == 0)
+      return set_err(c0, FBSE_ERR_UNSUPPORTED, "grouped pass: context %d has no fixed instance of size %d", i, n);
+    if ((rc = tile_pass_v<float>(cs[i], nv, kind, src[i], dst[i], nbatch[i], psi_in ? psi_in[i] : nullptr,
+                                 a1 ? a1[i] : 0.f, a0 ? a0[i] : 0.f, pjd ? pjd[i] : nullptr,
+                                 fwd_dst ? fwd_dst[i] : nullptr, 1, &G.a[i]))) {
+      if (cs[i] != c0) set_err(c0, rc, "grouped pass, context %d: %s", i, cs[i]->err.c_str());
+      return rc;
+    }
+    most = std::max(most, nbatch[i] * (long)G.a[i].tiles_per_item);
+  }
+  if (most <= 0) return FBSE_OK;
+  int g;
+  switch (kind) {
+    case kTileFwdXY: g = fx_group_grid<kTileFwdXY>(c0, n, n_ctx); break;
+    case kTileKinZ: g = fx_group_grid<kTileKinZ>(c0, n, n_ctx); break;
+    default: g = fx_group_grid<kTileBwdXYCombine>(c0, n, n_ctx); break;
+  }
+  if (g <= 0) return set_err(c0, FBSE_ERR_UNSUPPORTED, "grouped FFT instance %d not resident", n);
+  const dim3 gd((unsigned)std::min<long>(g, most), (unsigned)n_ctx), bd((unsigned)fx_threads<float>(n));
+  const size_t lds = tile_lds_bytes<float>(n, n);
+  hipEvent_t ev = nullptr;
+  const int pk = kind == kTileBwdXYCombine ? 1 : 0;
+  prof_begin(c0, pk, &ev);
+#define FBSE_FX_GL(N, R1, R2, NT, W)                                                                   \
+  if (n == N) {                                                                                       \
+    switch (kind) {                                                                                   \
+      case kTileFwdXY: hipLaunchKernelGGL((fft_fx_f32_n##N##_grp<kTileFwdXY>), gd, bd, lds, c0->stream, G); break; \
+      case kTileKinZ: hipLaunchKernelGGL((fft_fx_f32_n##N##_grp<kTileKinZ>), gd, bd, lds, c0->stream, G); break;   \
+      default:                                                                                        \
+        hipLaunchKernelGGL((fft_fx_f32_n##N##_grp<kTileBwdXYCombine>), gd, bd, lds, c0->stream, G);   \
+        break;                                                                                        \
+    }                                                                                                 \
+  }
+  FBSE_FX_F32(FBSE_FX_GL)
+#undef FBSE_FX_GL
+  FBSE_HIP(c0, hipGetLastError());
+  prof_end(c0, pk, ev);
+  return FBSE_OK;
 }
 
 // out = a1*(T + V) psi + a0*psi over nstates*nspin grids (projectors are

@@ -117,12 +117,19 @@ int fbse_filter_dev(fbse_ctx *ctx, double *d_psi_rank, long n_starts, long ms, l
                     const double *zn, double Vmin, double dE_1, int precision);
 
 /* Filters of several dots (one context each, e.g. of different grid sizes) in
- * one call: each dot's Newton recurrence runs on its own context's stream, the
- * host issues the dots' steps and graph replays interleaved so their kernels
- * overlap on the GPU, and the window depth is sized for all rings together.
- * Per dot d the arguments mean what they mean for fbse_filter_dev; ms and M are
- * common.  Results equal n_dots separate fbse_filter_dev calls.  No counterpart
- * in the reference (one dot per run); this is the batched uneven-dot workload. */
+ * one call.  In mixed precision, dots of one grid size that run the fixed
+ * FP32 plane/slab instance (cubic 60 ... 120 grids; projectors absent or with
+ * one radial projector) form groups of up to 8 whose FFT passes are single
+ * grouped launches over all of the group's dots (one launch per pass and
+ * Newton step; the group's windows are one hipGraph) on the first dot's
+ * stream; every other dot's recurrence runs on its own context's stream.  The
+ * host issues the groups' and dots' steps and graph replays interleaved so
+ * their kernels overlap on the GPU, and the window depth is sized for all
+ * rings together.  Per dot d the arguments mean what they mean for
+ * fbse_filter_dev; ms and M are common.  Results equal n_dots separate
+ * fbse_filter_dev calls that run the same FFT instance (grouped dots always
+ * run the fixed one).  No counterpart in the reference (one dot per run);
+ * this is the batched uneven-dot workload. */
 int fbse_filter_many(int n_dots, fbse_ctx *const *ctx, double *const *d_psi_rank, const long *n_starts, long ms,
                      long M, const double *const *an, const double *const *zn, const double *Vmin,
                      const double *dE_1, int precision);
@@ -260,6 +267,10 @@ long fbse_graph_replays(const fbse_ctx *ctx);
 /* Accumulation window depth K of the last filter call (fbse_set_window, or
  * the depth the free memory allowed); for reporting. */
 int fbse_last_window(const fbse_ctx *ctx);
+/* Dots whose FFT passes ran as shared grouped launches with this context in
+ * its last fbse_filter_many call (1: its own launches; 0: no such call yet).
+ * For reporting.  No counterpart in the reference (one dot per run). */
+int fbse_last_group(const fbse_ctx *ctx);
 /* Device memory the context holds between calls: out[0..3] = the filter ring /
  * work buffer, the second work buffer, the partial Grams, the rocBLAS
  * workspace (bytes).  For reporting; fbse_trim releases the first three. */

@@ -76,3 +76,39 @@ def test_filter_many_rejects_shared_context(dots):
         api.filter_many([g, g], [d.data_ptr(), d.data_ptr()], [1, 1], 2, [an, an], [zn, zn], [Vmin, Vmin],
                         [dE_1, dE_1])
     assert ei.value.code == api.FBSE_ERR_ARG
+
+
+def test_filter_many_grouped_launches():
+    """Mixed precision, dots of one size on the fixed FP32 plane instance
+    (three 60^3 dots and two 64^3 dots, spinors with SO + NL projectors) share
+    grouped launches (one launch per FFT pass for the whole group), a 20^3
+    dot runs on its own: every dot's result is bit-identical to its separate
+    filter on the same instance and window, graph replays included."""
+    import torch
+    from filterbse_amd import api
+    spec = [(60, 1, 2), (60, 2, 1), (64, 3, 2), (60, 4, 3), (64, 5, 1), (20, 6, 2)]
+    dots_ = []
+    for n, seed, ns in spec:
+        s = small_system(n=(n, n, n), dx=0.5, nspin=2, is_complex=1, so=1, nl=1, natoms=6, n_so_proj=1,
+                         n_nl_proj=1, seed=seed)
+        g = gpu_context(s)
+        if n != 20:
+            g.set_fft_kernel(1, 4)  # the fixed FP32 instance
+        g.set_window(3)
+        dots_.append((s, g, ns))
+    ms, M = 3, 42
+    per = [inputs(s, ns, ms, M, 11 + k) for k, (s, g, ns) in enumerate(dots_)]
+    ref = [g.filter(p[0], ns, ms, p[1], p[2], p[3], p[4], precision=1) for (s, g, ns), p in zip(dots_, per)]
+    d = [torch.from_numpy(p[0]).to("cuda:0") for p in per]
+    api.filter_many([g for s, g, ns in dots_], [x.data_ptr() for x in d], [ns for s, g, ns in dots_], ms,
+                    [p[1] for p in per], [p[2] for p in per], [p[3] for p in per], [p[4] for p in per], 1)
+    torch.cuda.synchronize()
+    groups = [g.last_group() for s, g, ns in dots_]
+    print("group sizes", groups)
+    assert groups == [3, 3, 2, 3, 2, 1]
+    for k, (s, g, ns) in enumerate(dots_):
+        assert g.graph_replays() == 9, k
+        out = d[k].cpu().numpy()
+        assert np.array_equal(out, ref[k]), (k, np.abs(out - ref[k]).max())
+    for s, g, ns in dots_:
+        g.close()
