@@ -332,6 +332,10 @@ int apply_h_chain(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *
 // stream (fft_kernels.hip): projections per context, FFT passes as grouped
 // launches (tile_groupable contexts, FP32, projectors absent or expanded in
 // the plane passes).
+// projection-only launch for n <= kTileGroupMax contexts (FP32, one radial
+// projector each) on cs[0]'s stream (ham_kernels.hip)
+int proj_project_group(int n, fbse_ctx *const *cs, const float2 *const *psi, const long *nstates, const float *a1,
+                       float2 *const *d);
 int apply_h_group(int n, fbse_ctx *const *cs, const float2 *const *psi, float2 *const *out, float2 *const *work,
                   const long *nstates, const float *a1, const float *a0, bool first);
 

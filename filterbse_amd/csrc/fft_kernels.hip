@@ -732,15 +732,30 @@ int apply_h_group(int n, fbse_ctx *const *cs, const float2 *const *psi, float2 *
                   const long *nstates, const float *a1, const float *a0, bool first) {
   const float2 *pjd[kTileGroupMax] = {};
   long nb[kTileGroupMax];
-  int rc;
+  // contexts with projectors: one grouped projection launch
+  fbse_ctx *pc[kTileGroupMax];
+  const float2 *pp[kTileGroupMax];
+  float2 *pd[kTileGroupMax];
+  long pn[kTileGroupMax];
+  float pa1[kTileGroupMax];
+  int np = 0, rc;
   for (int i = 0; i < n; ++i) {
     fbse_ctx *c = cs[i];
     nb[i] = nstates[i] * c->nspin;
     if (c->has_proj) {
       if ((rc = ensure_pjd<float>(c, nstates[i]))) return rc;
       pjd[i] = (const float2 *)c->d_pjd;
-      if ((rc = proj_project<float>(c, psi[i], nstates[i], (double)a1[i], (float2 *)c->d_pjd))) return rc;
+      pc[np] = c;
+      pp[np] = psi[i];
+      pd[np] = (float2 *)c->d_pjd;
+      pn[np] = nstates[i];
+      pa1[np] = a1[i];
+      ++np;
     }
+  }
+  if (np && (rc = proj_project_group(np, pc, pp, pn, pa1, pd))) {
+    if (pc[0] != cs[0]) set_err(cs[0], rc, "%s", pc[0]->err.c_str());
+    return rc;
   }
   if (first && (rc = tile_group_pass(n, cs, kTileFwdXY, psi, work, nb, nullptr, nullptr, nullptr, nullptr, nullptr)))
     return rc;

@@ -41,8 +41,8 @@ template <typename C> __device__ __forceinline__ C cjm(C a, C b) {
 constexpr int kPjCols = 16;
 constexpr int kPjLanes = 16;  // kPjCols * kPjLanes = 256 threads
 
-template <typename T, int MAXP, bool EXPAND = true>
-__global__ __launch_bounds__(256) void proj_color_kernel(ProjArgs<T> P) {
+template <typename T, int MAXP, bool EXPAND>
+__device__ __forceinline__ void proj_color_body(const ProjArgs<T> &P) {
   using C = cplx_t<T>;
   __shared__ C s_d[kProjRows][kPjCols];
   // expansion launches: atoms along x; projection-only launch: column chunks
@@ -229,6 +229,24 @@ __global__ __launch_bounds__(256) void proj_color_kernel(ProjArgs<T> P) {
   }
 }
 
+template <typename T, int MAXP, bool EXPAND = true>
+__global__ __launch_bounds__(256) void proj_color_kernel(ProjArgs<T> P) {
+  proj_color_body<T, MAXP, EXPAND>(P);
+}
+
+// Projection-only launches of several contexts at once (the grouped Newton
+// steps of the batched uneven dots): blockIdx.z selects the context, whose
+// work-groups past its column chunks or atoms leave at once.
+template <typename T> struct ProjGroup {
+  ProjArgs<T> a[kTileGroupMax];
+};
+template <typename T>
+__global__ __launch_bounds__(256) void proj_group_kernel(ProjGroup<T> G) {
+  const ProjArgs<T> &P = G.a[blockIdx.z];
+  if ((int)blockIdx.x * kPjCols >= P.ncols || (int)blockIdx.y >= P.n_atoms) return;
+  proj_color_body<T, 1, false>(P);
+}
+
 template <typename T>
 int proj_apply(fbse_ctx *c, const cplx_t<T> *psi_in, cplx_t<T> *psi_out, long nstates, double a1) {
   ProjArgs<T> P{};
@@ -281,10 +299,11 @@ bool proj_fusable(const fbse_ctx *c) {
   return nso <= 1 && nnl <= 1 && c->pj.n_atoms <= 65535;  // the projection launch puts atoms along grid y
 }
 
+// arguments of a projection-only launch (mixed coefficients into d)
 template <typename T>
-int proj_project(fbse_ctx *c, const cplx_t<T> *psi_in, long nstates, double a1, cplx_t<T> *d) {
-  if (!proj_fusable(c)) return set_err(c, FBSE_ERR_UNSUPPORTED, "projection-only pass needs one radial projector");
-  ProjArgs<T> P{};
+static void proj_only_args(fbse_ctx *c, const cplx_t<T> *psi_in, long nstates, double a1, cplx_t<T> *d,
+                           ProjArgs<T> &P) {
+  P = ProjArgs<T>{};
   const ProjTables<T> *pt;
   if constexpr (sizeof(T) == 8) pt = &c->pj;
   else pt = &c->pj_f;
@@ -311,6 +330,13 @@ int proj_project(fbse_ctx *c, const cplx_t<T> *psi_in, long nstates, double a1, 
   P.ncols = (int)(nstates * c->nspin);
   P.scale = (T)(c->dv * a1);
   P.atoms = pt->spatial_atoms;  // every atom (any order works: nothing is written to the grid)
+}
+
+template <typename T>
+int proj_project(fbse_ctx *c, const cplx_t<T> *psi_in, long nstates, double a1, cplx_t<T> *d) {
+  if (!proj_fusable(c)) return set_err(c, FBSE_ERR_UNSUPPORTED, "projection-only pass needs one radial projector");
+  ProjArgs<T> P;
+  proj_only_args<T>(c, psi_in, nstates, a1, d, P);
   if (P.n_atoms == 0 || P.ncols == 0) return FBSE_OK;
   hipEvent_t ev = nullptr;
   prof_begin(c, 2, &ev);
@@ -321,6 +347,30 @@ int proj_project(fbse_ctx *c, const cplx_t<T> *psi_in, long nstates, double a1, 
   prof_end(c, 2, ev);
   return FBSE_OK;
 }
+
+int proj_project_group(int n, fbse_ctx *const *cs, const float2 *const *psi, const long *nstates, const float *a1,
+                       float2 *const *d) {
+  fbse_ctx *c0 = cs[0];
+  if (n < 1 || n > kTileGroupMax) return set_err(c0, FBSE_ERR_ARG, "%d contexts in one grouped projection", n);
+  ProjGroup<float> G{};
+  int most_chunks = 0, most_atoms = 0;
+  for (int i = 0; i < n; ++i) {
+    if (!proj_fusable(cs[i]))
+      return set_err(c0, FBSE_ERR_UNSUPPORTED, "grouped projection: context %d needs one radial projector", i);
+    proj_only_args<float>(cs[i], psi[i], nstates[i], (double)a1[i], d[i], G.a[i]);
+    if (G.a[i].n_atoms > 65535) return set_err(c0, FBSE_ERR_UNSUPPORTED, "%d atoms in one projection launch", G.a[i].n_atoms);
+    most_chunks = std::max(most_chunks, (G.a[i].ncols + kPjCols - 1) / kPjCols);
+    most_atoms = std::max(most_atoms, G.a[i].n_atoms);
+  }
+  if (most_chunks == 0 || most_atoms == 0) return FBSE_OK;
+  hipEvent_t ev = nullptr;
+  prof_begin(c0, 2, &ev);
+  hipLaunchKernelGGL(proj_group_kernel<float>, dim3(most_chunks, most_atoms, n), dim3(256), 0, c0->stream, G);
+  FBSE_HIP(c0, hipGetLastError());
+  prof_end(c0, 2, ev);
+  return FBSE_OK;
+}
+
 // Expansion of the rank-k records of every plane into out (one batch item
 // per blockIdx.z, one plane per blockIdx.y): the arithmetic of the plane
 // pass's rounds; the records of one rank touch distinct points, and ranks run
