@@ -1043,7 +1043,13 @@ template <typename T> struct FilterRun {
       }
     FBSE_HIP(c, hipMalloc(&d_an, an_t.size() * sizeof(double)));
     FBSE_HIP(c, hipMemcpyAsync(d_an, an_t.data(), an_t.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    {
+    // start states -> ring slot 0 (complex FP32: converted in place of a
+    // staging copy).  psi_rank is not cleared: the first window's
+    // accumulation writes it.
+    if (!packed && sizeof(T) == 4) {
+      if ((rc = to_f32_strided(c, (const double2 *)d_psi_rank, ms * sl / 2, (float2 *)ring, nsg, n_starts)))
+        return rc;
+    } else {
       double2 *stage;
       if constexpr (sizeof(T) == 8) stage = (double2 *)ring;
       else {
@@ -1060,7 +1066,6 @@ template <typename T> struct FilterRun {
         if ((rc = to_f32(c, stage, (float2 *)ring, slot_elems))) return rc;
       }
     }
-    if ((rc = fill_zero(c, d_psi_rank, (size_t)n_starts * ms * sl * sizeof(double)))) return rc;
     n_full = M / R - 1;  // full windows w = 1 .. n_full
     graph = c->use_graph && !c->profile && n_full >= 2;
     c->graph_replays = 0;
@@ -1085,7 +1090,7 @@ template <typename T> struct FilterRun {
     if (j + 1 - acc_lo == R || j == M - 1) {
       int r;
       if ((r = accumulate_window<T>(c, ring, R, slot_elems, acc_lo, j + 1, d_an, ms, M, d_psi_rank, n_starts,
-                                    n_items, packed, real_an)))
+                                    n_items, packed, real_an, nullptr, acc_lo == 0)))
         return r;
       acc_lo = j + 1;
     }
@@ -1171,7 +1176,7 @@ template <typename T> struct FilterRun {
     int rc = FBSE_OK;
     if (M == 1)
       rc = accumulate_window<T>(c, ring, R, slot_elems, 0, 1, d_an, ms, M, d_psi_rank, n_starts, n_items, packed,
-                                real_an);
+                                real_an, nullptr, true);
     (void)hipStreamSynchronize(c->stream);  // replays done before the graph exec goes away
     release();
     if (!rc) {

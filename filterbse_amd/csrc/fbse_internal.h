@@ -368,6 +368,7 @@ bool proj_fusable(const fbse_ctx *c);
 int pack_real(fbse_ctx *c, const double *src, long src_stride, long nstates, double2 *dst);
 int unpack_real(fbse_ctx *c, const double2 *src, long nstates, double *dst, long dst_stride);
 int to_f32(fbse_ctx *c, const double2 *src, float2 *dst, long n);
+int to_f32_strided(fbse_ctx *c, const double2 *src, long src_stride, float2 *dst, long n, long count);
 int fill_zero(fbse_ctx *c, void *dst, size_t bytes);
 // per-state sums over nsg complex elements (or real layout): out[s] = sum |a|^2,
 // sum Re(conj(a) b), sum |b - e a|^2 ...
@@ -395,11 +396,13 @@ int launch_rot_mfma(hipStream_t s, bool cplx, const double *A, long lda, long nr
 
 // window accumulation (ham_kernels.hip)
 // jlo_dev != nullptr: the window starts at step *jlo_dev (read on the device,
-// for graph replays); j_hi - j_lo still gives the window length.
+// for graph replays); j_hi - j_lo still gives the window length.  store: the
+// window's sums are written instead of added (the first window of a filter,
+// so psi_rank needs no clearing beforehand).
 template <typename T>
 int accumulate_window(fbse_ctx *c, const cplx_t<T> *ring, int ring_slots, long slot_stride, long j_lo,
                       long j_hi, const double *d_an, long ms, long ncheby, double *psi_rank, long n_starts,
-                      long n_items, int packed, int real_coef, const long *jlo_dev = nullptr);
+                      long n_items, int packed, int real_coef, const long *jlo_dev = nullptr, bool store = false);
 // *step += by (one thread; graph-safe counter update)
 int step_advance(fbse_ctx *c, long *step, long by);
 

@@ -484,6 +484,26 @@ int to_f32(fbse_ctx *c, const double2 *src, float2 *dst, long n) {
   FBSE_HIP(c, hipGetLastError());
   return FBSE_OK;
 }
+// count blocks of n elements, block q read from src + q*src_stride, written
+// contiguously (the start states of a psi_rank block into an FP32 ring slot)
+__global__ void to_f32_strided_kernel(const double2 *src, long src_stride, float2 *dst, long n) {
+  const long q = blockIdx.y;
+  const double2 *s = src + q * src_stride;
+  float2 *d = dst + q * n;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double2 v = s[i];
+    d[i] = make_float2((float)v.x, (float)v.y);
+  }
+}
+int to_f32_strided(fbse_ctx *c, const double2 *src, long src_stride, float2 *dst, long n, long count) {
+  for (long q0 = 0; q0 < count; q0 += 65535) {
+    const long nq = count - q0 < 65535 ? count - q0 : 65535;
+    hipLaunchKernelGGL(to_f32_strided_kernel, grid1d(n, nq), dim3(256), 0, c->stream, src + q0 * src_stride,
+                       src_stride, dst + q0 * n, n);
+    FBSE_HIP(c, hipGetLastError());
+  }
+  return FBSE_OK;
+}
 // zero fill by kernel (graph-capture safe, unlike hipMemsetAsync)
 __global__ void fill_zero_kernel(uint4 *dst, size_t n16, unsigned char *tail, size_t ntail) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
@@ -632,7 +652,8 @@ __global__ __launch_bounds__(256) void accumulate_kernel(const cplx_t<T> *__rest
                                                          long slot_stride, long j_lo, int nj,
                                                          const double *__restrict__ an, int ms, int t0, int mc,
                                                          double *__restrict__ psi_rank, long n_starts, long n_items,
-                                                         int packed, long nsg, const long *__restrict__ jlo_dev) {
+                                                         int packed, long nsg, const long *__restrict__ jlo_dev,
+                                                         int store) {
   const long item = blockIdx.y;  // complex: start state; packed: start pair
   if (jlo_dev) j_lo = *jlo_dev;
   const long slot0 = j_lo % ring_slots;
@@ -691,10 +712,14 @@ __global__ __launch_bounds__(256) void accumulate_kernel(const cplx_t<T> *__rest
       for (int t = 0; t < kMaxMs; ++t) {
         if (t < mc) {
           double2 *dst = reinterpret_cast<double2 *>(psi_rank + (item * ms + t0 + t) * stlen + 2 * g);
-          double2 o = *dst;
-          o.x += acc[t].x;
-          o.y += acc[t].y;
-          *dst = o;
+          if (store) {
+            *dst = acc[t];
+          } else {
+            double2 o = *dst;
+            o.x += acc[t].x;
+            o.y += acc[t].y;
+            *dst = o;
+          }
         }
       }
     } else {
@@ -702,8 +727,12 @@ __global__ __launch_bounds__(256) void accumulate_kernel(const cplx_t<T> *__rest
 #pragma unroll
       for (int t = 0; t < kMaxMs; ++t) {
         if (t < mc) {
-          psi_rank[((2 * item) * ms + t0 + t) * stlen + g] += acc[t].x;
-          if (2 * item + 1 < n_starts) psi_rank[((2 * item + 1) * ms + t0 + t) * stlen + g] += acc[t].y;
+          double *d0 = psi_rank + ((2 * item) * ms + t0 + t) * stlen + g;
+          *d0 = store ? acc[t].x : *d0 + acc[t].x;
+          if (2 * item + 1 < n_starts) {
+            double *d1 = psi_rank + ((2 * item + 1) * ms + t0 + t) * stlen + g;
+            *d1 = store ? acc[t].y : *d1 + acc[t].y;
+          }
         }
       }
     }
@@ -713,7 +742,7 @@ __global__ __launch_bounds__(256) void accumulate_kernel(const cplx_t<T> *__rest
 template <typename T>
 int accumulate_window(fbse_ctx *c, const cplx_t<T> *ring, int ring_slots, long slot_stride, long j_lo, long j_hi,
                       const double *d_an, long ms, long ncheby, double *psi_rank, long n_starts, long n_items,
-                      int packed, int real_coef, const long *jlo_dev) {
+                      int packed, int real_coef, const long *jlo_dev, bool store) {
   const int nj = (int)(j_hi - j_lo);
   if (nj <= 0) return FBSE_OK;
   if (nj > kMaxWin) return set_err(c, FBSE_ERR_UNSUPPORTED, "window %d > %d", nj, kMaxWin);
@@ -728,20 +757,21 @@ int accumulate_window(fbse_ctx *c, const cplx_t<T> *ring, int ring_slots, long s
     const int mc = (int)std::min<long>(kMaxMs, ms - t0);
     if (real_coef || packed)
       hipLaunchKernelGGL((accumulate_kernel<T, true>), grid, dim3(256), 0, c->stream, ring, ring_slots, slot_stride,
-                         j_lo, nj, d_an, (int)ms, (int)t0, mc, psi_rank, n_starts, n_items, packed, c->nsg, jlo_dev);
+                         j_lo, nj, d_an, (int)ms, (int)t0, mc, psi_rank, n_starts, n_items, packed, c->nsg, jlo_dev,
+                         store ? 1 : 0);
     else
       hipLaunchKernelGGL((accumulate_kernel<T, false>), grid, dim3(256), 0, c->stream, ring, ring_slots,
                          slot_stride, j_lo, nj, d_an, (int)ms, (int)t0, mc, psi_rank, n_starts, n_items, packed,
-                         c->nsg, jlo_dev);
+                         c->nsg, jlo_dev, store ? 1 : 0);
   }
   FBSE_HIP(c, hipGetLastError());
   prof_end(c, 3, ev);
   return FBSE_OK;
 }
 template int accumulate_window<double>(fbse_ctx *, const double2 *, int, long, long, long, const double *, long,
-                                       long, double *, long, long, int, int, const long *);
+                                       long, double *, long, long, int, int, const long *, bool);
 template int accumulate_window<float>(fbse_ctx *, const float2 *, int, long, long, long, const double *, long, long,
-                                      double *, long, long, int, int, const long *);
+                                      double *, long, long, int, int, const long *, bool);
 
 __global__ void step_advance_kernel(long *step, long by) {
   if (blockIdx.x == 0 && threadIdx.x == 0) step[0] += by;
