@@ -649,10 +649,17 @@ def main():
     t_all = prof[4][1] / steps_p
     ach = b_hpsi * n_items / (t_h * 1e-3) / 1e9
     whole = (b_hpsi + b_acc) * n_items / (t_all * 1e-3) / 1e9
-    # dominant kernel on its own bytes: one FFT sweep reads and writes the batch
+    # one FFT sweep on its own bytes: it reads and writes the batch
     l0, m0 = prof[0]
     own_bytes = 2.0 * n_items * s_ * ngrid * 8
     own = own_bytes / (m0 / max(l0, 1) * 1e-3) / 1e9
+    # the x combine on its own bytes (per-axis passes, fused: it reads the work
+    # buffer, psi_{j-1}, V_loc and the projector records, and writes psi_j and
+    # the next step's forward x transform)
+    l1, m1 = prof[1]
+    n_rec = int(np.sum(job["proj"]["n_gpt"])) if job.get("proj") is not None else 0
+    comb_bytes = 4.0 * n_items * s_ * ngrid * 8 + s_ * ngrid * 4 + n_rec * 40
+    comb = comb_bytes / (m1 / max(l1, 1) * 1e-3) / 1e9
     if fft_path == 1:
         kname = ("fft_fx_f32_n%d<*>" % job["nx"]) if fft_inst == 4 else "fft_tile_f32_e*<*>"
     else:
@@ -667,6 +674,11 @@ def main():
         kernel_own=dict(kernel=kname + " (one FFT sweep)", bytes_per_launch=own_bytes,
                         avg_launch_ms=round(m0 / max(l0, 1), 4), achieved=round(own, 1),
                         frac=round(own / HBM_PEAK_GBS, 4), traffic=measured_traffic(args.config, "fft_sweep")),
+        combine_own=dict(kernel="x backward + V_loc + Newton shift + projector expansion + next forward x "
+                                "(the longest launch of the step)",
+                         bytes_per_launch=comb_bytes, records=n_rec, avg_launch_ms=round(m1 / max(l1, 1), 4),
+                         achieved=round(comb, 1), frac=round(comb / HBM_PEAK_GBS, 4),
+                         traffic=measured_traffic(args.config, "x_bwd_combine")),
         whole_filter=dict(bytes_per_step=round((b_hpsi + b_acc) * n_items), window_K=K_prof,
                           avg_ms_per_step=round(t_all, 4), achieved=round(whole, 1),
                           frac=round(whole / HBM_PEAK_GBS, 4)),
