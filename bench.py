@@ -142,12 +142,14 @@ def one_pass(S, precision, warm=False, refine=0, profile=False):
     S["refine_log"] = []
     for k in range(refine):
         # FP64 refinement over the Ritz pairs within 4 filter widths of the targets
+        t_k = time.perf_counter()
         w = 4.0 * S["sig_f"]
         ev2, sig2, nrep = ctx.refine_dev(ptr, rank, ev, sig, S["targets"].min() - w, S["targets"].max() + w,
                                          e_p=S["cfg"].get("refine_ep", 0.25), sigma_max=1e-2)
         S["refine_log"].append(dict(replaced=int(nrep), converged_before=int((sig < RESID).sum()),
                                     converged_after=int((sig2 < RESID).sum()),
-                                    best_before=float(sig.min()), best_after=float(sig2.min())))
+                                    best_before=float(sig.min()), best_after=float(sig2.min()),
+                                    seconds=round(time.perf_counter() - t_k, 3)))
         ev, sig = ev2, sig2
     torch.cuda.synchronize()
     t.append(time.perf_counter())

@@ -549,6 +549,21 @@ __global__ void state_red_kernel(const double *a, const double *b, const double 
   }
   if (threadIdx.x == 0) partial[s * gridDim.x + blockIdx.x] = sh[0];
 }
+// second stage for many partials per state: one work-group per state, a
+// fixed strided split over 256 threads, then a fixed tree
+__global__ void red_final_block_kernel(const double *partial, int nb, double *out) {
+  const long s = blockIdx.x;
+  double t = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) t += partial[s * nb + b];
+  __shared__ double sh[256];
+  sh[threadIdx.x] = t;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[s] = sh[0];
+}
 __global__ void red_final_kernel(const double *partial, int nb, long nstates, double *out) {
   for (long s = blockIdx.x * (long)blockDim.x + threadIdx.x; s < nstates; s += (long)gridDim.x * blockDim.x) {
     double t = 0.0;
@@ -560,15 +575,28 @@ template <int OP>
 static int state_red(fbse_ctx *c, const double *a, const double *b, const double *eps, long nstates, long elems,
                      double *d_out) {
   if (nstates == 0) return FBSE_OK;
-  int rc = ensure_red(c, (size_t)nstates * kRedBlocks * sizeof(double));
+  // blocks per state: kRedBlocks, more for few large states (one state of a
+  // 256^3 spinor grid is 67M doubles: 64 blocks left most CUs idle), at
+  // least 4096 doubles per block; the partition depends only on (nstates,
+  // elems), so results are reproducible
+  long nblk = kRedBlocks;
+  if (nstates < 32) {
+    nblk = std::min<long>(2048 / nstates, elems / 4096);
+    if (nblk < kRedBlocks) nblk = kRedBlocks;
+  }
+  int rc = ensure_red(c, (size_t)std::min<long>(nstates, 65535) * nblk * sizeof(double));
   if (rc) return rc;
   for (long s0 = 0; s0 < nstates; s0 += 65535) {
     const long ns = nstates - s0 < 65535 ? nstates - s0 : 65535;
-    hipLaunchKernelGGL(state_red_kernel<OP>, dim3(kRedBlocks, ns), dim3(256), 0, c->stream, a + s0 * elems,
+    hipLaunchKernelGGL(state_red_kernel<OP>, dim3((unsigned)nblk, ns), dim3(256), 0, c->stream, a + s0 * elems,
                        b ? b + s0 * elems : nullptr, eps ? eps + s0 : nullptr, elems, c->d_red);
     FBSE_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(red_final_kernel, dim3((ns + 255) / 256), dim3(256), 0, c->stream, c->d_red, kRedBlocks, ns,
-                       d_out + s0);
+    if (nblk > kRedBlocks)
+      hipLaunchKernelGGL(red_final_block_kernel, dim3((unsigned)ns), dim3(256), 0, c->stream, c->d_red, (int)nblk,
+                         d_out + s0);
+    else
+      hipLaunchKernelGGL(red_final_kernel, dim3((ns + 255) / 256), dim3(256), 0, c->stream, c->d_red, (int)nblk, ns,
+                         d_out + s0);
     FBSE_HIP(c, hipGetLastError());
   }
   return FBSE_OK;
