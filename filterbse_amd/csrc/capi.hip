@@ -744,14 +744,38 @@ extern "C" int fbse_apply_h_dev(fbse_ctx *c, const double *d_psi, double *d_hpsi
   return apply_h_dev_impl(c, d_psi, d_hpsi, nstates);
 }
 
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Bytes at the front of the context's work buffer that apply_h_dev_impl (and
+// precondition_dev) use for n states: the FFT work space of one chunk.
+static size_t h_work_bytes(const fbse_ctx *c, long nstates) {
+  const long ch = h_chunk(c);
+  const size_t per = (size_t)c->nsg * sizeof(double2);
+  if (c->is_complex) return (size_t)std::min(ch, nstates) * per;
+  const long chp = std::max(2L, ch & ~1L);
+  return (size_t)((std::min(chp, nstates) + 1) / 2) * per * 2;
+}
+// After a filter the context's work buffer holds its ring, which the stages
+// that follow no longer need.  Their large temporaries (H U of Rayleigh-Ritz,
+// the refinement's blocks) are taken from the buffer past the front bytes the
+// H applications use, instead of freeing the ring and allocating anew: a fresh
+// allocation of tens of GB costs seconds (about 2 s for 75 GB on the MI355X),
+// and the next filter then finds its ring in place.
+// States of per_state bytes the buffer holds past `front`, and their address.
+static long arena_states(const fbse_ctx *c, size_t front, size_t per_state) {
+  front = align256(front);
+  return c->d_scratch && c->scratch_bytes > front ? (long)((c->scratch_bytes - front) / per_state) : 0;
+}
+static double *arena_at(const fbse_ctx *c, size_t front) {
+  return reinterpret_cast<double *>(static_cast<char *>(c->d_scratch) + align256(front));
+}
+
 // fbse_apply_h_dev without device allocation: the FFT work buffer(s) and the
 // projection coefficients of apply_h_dev_impl are carved out of the library's
 // static work space instead of the context's buffers, and the autotuners
 // (which time candidates in a scratch allocation) and profiling events stay
 // off for the call; every other step is the same code.  No byte of the work
 // space is read before this call writes it.
-static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
-
 extern "C" int fbse_apply_h_dev_noalloc(fbse_ctx *c, const double *d_psi, double *d_hpsi, long nstates) {
   if (!c || nstates < 0 || (nstates && (!d_psi || !d_hpsi))) return FBSE_ERR_ARG;
   FBSE_HIP(c, hipSetDevice(c->device));
@@ -886,12 +910,20 @@ extern "C" int fbse_normalize_all(fbse_ctx *c, double *psi, long n) {
   });
 }
 
-// energies (and optionally residual norms) of n device states, chunked
-static int energies_dev_impl(fbse_ctx *c, const double *d_psi, long n, double *ene, double *sig) {
+// energies (and optionally residual norms) of n device states, chunked; the
+// H psi chunk comes from the work buffer past `arena_front` bytes (the front
+// holds the H applications' FFT work space, and whatever the caller keeps
+// there) when it has room, else it is allocated
+static int energies_dev_impl(fbse_ctx *c, const double *d_psi, long n, double *ene, double *sig,
+                             size_t arena_front) {
   const long sl = stlen_of(c);
   const long ch = std::max(1L, std::min(n, h_chunk(c)));
   double *d_h = nullptr, *d_e = nullptr, *d_r = nullptr;
-  FBSE_HIP(c, hipMalloc(&d_h, (size_t)ch * sl * 8));
+  const bool in_arena = arena_front >= h_work_bytes(c, n) && arena_states(c, arena_front, (size_t)sl * 8) >= ch;
+  if (in_arena)
+    d_h = arena_at(c, arena_front);
+  else
+    FBSE_HIP(c, hipMalloc(&d_h, (size_t)ch * sl * 8));
   FBSE_HIP(c, hipMalloc(&d_e, (size_t)n * 8));
   FBSE_HIP(c, hipMalloc(&d_r, (size_t)n * 8));
   int rc = FBSE_OK;
@@ -924,7 +956,7 @@ static int energies_dev_impl(fbse_ctx *c, const double *d_psi, long n, double *e
     }
   }
   hipStreamSynchronize(c->stream);
-  hipFree(d_h);
+  if (!in_arena) hipFree(d_h);
   hipFree(d_e);
   hipFree(d_r);
   return rc;
@@ -932,13 +964,13 @@ static int energies_dev_impl(fbse_ctx *c, const double *d_psi, long n, double *e
 extern "C" int fbse_energy_all_dev(fbse_ctx *c, const double *d_psi, long n, double *ene) {
   if (!c || n < 0 || !ene) return FBSE_ERR_ARG;
   FBSE_HIP(c, hipSetDevice(c->device));
-  return energies_dev_impl(c, d_psi, n, ene, nullptr);
+  return energies_dev_impl(c, d_psi, n, ene, nullptr, h_work_bytes(c, n));
 }
 extern "C" int fbse_sigma_dev(fbse_ctx *c, const double *d_psi, long n, double *sigma, double *eval) {
   if (!c || n < 0 || !sigma) return FBSE_ERR_ARG;
   FBSE_HIP(c, hipSetDevice(c->device));
   std::vector<double> e(n);
-  int rc = energies_dev_impl(c, d_psi, n, e.data(), sigma);
+  int rc = energies_dev_impl(c, d_psi, n, e.data(), sigma, h_work_bytes(c, n));
   if (!rc && eval) memcpy(eval, e.data(), n * 8);
   return rc;
 }
@@ -955,7 +987,7 @@ template <typename F> static int with_device_copy(fbse_ctx *c, const double *h, 
 extern "C" int fbse_energy_all(fbse_ctx *c, const double *psi, long n, double *ene) {
   if (!c || n < 0 || !ene) return FBSE_ERR_ARG;
   FBSE_HIP(c, hipSetDevice(c->device));
-  return with_device_copy(c, psi, n, [&](double *d) { return energies_dev_impl(c, d, n, ene, nullptr); });
+  return with_device_copy(c, psi, n, [&](double *d) { return energies_dev_impl(c, d, n, ene, nullptr, h_work_bytes(c, n)); });
 }
 extern "C" int fbse_sigma(fbse_ctx *c, const double *psi, long n, double *sigma, double *eval) {
   if (!c || n < 0 || !sigma) return FBSE_ERR_ARG;
@@ -1699,14 +1731,18 @@ static int ortho_dev_impl(fbse_ctx *c, double *A, long n, double svd_eps, long *
   std::vector<double> mu;
   if (ns > 0) {
     double *Bs = nullptr;
-    if (big_malloc(c, (void **)&Bs, (size_t)la.m * ns * es) != hipSuccess) {
+    // from the work buffer (the filter's ring) when it holds the block
+    const bool in_arena = arena_states(c, 0, (size_t)la.m * es) >= ns;
+    if (in_arena) {
+      Bs = arena_at(c, 0);
+    } else if (big_malloc(c, (void **)&Bs, (size_t)la.m * ns * es) != hipSuccess) {
       cleanup();
       return set_err(c, FBSE_ERR_NOMEM, "ortho: %ld x %ld block of small singular directions", la.m, ns);
     }
     rc = la.product(A, n, G, n, ns, Bs);
     if (!rc) rc = la.gram_self(Bs, ns, Z);
     hipStreamSynchronize(c->stream);
-    hipFree(Bs);
+    if (!in_arena) hipFree(Bs);
     if (rc) {
       cleanup();
       return rc;
@@ -1856,15 +1892,32 @@ static int rr_dev_impl(fbse_ctx *c, double *U, long n, double *eval) {
   double *Hs = nullptr, *w = nullptr, *HU = nullptr;
   // H U is held for as many columns as memory allows: U^H (H U) over a narrow
   // block of columns runs far below the GEMM peak, so one wide product is used
-  // whenever it fits (1024 states of a 96^3 spinor grid take 29 GB).
+  // whenever it fits (1024 states of a 96^3 spinor grid take 29 GB).  It comes
+  // from the work buffer when that holds at least as many columns as a fresh
+  // block from the free memory would.
+  const size_t hw = h_work_bytes(c, n);
   long ch = std::min(n, states_that_fit(c, (size_t)sl * 8, 0.5));
-  if (ch < n && c->scratch_bytes) {  // the filter's ring is no longer needed: make room for one wide product
+  const long ch_arena = std::min(n, arena_states(c, hw, (size_t)sl * 8));
+  const bool in_arena = ch_arena >= 1 && ch_arena >= ch;
+  if (in_arena) {
+    ch = ch_arena;
+  } else if (ch < n && c->scratch_bytes) {  // the filter's ring is no longer needed: make room for one wide product
     release_work(c);
     ch = std::min(n, states_that_fit(c, (size_t)sl * 8, 0.5));
   }
   FBSE_HIP(c, hipMalloc(&Hs, (size_t)n * n * es));
   FBSE_HIP(c, hipMalloc(&w, (size_t)n * 8));
-  FBSE_HIP(c, big_malloc(c, (void **)&HU, (size_t)ch * sl * 8));
+  if (in_arena) {
+    // the FFT work space of apply_h_dev_impl stays at the front of the buffer
+    if ((rc = ensure_scratch(c, hw))) {
+      hipFree(Hs);
+      hipFree(w);
+      return rc;
+    }
+    HU = arena_at(c, hw);
+  } else {
+    FBSE_HIP(c, big_malloc(c, (void **)&HU, (size_t)ch * sl * 8));
+  }
   if (ch == n) {
     // all of H U at once: U^H (H U) is Hermitian, its upper blocks suffice
     rc = apply_h_dev_impl(c, U, HU, n);
@@ -1880,15 +1933,15 @@ static int rr_dev_impl(fbse_ctx *c, double *U, long n, double *eval) {
   if (!rc) rc = la.eigh(Hs, n, w);
   if (!rc) rc = la.rotate(U, n, Hs, n, n);
   if (!rc) {
-    std::vector<double> hw(n);
-    hipMemcpyAsync(hw.data(), w, n * 8, hipMemcpyDeviceToHost, c->stream);
+    std::vector<double> wh(n);
+    hipMemcpyAsync(wh.data(), w, n * 8, hipMemcpyDeviceToHost, c->stream);
     hipStreamSynchronize(c->stream);
-    for (long i = 0; i < n; ++i) eval[i] = hw[i] * c->dv;
+    for (long i = 0; i < n; ++i) eval[i] = wh[i] * c->dv;
   }
   hipStreamSynchronize(c->stream);
   hipFree(Hs);
   hipFree(w);
-  hipFree(HU);
+  if (!in_arena) hipFree(HU);
   if (rc) return rc;
   return normalize_dev_impl(c, U, n);
 }
@@ -2056,11 +2109,22 @@ static int refine_impl(fbse_ctx *c, double *d_psi, long n, const double *eval, c
     if (eval[i] >= e_lo && eval[i] <= e_hi && sigma[i] < sigma_max) sel.push_back(i);
   if (sel.empty()) return FBSE_OK;
   std::stable_sort(sel.begin(), sel.end(), [&](long a, long b) { return sigma[a] < sigma[b]; });
-  // two blocks of k states (corrections W and H W / the new vectors); the
-  // filter's ring is not needed any more
-  if (c->scratch_bytes) release_work(c);
-  const long kfit = states_that_fit(c, (size_t)2 * sl * 8, 0.6) ;
-  const long k = std::max(1L, std::min((long)sel.size(), kfit));
+  // two blocks of k states (corrections W and H W / the new vectors), later
+  // two chunks of H U and H^2 U: from the work buffer (the filter's ring, not
+  // needed any more) past the H applications' FFT work space when it holds
+  // four blocks of min(k, 32) states or more, else freshly allocated after
+  // the ring is released
+  const size_t hw = h_work_bytes(c, n), st_b = (size_t)sl * 8;
+  const long want = (long)sel.size();
+  const long k_arena = arena_states(c, hw, st_b) / 4;
+  const bool in_arena = k_arena >= std::min(want, 32L);
+  long k;
+  if (in_arena) {
+    k = std::min(want, k_arena);
+  } else {
+    if (c->scratch_bytes) release_work(c);
+    k = std::max(1L, std::min(want, states_that_fit(c, 2 * st_b, 0.6)));
+  }
   sel.resize(k);
   int rc = FBSE_OK;
   double *X = nullptr, *Y = nullptr, *d_th = nullptr, *Cm = nullptr, *M = nullptr, *d_w = nullptr, *M2 = nullptr;
@@ -2068,11 +2132,17 @@ static int refine_impl(fbse_ctx *c, double *d_psi, long n, const double *eval, c
   int *Ginfo = nullptr;
   auto cleanup = [&]() {
     hipStreamSynchronize(c->stream);
-    for (double *p : {X, Y, d_th, Cm, M, d_w, M2, T1, T2, G, Gw, Ge}) hipFree(p);
+    for (double *p : {d_th, Cm, M, d_w, M2, G, Gw, Ge}) hipFree(p);
+    if (!in_arena)
+      for (double *p : {X, Y, T1, T2}) hipFree(p);
     hipFree(Ginfo);
   };
-  if (big_malloc(c, (void **)&X, (size_t)k * sl * 8) != hipSuccess ||
-      big_malloc(c, (void **)&Y, (size_t)k * sl * 8) != hipSuccess) {
+  if (in_arena) {
+    if ((rc = ensure_scratch(c, hw))) return rc;  // (a no-op: the buffer is larger)
+    X = arena_at(c, hw);
+    Y = X + (size_t)k * sl;
+  } else if (big_malloc(c, (void **)&X, (size_t)k * st_b) != hipSuccess ||
+             big_malloc(c, (void **)&Y, (size_t)k * st_b) != hipSuccess) {
     cleanup();
     return set_err(c, FBSE_ERR_NOMEM, "refine: two blocks of %ld states", k);
   }
@@ -2157,11 +2227,19 @@ static int refine_impl(fbse_ctx *c, double *d_psi, long n, const double *eval, c
   REF_HIP(hipGetLastError());
   REF_RC(la.gram(Y, k2, Y, k2, M2 + ((size_t)n * nt + n) * cw, nt));
   {
-    const long cch = std::max(1L, std::min(n, states_that_fit(c, (size_t)2 * sl * 8, 0.7)));
-    if (big_malloc(c, (void **)&T1, (size_t)cch * sl * 8) != hipSuccess ||
-        big_malloc(c, (void **)&T2, (size_t)cch * sl * 8) != hipSuccess) {
-      cleanup();
-      return set_err(c, FBSE_ERR_NOMEM, "refine: two blocks of %ld states for H U", cch);
+    long cch;
+    if (in_arena) {
+      // past X and Y: at least 2 k_arena states, so cch >= k
+      cch = std::max(1L, std::min(n, (arena_states(c, hw, st_b) - 2 * k) / 2));
+      T1 = Y + (size_t)k * sl;
+      T2 = T1 + (size_t)cch * sl;
+    } else {
+      cch = std::max(1L, std::min(n, states_that_fit(c, 2 * st_b, 0.7)));
+      if (big_malloc(c, (void **)&T1, (size_t)cch * st_b) != hipSuccess ||
+          big_malloc(c, (void **)&T2, (size_t)cch * st_b) != hipSuccess) {
+        cleanup();
+        return set_err(c, FBSE_ERR_NOMEM, "refine: two blocks of %ld states for H U", cch);
+      }
     }
     for (long a0 = 0; a0 < n; a0 += cch) {
       const long cc = std::min(cch, n - a0);
@@ -2171,8 +2249,10 @@ static int refine_impl(fbse_ctx *c, double *d_psi, long n, const double *eval, c
       REF_RC(la.gram(T1, cc, Y, k2, M2 + ((size_t)n * nt + a0) * cw, nt));
     }
     REF_HIP(hipStreamSynchronize(c->stream));
-    hipFree(T1);
-    hipFree(T2);
+    if (!in_arena) {
+      hipFree(T1);
+      hipFree(T2);
+    }
     T1 = T2 = nullptr;
   }
   hipLaunchKernelGGL(mirror_upper_kernel, dim3(256), dim3(256), 0, c->stream, M2, nt, cw - 1);
@@ -2368,7 +2448,8 @@ static int refine_impl(fbse_ctx *c, double *d_psi, long n, const double *eval, c
   }
   REF_RC(normalize_dev_impl(c, Y, nc));
   std::vector<double> ev2(nc), sg2(nc);
-  REF_RC(energies_dev_impl(c, Y, nc, ev2.data(), sg2.data()));
+  // (its H U chunk past X and Y when they are in the work buffer)
+  REF_RC(energies_dev_impl(c, Y, nc, ev2.data(), sg2.data(), in_arena ? align256(hw) + 2 * k * st_b : hw));
   long nrep = 0;
   for (long q2 = 0; q2 < nc; ++q2) {
     const long i = acc[q2].slot;
