@@ -2371,12 +2371,21 @@ static int refine_impl(fbse_ctx *c, double *d_psi, long n, const double *eval, c
   auto w2 = [&](const double *y, long l) {
     return cw == 2 ? y[2 * l] * y[2 * l] + y[2 * l + 1] * y[2 * l + 1] : y[l] * y[l];
   };
+  // cluster of a pair when the cluster holds several (one degenerate level
+  // split by the noise), else -1
+  std::vector<long> grp(n, -1);
+  for (long b = 0; b < nb; ++b)
+    if (clusters[b].size() > 1)
+      for (long i : clusters[b]) grp[i] = b;
   struct Cand {
     long slot;
     double pred;
     const double *y;
+    long group;
   };
-  std::vector<Cand> best(n, Cand{-1, 1e300, nullptr});
+  std::vector<Cand> best(n, Cand{-1, 1e300, nullptr, -1});
+  // single pairs: the Rayleigh-Ritz or the refined candidate, whichever has
+  // the lower predicted residual (when below the pair's sigma)
   for (long j = 0; j < nt; ++j) {
     const double *y = &Yrr[(size_t)j * nt * cw];
     long bi = -1;
@@ -2386,45 +2395,77 @@ static int refine_impl(fbse_ctx *c, double *d_psi, long n, const double *eval, c
         bw = w2(y, i);
         bi = i;
       }
-    if (bi < 0 || bw <= 0.5 || !is_sel[bi]) continue;
+    if (bi < 0 || bw <= 0.5 || !is_sel[bi] || grp[bi] >= 0) continue;
     const double m = mu[j] * dv;
     const double pred = sqrt(std::max(0.0, quad(A2h, y) - m * m));
-    if (pred < sigma[bi] && pred < best[bi].pred) best[bi] = Cand{bi, pred, y};
+    if (pred < sigma[bi] && pred < best[bi].pred) best[bi] = Cand{bi, pred, y, -1};
   }
+  // a cluster of s pairs: its s refined vectors, which span the level as a
+  // whole (the r-th lowest for the pair of r-th lowest sigma, reassigned by
+  // their measured residuals at the end)
   for (long b = 0; b < nb; ++b) {
     if (info[b] != 0) continue;
+    const bool multi = clusters[b].size() > 1;
     for (size_t r = 0; r < clusters[b].size(); ++r) {
       const long slot = clusters[b][r];
       const double pred = sqrt(std::max(0.0, lam_b[(size_t)b * nt + r]));
-      if (pred < sigma[slot] && pred < best[slot].pred)
-        best[slot] = Cand{slot, pred, &vec_b[((size_t)b * nt + r) * nt * cw]};
+      if (multi || (pred < sigma[slot] && pred < best[slot].pred))
+        best[slot] = Cand{slot, pred, &vec_b[((size_t)b * nt + r) * nt * cw], multi ? b : -1};
     }
   }
-  std::vector<Cand> cands;
-  for (long i = 0; i < n; ++i)
-    if (best[i].slot >= 0) cands.push_back(best[i]);
-  // a candidate that carries more than half its weight on another Ritz
-  // vector, or on a candidate taken before it, would duplicate that state
-  std::sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) { return a.pred < b.pred; });
-  std::vector<Cand> acc;
-  for (const Cand &cd : cands) {
-    bool ok = true;
-    for (long l = 0; l < n && ok; ++l)
-      if (l != cd.slot && w2(cd.y, l) > 0.5) ok = false;
-    for (const Cand &a : acc) {
-      if (!ok) break;
-      double re = 0.0, im = 0.0;
-      for (long l = 0; l < nt; ++l) {
-        if (cw == 2) {
-          re += a.y[2 * l] * cd.y[2 * l] + a.y[2 * l + 1] * cd.y[2 * l + 1];
-          im += a.y[2 * l] * cd.y[2 * l + 1] - a.y[2 * l + 1] * cd.y[2 * l];
-        } else {
-          re += a.y[l] * cd.y[l];
+  // units: a single pair's candidate, or all vectors of one cluster, taken in
+  // order of their lowest predicted residual.  A unit is dropped when one of
+  // its vectors carries more than half its weight on a Ritz vector outside
+  // the unit, or on a vector taken before: it would duplicate that state.
+  std::vector<std::vector<Cand>> units;
+  {
+    std::vector<long> unit_of(nb, -1);
+    for (long i = 0; i < n; ++i) {
+      const Cand &cd = best[i];
+      if (cd.slot < 0) continue;
+      if (cd.group < 0) {
+        units.push_back({cd});
+      } else {
+        if (unit_of[cd.group] < 0) {
+          unit_of[cd.group] = (long)units.size();
+          units.push_back({});
         }
+        units[unit_of[cd.group]].push_back(cd);
       }
-      if (re * re + im * im > 0.5) ok = false;
     }
-    if (ok) acc.push_back(cd);
+  }
+  auto unit_key = [](const std::vector<Cand> &u) {
+    double m = 1e300;
+    for (const Cand &cd : u) m = std::min(m, cd.pred);
+    return m;
+  };
+  std::stable_sort(units.begin(), units.end(),
+                   [&](const std::vector<Cand> &a, const std::vector<Cand> &b) { return unit_key(a) < unit_key(b); });
+  std::vector<Cand> acc;
+  std::vector<char> in_unit(n, 0);
+  for (const auto &u : units) {
+    bool ok = true;
+    for (const Cand &cd : u) in_unit[cd.slot] = 1;
+    for (const Cand &cd : u) {
+      for (long l = 0; l < n && ok; ++l)
+        if (!in_unit[l] && w2(cd.y, l) > 0.5) ok = false;
+      for (const Cand &a : acc) {
+        if (!ok) break;
+        double re = 0.0, im = 0.0;
+        for (long l = 0; l < nt; ++l) {
+          if (cw == 2) {
+            re += a.y[2 * l] * cd.y[2 * l] + a.y[2 * l + 1] * cd.y[2 * l + 1];
+            im += a.y[2 * l] * cd.y[2 * l + 1] - a.y[2 * l + 1] * cd.y[2 * l];
+          } else {
+            re += a.y[l] * cd.y[l];
+          }
+        }
+        if (re * re + im * im > 0.5) ok = false;
+      }
+      if (!ok) break;
+    }
+    for (const Cand &cd : u) in_unit[cd.slot] = 0;
+    if (ok) acc.insert(acc.end(), u.begin(), u.end());
   }
   const long nc = (long)acc.size();
   if (nc == 0) {
@@ -2450,15 +2491,39 @@ static int refine_impl(fbse_ctx *c, double *d_psi, long n, const double *eval, c
   std::vector<double> ev2(nc), sg2(nc);
   // (its H U chunk past X and Y when they are in the work buffer)
   REF_RC(energies_dev_impl(c, Y, nc, ev2.data(), sg2.data(), in_arena ? align256(hw) + 2 * k * st_b : hw));
+  // a single pair is replaced when its new residual is lower; a cluster's
+  // vectors, in ascending residual, replace its pairs in ascending sigma when
+  // every one of them is lower (a level is replaced as a whole, so no pair of
+  // it keeps a vector that a new one duplicates)
   long nrep = 0;
-  for (long q2 = 0; q2 < nc; ++q2) {
-    const long i = acc[q2].slot;
-    if (std::isfinite(sg2[q2]) && sg2[q2] < sigma[i]) {
-      REF_HIP(hipMemcpyAsync(d_psi + i * sl, Y + q2 * sl, sl * 8, hipMemcpyDeviceToDevice, c->stream));
-      eval_out[i] = ev2[q2];
-      sigma_out[i] = sg2[q2];
-      ++nrep;
+  auto put = [&](long q2, long i) -> int {
+    REF_HIP(hipMemcpyAsync(d_psi + i * sl, Y + q2 * sl, sl * 8, hipMemcpyDeviceToDevice, c->stream));
+    eval_out[i] = ev2[q2];
+    sigma_out[i] = sg2[q2];
+    ++nrep;
+    return FBSE_OK;
+  };
+  for (long q0 = 0; q0 < nc;) {
+    long q1 = q0 + 1;
+    while (acc[q0].group >= 0 && q1 < nc && acc[q1].group == acc[q0].group) ++q1;
+    if (acc[q0].group < 0) {
+      const long i = acc[q0].slot;
+      if (std::isfinite(sg2[q0]) && sg2[q0] < sigma[i] && (rc = put(q0, i))) return rc;
+    } else {
+      std::vector<long> qs, slots;
+      for (long q2 = q0; q2 < q1; ++q2) {
+        qs.push_back(q2);
+        slots.push_back(acc[q2].slot);
+      }
+      std::sort(qs.begin(), qs.end(), [&](long a, long b) { return sg2[a] < sg2[b]; });
+      std::sort(slots.begin(), slots.end(), [&](long a, long b) { return sigma[a] < sigma[b]; });
+      bool all = true;
+      for (size_t r = 0; r < qs.size() && all; ++r) all = std::isfinite(sg2[qs[r]]) && sg2[qs[r]] < sigma[slots[r]];
+      if (all)
+        for (size_t r = 0; r < qs.size(); ++r)
+          if ((rc = put(qs[r], slots[r]))) return rc;
     }
+    q0 = q1;
   }
 #undef REF_HIP
 #undef REF_RC
