@@ -130,7 +130,8 @@ def one_pass(S, precision, warm=False, refine=0, profile=False):
     if profile:
         S["prof"] = {k: ctx.profile_read(k) for k in range(5)}
         ctx.profile(False)
-    S.setdefault("windows", []).append(dict(precision=int(precision), K=ctx.last_window(), free_gb=round(free_gb, 1)))
+    S.setdefault("windows", []).append(dict(precision=int(precision), K=ctx.last_window(), runs=ctx.last_runs(),
+                                            free_gb=round(free_gb, 1)))
     t.append(time.perf_counter())
     ctx.normalize_all_dev(ptr, n)
     rank, sv = ctx.ortho_dev(ptr, n)
@@ -638,11 +639,13 @@ def main():
     prof = {k: ctx.profile_read(k) for k in range(5)}
     ctx.profile(False)
     K_prof = ctx.last_window()
+    runs_prof = max(ctx.last_runs(), 1)
     fft_path, fft_inst = ctx.fft_choice(mixed)
     steps_p = Mp - 1
     ngrid = job["nx"] * job["ny"] * job["nz"]
     s_ = job["nspin"]
     n_items = S["n_starts"] if job["is_complex"] else (S["n_starts"] + 1) // 2
+    items_run = -(-n_items // runs_prof)  # batch items of one launch (the filter's sub-batch runs)
     # algorithmic bytes (FP32 complex psi, FP32 V_loc): B_Hpsi per state and
     # the windowed accumulation per state and step (FP64 accumulators)
     b_hpsi = 2 * s_ * ngrid * 8 + ngrid * 4
@@ -653,14 +656,14 @@ def main():
     whole = (b_hpsi + b_acc) * n_items / (t_all * 1e-3) / 1e9
     # one FFT sweep on its own bytes: it reads and writes the batch
     l0, m0 = prof[0]
-    own_bytes = 2.0 * n_items * s_ * ngrid * 8
+    own_bytes = 2.0 * items_run * s_ * ngrid * 8
     own = own_bytes / (m0 / max(l0, 1) * 1e-3) / 1e9
     # the x combine on its own bytes (per-axis passes, fused: it reads the work
     # buffer, psi_{j-1}, V_loc and the projector records, and writes psi_j and
     # the next step's forward x transform)
     l1, m1 = prof[1]
     n_rec = int(np.sum(job["proj"]["n_gpt"])) if job.get("proj") is not None else 0
-    comb_bytes = 4.0 * n_items * s_ * ngrid * 8 + s_ * ngrid * 4 + n_rec * 40
+    comb_bytes = 4.0 * items_run * s_ * ngrid * 8 + s_ * ngrid * 4 + n_rec * 40
     comb = comb_bytes / (m1 / max(l1, 1) * 1e-3) / 1e9
     if fft_path == 1:
         kname = ("fft_fx_f32_n%d<*>" % job["nx"]) if fft_inst == 4 else "fft_tile_f32_e*<*>"
@@ -681,7 +684,7 @@ def main():
                          bytes_per_launch=comb_bytes, records=n_rec, avg_launch_ms=round(m1 / max(l1, 1), 4),
                          achieved=round(comb, 1), frac=round(comb / HBM_PEAK_GBS, 4),
                          traffic=measured_traffic(args.config, "x_bwd_combine")),
-        whole_filter=dict(bytes_per_step=round((b_hpsi + b_acc) * n_items), window_K=K_prof,
+        whole_filter=dict(bytes_per_step=round((b_hpsi + b_acc) * n_items), window_K=K_prof, runs=runs_prof,
                           avg_ms_per_step=round(t_all, 4), achieved=round(whole, 1),
                           frac=round(whole / HBM_PEAK_GBS, 4)),
         measured="HIP events on the library stream over a %d-step profiled mixed filter after the passes" % Mp)

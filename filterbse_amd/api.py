@@ -117,6 +117,8 @@ GPU_SIGNATURES = {
                                    C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]),
     "fbse_graph_replays": (C.c_long, [C.c_void_p]),
     "fbse_last_window": (C.c_int, [C.c_void_p]),
+    "fbse_last_runs": (C.c_long, [C.c_void_p]),
+    "fbse_set_run_starts": (C.c_int, [C.c_void_p, C.c_long]),
     "fbse_held_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
     "fbse_trim": (C.c_int, [C.c_void_p]),
     "fbse_fwrite_states": (C.c_int, [C.c_void_p, C.c_void_p, LP, C.c_long, C.c_int, C.c_void_p]),
@@ -506,6 +508,14 @@ class Context:
     def last_window(self):
         """Accumulation window depth of the last filter call."""
         return gpu().fbse_last_window(self.ctx)
+
+    def set_run_starts(self, n):
+        """Start states per filter run (0: chosen by the filter)."""
+        self._check(gpu().fbse_set_run_starts(self.ctx, n))
+
+    def last_runs(self):
+        """Sub-batch runs of the start states in the last filter call."""
+        return gpu().fbse_last_runs(self.ctx)
 
     def last_group(self):
         """Dots that shared grouped FFT launches with this context in its last

@@ -656,6 +656,12 @@ extern "C" int fbse_trim(fbse_ctx *c) {
 }
 extern "C" long fbse_graph_replays(const fbse_ctx *c) { return c ? c->graph_replays : -1; }
 extern "C" int fbse_last_window(const fbse_ctx *c) { return c ? c->last_window : -1; }
+extern "C" long fbse_last_runs(const fbse_ctx *c) { return c ? c->last_runs : -1; }
+extern "C" int fbse_set_run_starts(fbse_ctx *c, long n) {
+  if (!c || n < 0) return FBSE_ERR_ARG;
+  c->run_starts = n;
+  return FBSE_OK;
+}
 extern "C" int fbse_last_group(const fbse_ctx *c) { return c ? c->last_group : -1; }
 extern "C" int fbse_held_bytes(fbse_ctx *c, size_t *out) {
   if (!c || !out) return FBSE_ERR_ARG;
@@ -1237,18 +1243,72 @@ static int window_depth(int forced, size_t ring_bytes_per_slot, size_t held) {
   return (int)std::max(1L, std::min((long)kMaxWindow, kk));
 }
 
+// Window depth of one filter run's ring (K + 2 slots): window_depth(), lowered
+// to what the context's work buffer already holds when that costs at most 15%
+// of the depth (growing a buffer of ~100 GB means a free and a fresh
+// allocation, seconds on the MI355X).
+static int ring_depth(fbse_ctx *c, size_t slot) {
+  const int K = window_depth(c->window, slot, c->scratch_bytes);
+  if (c->window > 0 || !c->scratch_bytes || slot == 0) return K;
+  const long fit = (long)(c->scratch_bytes / slot) - 2;
+  return fit >= 1 && fit < K && fit >= (long)(0.85 * K) ? (int)fit : K;
+}
+
+// Batch items (start states; real states: pairs) per filter run.  Per Newton
+// step and grid point of an item the windowed accumulation reads the ring
+// slot (b bytes) and, once per window of K steps, reads and writes the FP64
+// accumulators of the ms targets (32 ms / K bytes).  A ring of fewer items per
+// slot is deeper, so a large batch runs as consecutive sub-batches of the
+// start states (each its own filter of the same Newton length) when that
+// lowers the step's traffic; each extra run reads the data its items share
+// through the L2 once more (V_loc, the projector records).  With projectors
+// a run keeps at least one full column chunk of the projection pass (16
+// grids: fewer leave its lanes idle while it still reads every atom's
+// tables).  Forced window depths keep the batch whole.
+template <typename T> static long filter_items_per_run(fbse_ctx *c, long items, long ms, long M) {
+  if (c->window > 0 || items < 2 || M < 3) return items;
+  const double b = (double)sizeof(cplx_t<T>), N = (double)c->nsg;
+  const double shared = (double)c->ngrid * c->nspin * sizeof(T) +
+                        (c->has_proj ? (double)c->pj.npts * (8.0 + 8.0 * sizeof(T)) : 0.0);
+  const long min_items = c->has_proj ? std::max(1L, (long)kProjColChunk / c->nspin) : 1;
+  long best_items = items;
+  double best = 1e300;
+  for (long m = 1; m <= std::min(items, 16L); ++m) {
+    const long per = (items + m - 1) / m;
+    if (m > 1 && ((items + per - 1) / per != m || per < min_items)) continue;  // (the same split as a smaller m)
+    const int K = ring_depth(c, (size_t)(per * N * b));
+    const double Ke = (double)std::min<long>(K, M - 1);
+    const double bytes = (double)items * N * (b + 32.0 * ms / Ke) + (double)(m - 1) * shared;
+    if (bytes < best * 0.98) {  // a split must save more than 2%
+      best = bytes;
+      best_items = per;
+    }
+  }
+  return best_items;
+}
+
 template <typename T>
 static int filter_impl(fbse_ctx *c, double *d_psi_rank, long n_starts, long ms, long M, const double *an,
                        const double *zn, double Vmin, double dE_1) {
-  FilterRun<T> r(c, d_psi_rank, n_starts, ms, M, an, zn, Vmin, dE_1);
   release_aux(c);
-  int rc = r.prepare(window_depth(c->window, r.slot_bytes(), c->scratch_bytes));
-  while (!rc && r.in_window0()) rc = r.step();  // eager (runs the autotuners)
-  if (!rc) rc = r.capture();
-  while (!rc && r.replays_left()) rc = r.replay();
-  while (!rc && r.tail_left()) rc = r.step();
-  int rc2 = r.finish();
-  return rc ? rc : rc2;
+  const long sl = stlen_of(c);
+  const long unit = c->is_complex ? 1 : 2;  // real states run as packed pairs
+  const long items = (n_starts + unit - 1) / unit;
+  const long per_run = c->run_starts > 0 ? (c->run_starts + unit - 1) / unit * unit
+                                          : filter_items_per_run<T>(c, items, ms, M) * unit;
+  c->last_runs = (n_starts + per_run - 1) / per_run;
+  int rc = FBSE_OK;
+  for (long s0 = 0; s0 < n_starts && !rc; s0 += per_run) {
+    FilterRun<T> r(c, d_psi_rank + s0 * ms * sl, std::min(per_run, n_starts - s0), ms, M, an, zn, Vmin, dE_1);
+    rc = r.prepare(ring_depth(c, r.slot_bytes()));
+    while (!rc && r.in_window0()) rc = r.step();  // eager (runs the autotuners)
+    if (!rc) rc = r.capture();
+    while (!rc && r.replays_left()) rc = r.replay();
+    while (!rc && r.tail_left()) rc = r.step();
+    const int rc2 = r.finish();
+    if (!rc) rc = rc2;
+  }
+  return rc;
 }
 
 // Several dots at once: every run on its own context's stream, launches

@@ -86,6 +86,9 @@ template <typename T> struct AxisArgs {
   int xcd_order;
 };
 
+// grids (state, spin columns) one work-group of the projection pass takes
+constexpr int kProjColChunk = 16;
+
 enum PassKind { kPassFwd = 0, kPassBwd = 1, kPassKin = 2, kPassBwdCombine = 3, kPassPrec = 4 };
 
 // Device projector tables (compact per-atom storage).
@@ -228,6 +231,8 @@ struct fbse_ctx {
   long *d_step = nullptr;       // device step counter of the replayed window
   long graph_replays = 0;       // replays in the last filter
   int last_window = 0;          // accumulation window depth K of the last filter
+  long last_runs = 0;           // consecutive sub-batch runs of the start states in the last filter
+  long run_starts = 0;          // start states per filter run (0: chosen by the filter)
   int last_group = 0;           // contexts that shared grouped FFT launches in the last fbse_filter_many (1: own)
   double tune_ms[2][2][3][3] = {};
   int autotune = 1;

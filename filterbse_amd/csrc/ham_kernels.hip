@@ -38,7 +38,7 @@ template <typename C> __device__ __forceinline__ C cjm(C a, C b) {
 // walks every kPjLanes-th point of the sphere holding all 30 projections of
 // its column in registers; the lanes are then summed through LDS.
 // ---------------------------------------------------------------------------
-constexpr int kPjCols = 16;
+constexpr int kPjCols = kProjColChunk;
 constexpr int kPjLanes = 16;  // kPjCols * kPjLanes = 256 threads
 
 template <typename T, int MAXP, bool EXPAND>

@@ -267,6 +267,18 @@ long fbse_graph_replays(const fbse_ctx *ctx);
 /* Accumulation window depth K of the last filter call (fbse_set_window, or
  * the depth the free memory allowed); for reporting. */
 int fbse_last_window(const fbse_ctx *ctx);
+/* Consecutive runs the last fbse_filter call split its start states into (1:
+ * one batch).  A large batch runs as sub-batches of the start states when
+ * their deeper rings (fewer items per ring slot) lower the accumulation's
+ * traffic by more than the extra runs cost; fbse_set_window keeps the batch
+ * whole.  For reporting.  No counterpart in the reference (its ranks filter
+ * one start each, filter.c:102-116). */
+long fbse_last_runs(const fbse_ctx *ctx);
+/* Start states per filter run: 0 (default) chooses as above; n > 0 runs the
+ * start states in consecutive sub-batches of n (real states: n rounded up to
+ * an even count), whatever the window depth.  Results equal one batch's at the
+ * same window depth.  No counterpart in the reference. */
+int fbse_set_run_starts(fbse_ctx *ctx, long n);
 /* Dots whose FFT passes ran as shared grouped launches with this context in
  * its last fbse_filter_many call (1: its own launches; 0: no such call yet).
  * For reporting.  No counterpart in the reference (one dot per run). */

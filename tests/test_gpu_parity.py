@@ -115,6 +115,28 @@ def test_filter_mixed(system, variant, lines):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
+def test_filter_sub_batches(system, precision):
+    """A filter whose start states run as consecutive sub-batches (fbse_set_run_starts:
+    2 starts per run over 5 starts, real states in packed pairs) gives the same
+    bits as one batch at the same window depth, against the oracle as well."""
+    name, s, h, g = system
+    n_starts, ms, M = 5, 3, 30
+    psi_rank, stlen, an, zn, Vmin, dE_1 = filter_inputs(s, n_starts, ms, M)
+    g.set_window(4)
+    one = g.filter(psi_rank, n_starts, ms, an, zn, Vmin, dE_1, precision=precision)
+    assert g.last_runs() == 1
+    g.set_run_starts(2)
+    sub = g.filter(psi_rank, n_starts, ms, an, zn, Vmin, dE_1, precision=precision)
+    assert g.last_runs() == 3
+    g.set_run_starts(0)
+    g.set_window(0)
+    assert np.array_equal(sub, one), (name, rel(sub, one))
+    if precision == 0:
+        ref = h.filter_cycles(psi_rank, n_starts, ms, an, zn, Vmin, dE_1)
+        assert rel(sub, ref) < 1e-10, name
+
+
+@pytest.mark.parametrize("precision", [0, 1])
 def test_filter_graph_replay(system, precision):
     """Full accumulation windows replayed from one captured hipGraph give the
     same bits as launching every kernel from the host; the graph path is run
