@@ -84,6 +84,9 @@ template <typename T> struct AxisArgs {
   // follow each other on one XCD (work-group b runs on XCD b mod 8), so the
   // row block's records are fetched into one L2 once
   int xcd_order;
+  // combine of spinors (16-line fixed instances): a tile holds 8 rows of both
+  // spins of one state (batch items in spin pairs, ny a multiple of 8)
+  int spin_pair;
 };
 
 // grids (state, spin columns) one work-group of the projection pass takes

@@ -405,10 +405,14 @@ static int fft_pass_lines(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src,
       A.pt_ranks = c->pt_ranks;
     }
     A.fwd_dst = (cplx_t<T> *)c->ax_fwd;
+    // spinors in the automatic mode: tiles of 8 rows of both spins of a state
+    A.spin_pair = c->axis_fuse == 1 && c->nspin == 2 && lines == 16 && c->ny % 8 == 0 ? 1 : 0;
   }
   // the combine's tiles of one row block for all batch items on one XCD: its
   // V_loc rows (and expansion records) are then fetched into one L2 once
-  if (kind == kPassBwdCombine && fx && c->ny % lines == 0) A.xcd_order = (c->ny * c->nz / lines) % 8 == 0 ? 1 : 0;
+  const int block_rows = A.spin_pair ? lines / 2 : lines;
+  if (kind == kPassBwdCombine && fx && c->ny % block_rows == 0)
+    A.xcd_order = (c->ny * c->nz / block_rows) % 8 == 0 ? 1 : 0;
   if (fx) {
     hipEvent_t ev = nullptr;
     const int pk = (kind == kPassBwdCombine) ? 1 : 0;

@@ -768,9 +768,14 @@ FBSE_FX_F32(FBSE_FX_G32)
 // passes, and each work-group loops over tiles.  x tiles are L consecutive
 // rows; y and z tiles are L consecutive inner indices (each axis point a
 // contiguous 8*L- or 16*L-byte segment).
-template <typename T, int KIND, int N, int R1, int R2, int L, int NT, bool XAX, int EXP_GROUP>
+// SP (spin-paired combine, x rows): a tile holds rows y0 .. y0+L/2-1 of one
+// plane of both spins of one state (lines l < L/2 spin up, the rest spin
+// down), so the tile's projector records are read once for the two grids.
+template <typename T, int KIND, int N, int R1, int R2, int L, int NT, bool XAX, int EXP_GROUP, bool SP = false>
 __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
   using C = cplx_t<T>;
+  static_assert(!SP || (XAX && KIND == kPassBwdCombine && L % 2 == 0), "spin-paired tiles are combine tiles");
+  constexpr int HL = SP ? L / 2 : L;  // rows of one grid in a tile
   constexpr int LS = N | 1, E = N * L / NT;
   constexpr int LB = E % 8 == 0 ? 8 : (E % 5 == 0 ? 5 : E);  // global rows in flight per batch
   static_assert(NT % L == 0 && E * NT == N * L && E % LB == 0 && R1 * R2 == N, "fixed axis tile shape");
@@ -794,18 +799,25 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
     int nl;
     if (XAX) {
       long tt = t;
+      // batch items of the tile order: grids, or states when spin-paired
+      const long rows = (long)A.ny * A.nz, nbat = A.total_lines / rows / (SP ? 2 : 1);
       if (KIND == kPassBwdCombine && A.xcd_order) {
         // t -> XCD t mod 8, place s = t / 8 in that XCD's sequence of (row
         // block, batch item) pairs, row blocks xcd, xcd + 8, ... (the host
         // sets xcd_order only when the row blocks per item are a multiple of 8)
-        const long rows = (long)A.ny * A.nz, nbat = A.total_lines / rows;
         const long s = t >> 3, q = s / nbat;
-        tt = (s - q * nbat) * (rows / L) + (t & 7) + 8 * q;
+        tt = (s - q * nbat) * (rows / HL) + (t & 7) + 8 * q;
       }
-      line0 = tt * L;
+      if (SP) {
+        const long st = tt / (rows / HL);
+        line0 = st * 2 * rows + (tt - st * (rows / HL)) * HL;  // spin-up grid's first row
+        nl = L;
+      } else {
+        line0 = tt * L;
+        const long rem = A.total_lines - line0;
+        nl = rem < L ? (int)rem : L;
+      }
       base = line0 * N;
-      const long rem = A.total_lines - line0;
-      nl = rem < L ? (int)rem : L;
     } else {
       const long o = t / A.tiles_per_outer;
       i0 = (t - o * A.tiles_per_outer) * L;
@@ -814,8 +826,13 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
       nl = rem < L ? (int)rem : L;
     }
     const unsigned inner = (unsigned)A.inner;
+    // spin-paired: line l at (l / HL) grids and (l % HL) rows from the tile base
+    const unsigned rowsN = (unsigned)A.ny * (unsigned)A.nz * (unsigned)N;
+    auto xoff = [&](int e) -> unsigned {
+      return SP ? (unsigned)((e / N) / HL) * rowsN + (unsigned)(((e / N) % HL) * N + e % N) : (unsigned)e;
+    };
     auto goff = [&](int e) -> unsigned {
-      return XAX ? (unsigned)e : (unsigned)(e / L) * inner + (unsigned)(e % L);
+      return XAX ? xoff(e) : (unsigned)(e / L) * inner + (unsigned)(e % L);
     };
     auto lpos = [&](int e) -> int { return XAX ? (e / N) * LS + e % N : (e % L) * LS + e / L; };
     auto line = [&](int e) -> int { return XAX ? e / N : e % L; };
@@ -878,15 +895,15 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
         const int e = tid + u * NT;
         const int l = e / N, p = e % N;
         if (l < nl) {
-          const long row = (line0 + l) % A.vloc_rows;
+          const long row = (line0 + (SP ? (long)(l / HL) * A.ny * A.nz + l % HL : l)) % A.vloc_rows;
           const T vv = A.vloc[row * N + p];
-          const C q = pin[e];
+          const C q = pin[xoff(e)];
           const C r = buf[l * LS + p];
           C o;
           o.x = A.a1 * (r.x + vv * q.x) + a0 * q.x;
           o.y = A.a1 * (r.y + vv * q.y) + a0 * q.y;
           if (keep) buf[l * LS + p] = o;
-          if (!expand) dst[e] = o;
+          if (!expand) dst[xoff(e)] = o;
         }
       }
       if (expand) {
@@ -899,11 +916,14 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
         // thread takes kG records at a time and issues all their loads before
         // it adds any term: a record's loads wait on its position and atom,
         // and a loop of one record per step would wait once per record.
-        constexpr int kG = EXP_GROUP;
+        // (spin-paired: every record adds its term to the spin-up row with the
+        // spin-up coefficients and to the spin-down row with the spin-down ones)
+        constexpr int kG = SP ? (EXP_GROUP + 1) / 2 : EXP_GROUP;
         const long rows = (long)A.ny * A.nz;
         const long item = line0 / rows, zy = line0 - item * rows;
         const int z = (int)(zy / A.ny), y0 = (int)(zy - (long)z * A.ny);
         const C *dcol = A.pjd + item * A.pj_natoms * 6;
+        const C *dcol2 = dcol + (long)A.pj_natoms * 6;  // SP: the spin-down grid (item + 1)
         const int2 *pa = reinterpret_cast<const int2 *>(A.rec_pa);
         const int nr = A.pt_ranks;
         int *seg_start = reinterpret_cast<int *>(kzl), *seg_len = seg_start + nr;
@@ -911,7 +931,7 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
           const int *ro = A.rec_yoff + ((long)z * nr + tid) * (A.ny + 1) + y0;
           const int s0 = ro[0];
           seg_start[tid] = s0;
-          seg_len[tid] = ro[L] - s0;
+          seg_len[tid] = ro[HL] - s0;
         }
         __syncthreads();  // also orders the combine's LDS writes before the rounds
         for (int k = 0; k < nr; ++k) {
@@ -919,7 +939,7 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
           if (n0 == 0) continue;
           for (int b0 = 0; b0 < n0; b0 += kG * NT) {
             int2 q[kG];
-            C tv[kG];
+            C tv[kG], tv2[kG];
 #pragma unroll
             for (int g = 0; g < kG; ++g) {
               const int i = b0 + g * NT + tid;
@@ -927,8 +947,16 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
             }
 #pragma unroll
             for (int g = 0; g < kG; ++g)
-              if (q[g].x >= 0)
-                tv[g] = record_term<T>(A.rec_val + (long)(s0 + b0 + g * NT + tid) * 8, dcol + (long)q[g].y * 6);
+              if (q[g].x >= 0) {
+                if (SP) {
+                  T rv[8];
+                  load_record<T>(A.rec_val + (long)(s0 + b0 + g * NT + tid) * 8, rv);
+                  tv[g] = record_term_r<T>(rv, dcol + (long)q[g].y * 6);
+                  tv2[g] = record_term_r<T>(rv, dcol2 + (long)q[g].y * 6);
+                } else {
+                  tv[g] = record_term<T>(A.rec_val + (long)(s0 + b0 + g * NT + tid) * 8, dcol + (long)q[g].y * 6);
+                }
+              }
 #pragma unroll
             for (int g = 0; g < kG; ++g) {
               if (q[g].x >= 0) {
@@ -937,6 +965,12 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
                 o.x += tv[g].x;
                 o.y += tv[g].y;
                 buf[pos] = o;
+                if (SP) {
+                  C o2 = buf[pos + HL * LS];
+                  o2.x += tv2[g].x;
+                  o2.y += tv2[g].y;
+                  buf[pos + HL * LS] = o2;
+                }
               }
             }
           }
@@ -946,7 +980,7 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
         for (int u = 0; u < E; ++u) {
           const int e = tid + u * NT;
           const int l = e / N, p = e % N;
-          if (l < nl) dst[e] = buf[l * LS + p];
+          if (l < nl) dst[xoff(e)] = buf[l * LS + p];
         }
       }
       if (A.fwd_dst) {
@@ -958,7 +992,7 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
         for (int u = 0; u < E; ++u) {
           const int e = tid + u * NT;
           const int l = e / N, p = e % N;
-          if (l < nl) fd[e] = buf[l * LS + p];
+          if (l < nl) fd[xoff(e)] = buf[l * LS + p];
         }
       }
     } else {
@@ -991,10 +1025,29 @@ FBSE_AXIS_FX(axis_fx_f64_n256_l32, double, 256, 16, 16, 32, 512, 2, 4)
 FBSE_AXIS_FX(axis_fx_f64_n160_l16, double, 160, 16, 10, 16, 256, 2, 4)
 FBSE_AXIS_FX(axis_fx_f64_n160_l32, double, 160, 16, 10, 32, 512, 2, 4)
 #undef FBSE_AXIS_FX
+// spin-paired combine tiles (16 lines: 8 rows of each spin)
+#define FBSE_AXIS_FX_SP(NAME, T, N, R1, R2, NT, WAVES, G)                                                   \
+  template <int KIND, bool XAX>                                                                          \
+  __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WAVES))) void NAME(AxisArgs<T> A) {  \
+    axis_fx_body<T, KIND, N, R1, R2, 16, NT, XAX, G, true>(A);                                           \
+  }
+FBSE_AXIS_FX_SP(axis_fx_f32_n256_p16, float, 256, 16, 16, 256, 4, 3)
+FBSE_AXIS_FX_SP(axis_fx_f32_n160_p16, float, 160, 16, 10, 256, 4, 4)
+FBSE_AXIS_FX_SP(axis_fx_f64_n256_p16, double, 256, 16, 16, 256, 2, 4)
+FBSE_AXIS_FX_SP(axis_fx_f64_n160_p16, double, 160, 16, 10, 256, 2, 4)
+#undef FBSE_AXIS_FX_SP
 
 bool axis_fx_length(int n) { return n == 160 || n == 256; }
 
-template <typename T, int KIND, bool XAX> static const void *axis_fx_fn(int n, int lines) {
+template <typename T, int KIND, bool XAX> static const void *axis_fx_fn(int n, int lines, bool sp) {
+  if constexpr (KIND == kPassBwdCombine && XAX) {
+    if (sp) {
+      if constexpr (sizeof(T) == 4)
+        return n == 256 ? (const void *)axis_fx_f32_n256_p16<KIND, XAX> : (const void *)axis_fx_f32_n160_p16<KIND, XAX>;
+      else
+        return n == 256 ? (const void *)axis_fx_f64_n256_p16<KIND, XAX> : (const void *)axis_fx_f64_n160_p16<KIND, XAX>;
+    }
+  }
   if constexpr (sizeof(T) == 4) {
     if (lines == 64)
       return n == 256 ? (const void *)axis_fx_f32_n256_l64<KIND, XAX> : (const void *)axis_fx_f32_n160_l64<KIND, XAX>;
@@ -1011,12 +1064,13 @@ template <typename T, int KIND, bool XAX> static const void *axis_fx_fn(int n, i
 template <typename T, int KIND, bool XAX>
 static int axis_fx_launch(fbse_ctx *c, const AxisArgs<T> &A, int n, int lines) {
   // device, precision, length, lines, kind, axis -> resident work-groups
-  static int cache[64][2][2][3][5][2] = {};
+  static int cache[64][2][2][4][5][2] = {};
+  const bool sp = KIND == kPassBwdCombine && XAX && A.spin_pair && lines == 16;
   const int d = c->device, pi = sizeof(T) == 8 ? 0 : 1, ni = n == 256 ? 1 : 0;
-  const int li = lines == 64 ? 2 : (lines == 32 ? 1 : 0);
+  const int li = sp ? 3 : (lines == 64 ? 2 : (lines == 32 ? 1 : 0));
   const int nt = lines * 16;  // 256, 512 or 1024 threads
   const size_t lds = (size_t)(lines * (n | 1) + n) * sizeof(cplx_t<T>) + (size_t)(n + lines) * sizeof(T);
-  const void *fn = axis_fx_fn<T, KIND, XAX>(n, lines);
+  const void *fn = axis_fx_fn<T, KIND, XAX>(n, lines, sp);
   if (d < 0 || d >= 64) return set_err(c, FBSE_ERR_UNSUPPORTED, "device %d", d);
   int &res = cache[d][pi][ni][li][KIND][XAX ? 1 : 0];
   if (res == 0) {
@@ -1032,6 +1086,19 @@ static int axis_fx_launch(fbse_ctx *c, const AxisArgs<T> &A, int n, int lines) {
   if (res < 0) return set_err(c, FBSE_ERR_UNSUPPORTED, "fixed axis instance %d/%d not resident", n, lines);
   if (A.ntiles <= 0) return FBSE_OK;
   const dim3 g((unsigned)std::min<long>(res, A.ntiles)), b((unsigned)nt);
+  if constexpr (KIND == kPassBwdCombine && XAX) {
+    if (sp) {
+      if constexpr (sizeof(T) == 4) {
+        if (n == 256) hipLaunchKernelGGL((axis_fx_f32_n256_p16<KIND, XAX>), g, b, lds, c->stream, A);
+        else hipLaunchKernelGGL((axis_fx_f32_n160_p16<KIND, XAX>), g, b, lds, c->stream, A);
+      } else {
+        if (n == 256) hipLaunchKernelGGL((axis_fx_f64_n256_p16<KIND, XAX>), g, b, lds, c->stream, A);
+        else hipLaunchKernelGGL((axis_fx_f64_n160_p16<KIND, XAX>), g, b, lds, c->stream, A);
+      }
+      FBSE_HIP(c, hipGetLastError());
+      return FBSE_OK;
+    }
+  }
   if constexpr (sizeof(T) == 4) {
     if (n == 256 && lines == 16) hipLaunchKernelGGL((axis_fx_f32_n256_l16<KIND, XAX>), g, b, lds, c->stream, A);
     else if (n == 256 && lines == 32) hipLaunchKernelGGL((axis_fx_f32_n256_l32<KIND, XAX>), g, b, lds, c->stream, A);

@@ -252,9 +252,12 @@ int fbse_set_graph(fbse_ctx *ctx, int enable);
  * V_loc, Newton shift) also adds the projector expansion of its rows and
  * writes the forward x transform of its output, which the next Newton step
  * starts from; 0 runs the expansion and the forward x transform as passes of
- * their own.  Results are identical.  No counterpart in the reference (its
- * kinetic and projector terms are separate loops, hamiltonian.c:62-99,
- * phamiltonian.c:140-444). */
+ * their own.  enable = 1 (default): tiles of 16 rows of one grid, for spinors
+ * 8 rows of both spins of one state (the tile's projector records are read
+ * once for the two grids); 16, 32 or 64 (FP32): tiles of that many
+ * consecutive rows of one grid.  Results are identical.  No counterpart in
+ * the reference (its kinetic and projector terms are separate loops,
+ * hamiltonian.c:62-99, phamiltonian.c:140-444). */
 int fbse_set_axis_fusion(fbse_ctx *ctx, int enable);
 
 /* Releases the context's work buffers (the filter's ring of Newton terms,

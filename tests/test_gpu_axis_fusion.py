@@ -8,7 +8,9 @@ oracle.  Windows are forced short so that graph replays carry the chained
 forward transform from one window into the next.  Grids: 160 x 32 x 48 (the
 combine's row blocks per item, 96, a multiple of 8: XCD tile order on) and
 256 x 16 x 20 (20 row blocks: plain order), spinors with SO + NL projectors
-and packed real states with NL projectors."""
+and packed real states with NL projectors.  Spinor combine tiles hold 8 rows
+of both spins of one state by default (fusion 1) or 16 consecutive rows of
+one grid (fusion 16): both must equal the separate passes bit for bit."""
 import numpy as np
 import pytest
 
@@ -42,6 +44,11 @@ def test_fused_combine_identical(case, precision):
     fused = g.filter(psi_rank, n_starts, ms, an, zn, Vmin, dE_1, precision=precision)
     assert g.graph_replays() > 0
     assert np.array_equal(fused, sep)
+    # spinors: the default tiles pair 8 rows of both spins; 16 consecutive rows of one grid
+    g.set_axis_fusion(16)
+    rows16 = g.filter(psi_rank, n_starts, ms, an, zn, Vmin, dE_1, precision=precision)
+    g.set_axis_fusion(1)
+    assert np.array_equal(rows16, sep)
     if precision == 0:
         ref = oracle_ham(s).filter_cycles(psi_rank, n_starts, ms, an, zn, Vmin, dE_1)
         err = np.abs(fused - ref).max() / np.abs(ref).max()
