@@ -934,47 +934,95 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
           seg_len[tid] = ro[HL] - s0;
         }
         __syncthreads();  // also orders the combine's LDS writes before the rounds
-        for (int k = 0; k < nr; ++k) {
-          const int s0 = seg_start[k], n0 = seg_len[k];
-          if (n0 == 0) continue;
-          for (int b0 = 0; b0 < n0; b0 += kG * NT) {
-            int2 q[kG];
-            C tv[kG], tv2[kG];
-#pragma unroll
-            for (int g = 0; g < kG; ++g) {
-              const int i = b0 + g * NT + tid;
-              q[g] = i < n0 ? pa[s0 + i] : int2{-1, 0};
+        if constexpr (SP) {
+          // A thread's first record of a round (position, atom, values) is
+          // loaded while the round before is processed (across its barrier), so
+          // that record waits once, for its atom's coefficients; the rest of
+          // the round takes kG records at a time, their loads issued together.
+          auto load_one = [&](int k, int i, int2 &q, T (&rv)[8]) {
+            if (i < seg_len[k]) {
+              const int e = seg_start[k] + i;
+              q = pa[e];
+              load_record<T>(A.rec_val + (long)e * 8, rv);
+            } else {
+              q = int2{-1, 0};
             }
+          };
+          auto add_one = [&](const int2 &q, const C &tv, const C &tv2) {
+            if (q.x < 0) return;
+            const int pos = ((q.x >> 16) - y0) * LS + (q.x & 0xffff);
+            C o = buf[pos];
+            o.x += tv.x;
+            o.y += tv.y;
+            buf[pos] = o;
+            C o2 = buf[pos + HL * LS];
+            o2.x += tv2.x;
+            o2.y += tv2.y;
+            buf[pos + HL * LS] = o2;
+          };
+          auto terms = [&](const int2 &q, const T (&rv)[8], C &tv, C &tv2) {
+            if (q.x < 0) return;
+            tv = record_term_r<T>(rv, dcol + (long)q.y * 6);
+            tv2 = record_term_r<T>(rv, dcol2 + (long)q.y * 6);
+          };
+          // round k from the preloaded (cq, cv), which then take round k + 1's
+          // first record (in flight during the round's LDS adds and barrier)
+          auto round = [&](int k, int2 &cq, T (&cv)[8]) {
+            {
+              C tv{}, tv2{};
+              terms(cq, cv, tv, tv2);
+              const int2 q0 = cq;
+              if (k + 1 < nr) load_one(k + 1, tid, cq, cv);
+              add_one(q0, tv, tv2);
+            }
+            for (int b0 = NT; b0 < seg_len[k]; b0 += kG * NT) {
+              int2 q[kG];
+              T rv[kG][8];
+              C tv[kG], tv2[kG];
 #pragma unroll
-            for (int g = 0; g < kG; ++g)
-              if (q[g].x >= 0) {
-                if (SP) {
-                  T rv[8];
-                  load_record<T>(A.rec_val + (long)(s0 + b0 + g * NT + tid) * 8, rv);
-                  tv[g] = record_term_r<T>(rv, dcol + (long)q[g].y * 6);
-                  tv2[g] = record_term_r<T>(rv, dcol2 + (long)q[g].y * 6);
-                } else {
+              for (int g = 0; g < kG; ++g) load_one(k, b0 + g * NT + tid, q[g], rv[g]);
+#pragma unroll
+              for (int g = 0; g < kG; ++g) terms(q[g], rv[g], tv[g], tv2[g]);
+#pragma unroll
+              for (int g = 0; g < kG; ++g) add_one(q[g], tv[g], tv2[g]);
+            }
+            __syncthreads();
+          };
+          int2 qa;
+          T va[8];
+          if (nr > 0) load_one(0, tid, qa, va);
+          for (int k = 0; k < nr; ++k) round(k, qa, va);
+        } else {
+          // inside a round a thread takes kG records at a time and issues all
+          // their loads before it adds any term
+          for (int k = 0; k < nr; ++k) {
+            const int s0 = seg_start[k], n0 = seg_len[k];
+            if (n0 == 0) continue;
+            for (int b0 = 0; b0 < n0; b0 += kG * NT) {
+              int2 q[kG];
+              C tv[kG];
+#pragma unroll
+              for (int g = 0; g < kG; ++g) {
+                const int i = b0 + g * NT + tid;
+                q[g] = i < n0 ? pa[s0 + i] : int2{-1, 0};
+              }
+#pragma unroll
+              for (int g = 0; g < kG; ++g)
+                if (q[g].x >= 0)
                   tv[g] = record_term<T>(A.rec_val + (long)(s0 + b0 + g * NT + tid) * 8, dcol + (long)q[g].y * 6);
-                }
-              }
 #pragma unroll
-            for (int g = 0; g < kG; ++g) {
-              if (q[g].x >= 0) {
-                const int pos = ((q[g].x >> 16) - y0) * LS + (q[g].x & 0xffff);
-                C o = buf[pos];
-                o.x += tv[g].x;
-                o.y += tv[g].y;
-                buf[pos] = o;
-                if (SP) {
-                  C o2 = buf[pos + HL * LS];
-                  o2.x += tv2[g].x;
-                  o2.y += tv2[g].y;
-                  buf[pos + HL * LS] = o2;
+              for (int g = 0; g < kG; ++g) {
+                if (q[g].x >= 0) {
+                  const int pos = ((q[g].x >> 16) - y0) * LS + (q[g].x & 0xffff);
+                  C o = buf[pos];
+                  o.x += tv[g].x;
+                  o.y += tv[g].y;
+                  buf[pos] = o;
                 }
               }
             }
+            __syncthreads();
           }
-          __syncthreads();
         }
 #pragma unroll 4
         for (int u = 0; u < E; ++u) {
