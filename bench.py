@@ -1,15 +1,18 @@
 """Benchmark of the filter-diagonalization path on one MI355X.
 
 Default workload: the full-stack configuration, a 4.5 nm spin-orbit
-zincblende dot on a 256^3 grid, 16 ran2 start states x 16 energy targets,
-Newton length 2048.  Two passes of the path run with the start states in HBM:
+zincblende dot on a 256^3 grid, 16 ran2 start states x 16 energy targets.
+Two passes of the path run with the start states in HBM:
 
 * mixed + warm: the learned warm start (network trained beforehand, untimed,
   on small dots) shapes the start states, the Newton filter runs in FP32
-  complex with FP64 coefficients and accumulators, then normalize_all, FP64
-  orthogonalisation (rank cut 1e-10), Rayleigh-Ritz, sigma_E and one FP64
-  refinement sweep;
-* plain FP64: ran2 start states, FP64 filter, normalize_all, ortho, RR, sigma_E.
+  complex with FP64 coefficients and accumulators at the configuration's
+  reduced Newton length (1152: the smallest at which the warm-started pass
+  converges the band-edge pairs; plain starts converge none there), then
+  normalize_all, FP64 orthogonalisation (rank cut 1e-10), Rayleigh-Ritz,
+  sigma_E and one FP64 refinement sweep;
+* plain FP64: ran2 start states, FP64 filter of Newton length 2048,
+  normalize_all, ortho, RR, sigma_E.
 
 value = eigenpairs with sigma_E < 1e-4 Ha per second of the mixed + warm pass;
 the FP64 pass and the eigenvalue error of every FP64-converged state against
@@ -92,6 +95,17 @@ def setup(cfg_name, dev, ncheby=0):
                 sig_f=guard["sigma_f"], guard=guard)
 
 
+def with_newton_length(S, M):
+    """The pass state S with Newton length M (Leja points, coefficients and
+    filter width for the same targets and spectrum bounds)."""
+    from filterbse_amd import api
+    Sm = dict(S)
+    guard, dt = filter_guard(S["targets"], S["dE"], M)
+    Sm.update(M=M, zn=api.leja_points(M), sig_f=guard["sigma_f"], guard=guard)
+    Sm["an"] = api.newton_coeff(Sm["zn"], S["targets"], S["Vmin"], S["dE"], dt)
+    return Sm
+
+
 def warm_start_network(S):
     """Trains the warm-start network (untimed set-up: small training dots,
     plain FP64 filters, Adam on the GPU) and keeps the dot's V_loc on the device."""
@@ -172,6 +186,20 @@ def per_state_errors(ev64, sig64, ev_m, sig_m):
         j = int(np.argmin(np.abs(ev_m - ev64[i])))
         out.append(dict(e_fp64=float(ev64[i]), e_mixed=float(ev_m[j]), abs_err=float(abs(ev_m[j] - ev64[i])),
                         sigma_fp64=float(sig64[i]), sigma_mixed=float(sig_m[j])))
+    return out
+
+
+def mixed_state_errors(ev_m, sig_m, ev64, sig64):
+    """For every state the mixed pass converged: its eigenvalue, the nearest
+    Ritz value of the FP64 pass (converged or not), |difference| and both
+    residuals."""
+    out = []
+    for j in np.argsort(ev_m):
+        if sig_m[j] >= RESID:
+            continue
+        i = int(np.argmin(np.abs(ev64 - ev_m[j])))
+        out.append(dict(e_mixed=float(ev_m[j]), e_fp64=float(ev64[i]), abs_err=float(abs(ev_m[j] - ev64[i])),
+                        sigma_mixed=float(sig_m[j]), sigma_fp64=float(sig64[i])))
     return out
 
 
@@ -549,7 +577,9 @@ def main():
                     help="only run an M-step mixed filter of the workload and one device copy of the bytes of "
                          "one FFT launch (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs), then exit")
     ap.add_argument("--ncheby", type=int, default=0, metavar="M",
-                    help="Newton length override (short filters for counter runs of the other stages)")
+                    help="Newton length override of both passes (short filters for counter runs of the other stages)")
+    ap.add_argument("--ncheby-warm", type=int, default=-1, metavar="M",
+                    help="Newton length of the warm-started mixed pass (-1: the configuration's, 0: the FP64 pass's)")
     ap.add_argument("--mlp-sample", type=int, default=0, metavar="N",
                     help="only measure the warm-start network's MFMA kernels on N grid points, then exit")
     ap.add_argument("--gemm-sample", default="", metavar="SHAPES",
@@ -589,6 +619,10 @@ def main():
     warm = not args.no_warm
     if warm:
         warm_start_network(S)
+    S["windows"] = []
+    # the warm-started mixed pass runs at its own (reduced) Newton length
+    M_w = args.ncheby_warm if args.ncheby_warm >= 0 else cfg.get("ncheby_warm", 0)
+    Sw = with_newton_length(S, M_w) if warm and M_w > 0 and M_w != S["M"] and not args.ncheby else S
 
     # steps: Newton steps of the mixed filter over the whole batch
     if args.warmup:
@@ -599,10 +633,10 @@ def main():
     ms_per_step = 1e3 * t_steps / args.steps
 
     # the headline passes
-    log("mixed%s pass" % ("+warm" if warm else ""))
-    ev, sig, rank, st = one_pass(S, mixed, warm=warm, refine=args.refine)
+    log("mixed%s pass, Newton length %d, filter guard %s" % ("+warm" if warm else "", Sw["M"], Sw["guard"]))
+    ev, sig, rank, st = one_pass(Sw, mixed, warm=warm, refine=args.refine)
     t_mixed = float(st.sum())
-    refine_log = S["refine_log"]
+    refine_log = Sw["refine_log"]
     ev_c, sig_c = converged(ev, sig)
     log("mixed: %.3fs (start states %.3f filter %.3f ortho %.3f RR %.3f sigma %.3f refine %.3f), rank %d, "
         "converged %d, refine %s" % (t_mixed, *st, rank, len(ev_c), refine_log))
@@ -613,6 +647,7 @@ def main():
         t64 = float(st64.sum())
         ev64_c, _ = converged(ev64, sig64)
         errs = per_state_errors(ev64, sig64, ev, sig)
+        errs_m = mixed_state_errors(ev, sig, ev64, sig64)
         fp = dict(seconds_per_pass=round(t64, 3), eigenpairs_per_s=len(ev64_c) / t64, converged=int(len(ev64_c)),
                   rank=int(rank64), stage_s=[round(float(x), 4) for x in st64],
                   hpsi_per_s=round(S["n_starts"] * (S["M"] - 1) / st64[1], 1),
@@ -620,7 +655,7 @@ def main():
         fp["max_eig_err_mixed_vs_fp64"] = max([e["abs_err"] for e in errs]) if errs else None
         log("fp64: %.3fs/pass, rank %d, converged %d, per-state errors %s" % (t64, rank64, len(ev64_c), errs))
     else:
-        errs = []
+        errs, errs_m = [], []
 
     # per-kernel HIP events over a short profiled mixed filter (its launches
     # are those of the full filter; every launch on the library stream)
@@ -696,7 +731,7 @@ def main():
         log("cpu baseline")
         cpu = cpu_baseline(S)
         log("cpu baseline: %s" % cpu)
-    hpsi_pass = S["n_starts"] * (S["M"] - 1) / st[1]
+    hpsi_pass = S["n_starts"] * (Sw["M"] - 1) / st[1]
     fp64_eps = fp["eigenpairs_per_s"] if fp else None
     line = {
         "metric": METRIC,
@@ -713,11 +748,12 @@ def main():
         "data": "synthetic (seeded zincblende dot, ran2 start states%s)" % (", learned warm start" if warm else ""),
         "config": {
             "workload": "%s: %.1f nm dot, %d atoms + %d ligands, %s, grid %dx%dx%d (dx %.3g bohr), KEmax %.1f Ha, "
-                        "%d starts x %d targets %s, Newton length %d" % (
+                        "%d starts x %d targets %s, Newton length %s" % (
                             args.config, cfg["radius_nm"], job["n_core"], len(job["syms"]) - job["n_core"],
                             "SO+NL spinors" if cfg["so"] else "real states", job["nx"], job["ny"], job["nz"],
                             cfg["dx"], cfg.get("ke_max", 20.0), S["n_starts"], S["ms"],
-                            "(setTargets %d %d, VB %.4f..%.4f, CB %.4f..%.4f)" % tuple(cfg["targets"]), S["M"]),
+                            "(setTargets %d %d, VB %.4f..%.4f, CB %.4f..%.4f)" % tuple(cfg["targets"]),
+                            "%d (FP64 pass), %d (mixed + warm pass)" % (S["M"], Sw["M"])),
             "parallelism": "single GPU; start states batched in one filter",
             "step": "one Newton step of the mixed filter over the batch (H on every start state + accumulation)",
         },
@@ -732,7 +768,10 @@ def main():
         "fp64": fp,
         "mixed_warm_over_fp64": (round((len(ev_c) / t_mixed) / fp64_eps, 3) if fp64_eps else None),
         "per_state_error_vs_fp64": errs,
+        "mixed_converged_vs_fp64": errs_m,
         "filter_guard": S["guard"],
+        "ncheby_mixed_warm": Sw["M"],
+        "filter_guard_mixed_warm": Sw["guard"],
         "warm_start": S.get("mlp_info"),
         "energy_range_s": round(S["t_range"], 3),
         "roofline": roofline,

@@ -408,8 +408,13 @@ CONFIGS = {
     # the middle of the gap across the conduction-band edge (-0.161 Ha)
     # (set_ene_targets, init.c:401-478: VB targets end at VBmax, CB targets
     # start at CBmin, spacing (max - min) / n)
+    # ncheby_warm: the Newton length of the warm-started mixed pass, the
+    # smallest in a scan (768 ... 2048) at which it converges the band-edge
+    # pairs (sigma < 1e-4 Ha, one FP64 refinement sweep); plain ran2 starts
+    # converge none there, in FP64 or mixed
     "c4_so_256": dict(radius_nm=4.5, n_req=(256, 256, 256), dx=0.71, so=True, nl=True, n_starts=16, ms=16,
-                      ncheby=2048, ke_max=5.0, targets=(8, 8, -0.3134, -0.2062, -0.19, -0.0828), builder="gpu"),
+                      ncheby=2048, ncheby_warm=1152, ke_max=5.0, targets=(8, 8, -0.3134, -0.2062, -0.19, -0.0828),
+                      builder="gpu"),
 }
 
 # Batched uneven dots: 32 spin-orbit dots with seeded radii uniform in
