@@ -87,6 +87,16 @@ template <typename T> struct AxisArgs {
   // combine of spinors (16-line fixed instances): a tile holds 8 rows of both
   // spins of one state (batch items in spin pairs, ny a multiple of 8)
   int spin_pair;
+  // y and z passes of the 16-line fixed instances (not x): source / result
+  // in the z-tile layout, where the 16 x values of one x block of all z of
+  // one (item, y) are one contiguous block of nz * 16 elements: element
+  // (item, z, y, x) at item * zt_grid + (y * nx / 16 + x / 16) * zt_nz16 +
+  // z * 16 + x % 16.  A tile of outer index o and first inner index i0 starts
+  // at (o / zt_q) * zt_grid + (o % zt_q) * 16 + (i0 / 16) * zt_nz16 (zt_q = nz
+  // for y lines, 1 for z lines), axis points zt_inner apart (nx * nz for y
+  // lines, 16 for z lines).
+  int zt_src, zt_dst;
+  long zt_q, zt_inner, zt_grid, zt_nz16;
 };
 
 // grids (state, spin columns) one work-group of the projection pass takes
@@ -196,6 +206,9 @@ struct fbse_ctx {
   int axis_fuse = 1;
   const void *ax_pjd = nullptr;
   void *ax_fwd = nullptr;
+  // the Newton step's y and z passes in the z-tile layout (AxisArgs::zt_*):
+  // the next per-axis pass reads (ax_zt & 1) / writes (ax_zt & 2) that layout
+  int ax_zt = 0;
 
   // scratch (grown on demand)
   void *d_scratch = nullptr;

@@ -408,6 +408,17 @@ static int fft_pass_lines(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src,
     // spinors in the automatic mode: tiles of 8 rows of both spins of a state
     A.spin_pair = c->axis_fuse == 1 && c->nspin == 2 && lines == 16 && c->ny % 8 == 0 ? 1 : 0;
   }
+  // y and z passes of the Newton step in the z-tile layout (AxisArgs::zt_*)
+  if (c->ax_zt) {
+    if (axis == 0 || !fx || lines != 16 || c->nx % 16 != 0)
+      return set_err(c, FBSE_ERR_UNSUPPORTED, "z-tile layout needs the 16-line fixed y / z instances");
+    A.zt_src = c->ax_zt & 1;
+    A.zt_dst = (c->ax_zt >> 1) & 1;
+    A.zt_q = axis == 1 ? c->nz : 1;
+    A.zt_inner = axis == 1 ? c->nx * c->nz : 16;
+    A.zt_grid = c->ngrid;
+    A.zt_nz16 = c->nz * 16;
+  }
   // the combine's tiles of one row block for all batch items on one XCD: its
   // V_loc rows (and expansion records) are then fetched into one L2 once
   const int block_rows = A.spin_pair ? lines / 2 : lines;
@@ -665,6 +676,14 @@ template <typename T> static bool axis_fused(const fbse_ctx *c) {
          (c->force_lines == 0 || c->force_lines == 1000 + fused_lines<T>(c));
 }
 
+// The Newton step's y and z passes in the z-tile layout (automatic mode of
+// fbse_set_axis_fusion): 16-line fixed instances on y and z (lengths 160 or
+// 256, separable k^2), nx a multiple of 16, tile heights not forced otherwise.
+template <typename T> static bool axis_zt(const fbse_ctx *c) {
+  return c->axis_fuse == 1 && c->nx % 16 == 0 && axis_fx_length((int)c->ny) && axis_fx_length((int)c->nz) &&
+         c->ksqr_sep && (c->force_lines == 0 || c->force_lines == 1016);
+}
+
 template <typename T>
 static int apply_h_axes_chain(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *work, long nstates,
                               T a1, T a0, bool first) {
@@ -678,9 +697,22 @@ static int apply_h_axes_chain(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out,
     if ((rc = project_forked<T>(c, psi, nstates, (double)a1, d, &joined))) return rc;
   }
   rc = first ? fft_pass<T>(c, 0, kPassFwd, psi, work, nb, nullptr, 0, 0) : FBSE_OK;
-  if (!rc) rc = fft_pass<T>(c, 1, kPassFwd, work, work, nb, nullptr, 0, 0);
-  if (!rc) rc = fft_pass<T>(c, 2, kPassKin, work, work, nb, nullptr, 0, 0);
-  if (!rc) rc = fft_pass<T>(c, 1, kPassBwd, work, work, nb, nullptr, 0, 0);
+  if (axis_zt<T>(c)) {
+    // y forward from work into out in the z-tile layout, z in place there
+    // (contiguous tiles), y backward back into work: out (the ring slot of
+    // this step's result) is free until the combine writes it
+    c->ax_zt = 2;
+    if (!rc) rc = fft_pass_lines<T>(c, 1, kPassFwd, work, out, nb, nullptr, 0, 0, 1016);
+    c->ax_zt = 3;
+    if (!rc) rc = fft_pass_lines<T>(c, 2, kPassKin, out, out, nb, nullptr, 0, 0, 1016);
+    c->ax_zt = 1;
+    if (!rc) rc = fft_pass_lines<T>(c, 1, kPassBwd, out, work, nb, nullptr, 0, 0, 1016);
+    c->ax_zt = 0;
+  } else {
+    if (!rc) rc = fft_pass<T>(c, 1, kPassFwd, work, work, nb, nullptr, 0, 0);
+    if (!rc) rc = fft_pass<T>(c, 2, kPassKin, work, work, nb, nullptr, 0, 0);
+    if (!rc) rc = fft_pass<T>(c, 1, kPassBwd, work, work, nb, nullptr, 0, 0);
+  }
   if (!joined) (void)hipStreamWaitEvent(c->stream, c->ev_join, 0);  // the projections are done
   if (rc) return rc;
   c->ax_pjd = d;

@@ -795,7 +795,7 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
   int tid = threadIdx.x;
   for (long t = blockIdx.x; t < A.ntiles; t += gridDim.x) {
     asm volatile("" : "+v"(tid));  // per-tile recomputation of the lane's offsets
-    long base, i0 = 0, line0 = 0;
+    long base, i0 = 0, line0 = 0, zbase = 0;
     int nl;
     if (XAX) {
       long tt = t;
@@ -822,6 +822,8 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
       const long o = t / A.tiles_per_outer;
       i0 = (t - o * A.tiles_per_outer) * L;
       base = o * A.outer_stride + i0;
+      if (!XAX && L == 16 && (A.zt_src || A.zt_dst))
+        zbase = (o / A.zt_q) * A.zt_grid + (o % A.zt_q) * 16 + (i0 / 16) * A.zt_nz16;
       const long rem = A.inner_count - i0;
       nl = rem < L ? (int)rem : L;
     }
@@ -834,9 +836,14 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
     auto goff = [&](int e) -> unsigned {
       return XAX ? xoff(e) : (unsigned)(e / L) * inner + (unsigned)(e % L);
     };
+    // y / z lines: source and result in the reference or the z-tile layout
+    const bool zs = !XAX && L == 16 && A.zt_src, zd = !XAX && L == 16 && A.zt_dst;
+    const unsigned zinner = (unsigned)A.zt_inner;
+    auto soff = [&](int e) -> unsigned { return zs ? (unsigned)(e / L) * zinner + (unsigned)(e % L) : goff(e); };
+    auto doff = [&](int e) -> unsigned { return zd ? (unsigned)(e / L) * zinner + (unsigned)(e % L) : goff(e); };
     auto lpos = [&](int e) -> int { return XAX ? (e / N) * LS + e % N : (e % L) * LS + e / L; };
     auto line = [&](int e) -> int { return XAX ? e / N : e % L; };
-    const C *src = A.src + base;
+    const C *src = A.src + (zs ? zbase : base);
     __syncthreads();  // the previous tile's LDS reads are done (first tile: tables written)
 #pragma unroll
     for (int ub = 0; ub < E; ub += LB) {
@@ -845,7 +852,7 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
       for (int u = 0; u < LB; ++u) {
         const int e = tid + (ub + u) * NT;
         if (line(e) < nl) {
-          const C v = src[goff(e)];
+          const C v = src[soff(e)];
           vr[u] = v.x;
           vi[u] = v.y;
         }
@@ -884,7 +891,7 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
       __syncthreads();
       fx_fft<T, N, R1, R2, L, 1, LS, NT, true>(tw_off);
     }
-    C *dst = A.dst + base;
+    C *dst = A.dst + (zd ? zbase : base);
     if (KIND == kPassBwdCombine) {  // x axis: out = a1*(T psi + V psi_in) + a0*psi_in
       const T a0 = A.a0_tab ? (T)A.a0_tab[*A.step_base + A.step_off] : A.a0;
       const C *pin = A.psi_in + base;
@@ -1047,7 +1054,7 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
 #pragma unroll
       for (int u = 0; u < E; ++u) {
         const int e = tid + u * NT;
-        if (line(e) < nl) dst[goff(e)] = buf[lpos(e)];
+        if (line(e) < nl) dst[doff(e)] = buf[lpos(e)];
       }
     }
   }
