@@ -86,6 +86,7 @@ GPU_SIGNATURES = {
     "fbse_fft_choice": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "fbse_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "fbse_set_axis_fusion": (C.c_int, [C.c_void_p, C.c_int]),
+    "fbse_set_combine_projection": (C.c_int, [C.c_void_p, C.c_int]),
     "fbse_last_group": (C.c_int, [C.c_void_p]),
     "fbse_build_local_pot": (C.c_int, [C.c_int, C.c_long, DP, DP, DP, C.c_long, C.c_long, C.c_long, DP,
                                        C.POINTER(C.c_int), C.c_long, DP, DP, LP, DP, C.c_long, C.c_long, DP]),
@@ -459,6 +460,12 @@ class Context:
         """Fused per-axis combine (projector expansion and the next forward x
         transform inside the x combine pass) on (1, default) or off (0)."""
         self._check(gpu().fbse_set_axis_fusion(self.ctx, int(enable)))
+
+    def set_combine_projection(self, enable):
+        """Projections summed inside the spin-paired combine and reduced per
+        atom by the next Newton step (1, default) or the projection pass every
+        step (0)."""
+        self._check(gpu().fbse_set_combine_projection(self.ctx, int(enable)))
 
     def trim(self):
         """Release the context's work buffers (allocated again on demand)."""

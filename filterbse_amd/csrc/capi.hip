@@ -6,6 +6,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -62,7 +63,13 @@ static void release_bufs(fbse_ctx *c, int first) {
     *sizes[i] = 0;
   }
 }
-static void release_work(fbse_ctx *c) { release_bufs(c, 0); }
+static void release_work(fbse_ctx *c) {
+  release_bufs(c, 0);
+  if (c->d_pp_part) (void)hipFree(c->d_pp_part);
+  c->d_pp_part = nullptr;
+  c->pp_part_bytes = 0;
+  c->pp_ready = 0;
+}
 // Before a filter sizes its ring from the free memory: the second scratch
 // buffer and the partial Grams that ortho, Rayleigh-Ritz or the refinement
 // left behind (tens of GB on a 256^3 block) are released, so the window depth
@@ -387,6 +394,76 @@ static int build_projectors(fbse_ctx *c, const fbse_proj_desc *pd) {
         yoff[b * (c->ny + 1) + y] = e;
       }
     }
+    // Projections inside the spin-paired combine (AxisArgs::pp_*): per tile
+    // of 8 rows of one plane, its records of every rank grouped by atom
+    // (atoms ascending, a group's records in point order, padded with
+    // zero-weight records to a multiple of kPpRecs), kPpRecs records per lane;
+    // an atom's lanes within one wavefront are one segment, whose sum is one
+    // slot, and pp_span[tile] is the tile's longest segment in lanes, rounded
+    // up to a power of two.  Slots are numbered atom-major, each atom's in
+    // tile order.
+    if (c->nspin == 2 && c->ny % 8 == 0) {
+      const long tpz = c->ny / 8, ntile = c->nz * tpz;
+      struct Piece {
+        int atom, end_lane;
+      };
+      std::vector<Piece> pieces;
+      std::vector<long> piece_off(ntile + 1, 0);
+      std::vector<int> lane_off(ntile + 1, 0), lane_w, meta, span(ntile, 1);
+      std::vector<double> val;
+      std::vector<std::array<int, 3>> tmp;  // (atom, (y << 16) | x, record)
+      for (long tau = 0; tau < ntile; ++tau) {
+        const long z = tau / tpz, y0 = (tau % tpz) * 8;
+        tmp.clear();
+        for (int k = 0; k < nr; ++k) {
+          const size_t b = ((size_t)z * nr + k) * (c->ny + 1);
+          for (int e = yoff[b + y0]; e < yoff[b + y0 + 8]; ++e)
+            tmp.push_back({pa[2 * (size_t)e + 1], pa[2 * (size_t)e], e});
+        }
+        std::sort(tmp.begin(), tmp.end());
+        const int base = (int)lane_w.size();  // tile's first lane
+        for (size_t i = 0; i < tmp.size();) {
+          size_t j = i;
+          while (j < tmp.size() && tmp[j][0] == tmp[i][0]) ++j;
+          const int n = (int)(j - i), nlr = (n + kPpRecs - 1) / kPpRecs;
+          int seg = 0;
+          for (int q = 0; q < nlr; ++q) {
+            const int li = (int)lane_w.size() - base;  // lane within the tile
+            const bool head = q == 0 || li % 64 == 0;
+            lane_w.push_back(head ? 1 : 0);
+            seg = head ? 1 : seg + 1;
+            while (span[tau] < seg) span[tau] *= 2;
+            for (int u = 0; u < kPpRecs; ++u) {
+              const size_t r = i + (size_t)std::min(q * kPpRecs + u, n - 1);
+              const bool pad = q * kPpRecs + u >= n;
+              const int yx = tmp[r][1];
+              meta.push_back((((yx >> 16) - (int)y0) << 16) | (yx & 0xffff));
+              for (int v = 0; v < 8; ++v) val.push_back(pad ? 0.0 : pe[(size_t)tmp[r][2] * 8 + v]);
+            }
+            // the segment ends at the group's last lane or a wavefront's last
+            if (q == nlr - 1 || (li + 1) % 64 == 0) pieces.push_back({tmp[i][0], (int)lane_w.size() - 1});
+          }
+          i = j;
+        }
+        lane_off[tau + 1] = (int)lane_w.size();
+        piece_off[tau + 1] = (long)pieces.size();
+      }
+      std::vector<int> aoff(na + 1, 0);
+      for (const Piece &pc : pieces) ++aoff[pc.atom + 1];
+      for (long a = 0; a < na; ++a) aoff[a + 1] += aoff[a];
+      std::vector<int> seen(aoff.begin(), aoff.end() - 1);
+      for (const Piece &pc : pieces) lane_w[pc.end_lane] |= (seen[pc.atom]++ + 1) << 1;
+      c->pp_nslots = aoff[na];
+      auto valf = cast_vec<float>(val.data(), val.size());
+      if ((rc = upload(c, (void **)&c->d_pp_lane_off, lane_off.data(), lane_off.size() * sizeof(int))) ||
+          (rc = upload(c, (void **)&c->d_pp_lanes, lane_w.data(), lane_w.size() * sizeof(int))) ||
+          (rc = upload(c, (void **)&c->d_pp_span, span.data(), span.size() * sizeof(int))) ||
+          (rc = upload(c, (void **)&c->d_pp_meta, meta.data(), meta.size() * sizeof(int))) ||
+          (rc = upload(c, (void **)&c->d_pp_val, val.data(), val.size() * sizeof(double))) ||
+          (rc = upload(c, (void **)&c->d_pp_val_f, valf.data(), valf.size() * sizeof(float))) ||
+          (rc = upload(c, (void **)&c->d_pp_atom_off, aoff.data(), aoff.size() * sizeof(int))))
+        return rc;
+    }
     auto pef = cast_vec<float>(pe.data(), (long)pe.size());
     if ((rc = upload(c, (void **)&c->d_rec_yoff, yoff.data(), yoff.size() * sizeof(int))) ||
         (rc = upload(c, (void **)&c->d_rec_off, roff.data(), roff.size() * sizeof(int))) ||
@@ -569,7 +646,8 @@ extern "C" void fbse_destroy(fbse_ctx *c) {
                   c->d_ksqr_f, c->d_scratch, c->d_scratch2, c->d_red, c->pj.a_off, c->pj.a_cnt, c->pj.jxyz,
                   c->pj.sign, c->pj.proj, c->pj.nlproj, c->pj.y1, c->pj.ls, c->pj_f.proj, c->pj_f.nlproj,
                   c->pj_f.y1, c->pj_f.ls, c->pj.color_atoms, c->pj.spatial_atoms, c->d_step, c->d_rec_off, c->d_rec_pa,
-                  c->d_rec_val, c->d_rec_val_f, c->d_pjd, c->d_gram, c->d_rec_yoff};
+                  c->d_rec_val, c->d_rec_val_f, c->d_pjd, c->d_gram, c->d_rec_yoff, c->d_pp_lane_off,
+                  c->d_pp_lanes, c->d_pp_span, c->d_pp_meta, c->d_pp_val, c->d_pp_val_f, c->d_pp_atom_off, c->d_pp_part};
   for (void *p : ptrs)
     if (p) hipFree(p);
   for (int a = 0; a < 3; ++a) {
@@ -644,6 +722,12 @@ extern "C" int fbse_set_graph(fbse_ctx *c, int enable) {
 extern "C" int fbse_set_axis_fusion(fbse_ctx *c, int enable) {
   if (!c || !(enable == 0 || enable == 1 || enable == 16 || enable == 32 || enable == 64)) return FBSE_ERR_ARG;
   c->axis_fuse = enable;
+  return FBSE_OK;
+}
+extern "C" int fbse_set_combine_projection(fbse_ctx *c, int enable) {
+  if (!c || enable < 0 || enable > 1) return FBSE_ERR_ARG;
+  c->pp_on = enable;
+  c->pp_ready = 0;
   return FBSE_OK;
 }
 extern "C" int fbse_trim(fbse_ctx *c) {
@@ -1045,6 +1129,8 @@ template <typename T> struct FilterRun {
   size_t slot_bytes() const { return (size_t)slot_elems * sizeof(C); }
   ~FilterRun() { release(); }
   void release() {
+    c->pp_chain = 0;
+    c->pp_ready = 0;
     if (ge) (void)hipGraphExecDestroy(ge);
     if (g) (void)hipGraphDestroy(g);
     if (d_an) (void)hipFree(d_an);
@@ -1057,6 +1143,10 @@ template <typename T> struct FilterRun {
   // ring (R = K+1 slots) + one work slot; coefficients; start states -> slot 0
   int prepare(int K) {
     int rc;
+    // the Newton steps of this run chain their projections through the
+    // combine's per-tile sums (each step's input is the step before's output)
+    c->pp_chain = 1;
+    c->pp_ready = 0;
     R = K + 1;
     c->last_window = K;
     if ((rc = ensure_scratch(c, slot_bytes() * (size_t)(R + 1)))) return rc;

@@ -97,10 +97,26 @@ template <typename T> struct AxisArgs {
   // lines, 16 for z lines).
   int zt_src, zt_dst;
   long zt_q, zt_inner, zt_grid, zt_nz16;
+  // spin-paired combine (pp_part != nullptr): also the raw projections of its
+  // output rows.  Tile tau = z * (ny / 8) + y0 / 8 has lanes pp_lane_off[tau]
+  // .. pp_lane_off[tau + 1] - 1; lane i holds the kPpRecs records
+  // pp_meta[kPpRecs * i + u] = ((y - y0) << 16) | x, values
+  // pp_val[(kPpRecs * i + u) * 8 ..] (the record layout of rec_val; padding
+  // records are zero) of one atom, pp_lanes[i] = (slot + 1) << 1 | head: head
+  // marks the first lane of a segment (an atom's lanes within one wavefront),
+  // slot + 1 > 0 its last lane, whose segment sums go to
+  // pp_part[(state * pp_nslots + slot) * 12 + spin * 6 + (SO m, NL m)];
+  // pp_span[tau]: the tile's longest segment (lanes, a power of two)
+  const int *pp_lane_off, *pp_lanes, *pp_span, *pp_meta;
+  const T *pp_val;
+  cplx_t<T> *pp_part;
+  long pp_nslots;
 };
 
 // grids (state, spin columns) one work-group of the projection pass takes
 constexpr int kProjColChunk = 16;
+// projector records per lane of the projections inside the spin-paired combine
+constexpr int kPpRecs = 4;
 
 enum PassKind { kPassFwd = 0, kPassBwd = 1, kPassKin = 2, kPassBwdCombine = 3, kPassPrec = 4 };
 
@@ -206,6 +222,25 @@ struct fbse_ctx {
   int axis_fuse = 1;
   const void *ax_pjd = nullptr;
   void *ax_fwd = nullptr;
+  // projections inside the spin-paired combine (fbse_set_combine_projection):
+  // per tile of 8 rows of one plane its records grouped by atom (AxisArgs::pp_*),
+  // the atoms' slots atom-major (atom a: slots pp_atom_off[a] .. pp_atom_off[a+1]-1);
+  // the combine writes the per-slot sums into d_pp_part, and the next Newton
+  // step reduces them per atom instead of running the projection pass when
+  // pp_ready and its input is pp_for (the last such combine's output)
+  int pp_on = 1;
+  int *d_pp_lane_off = nullptr, *d_pp_lanes = nullptr, *d_pp_span = nullptr, *d_pp_meta = nullptr;
+  int *d_pp_atom_off = nullptr;
+  double *d_pp_val = nullptr;
+  float *d_pp_val_f = nullptr;
+  long pp_nslots = 0;
+  void *d_pp_part = nullptr;
+  size_t pp_part_bytes = 0;
+  int pp_chain = 0;             // set while a filter run chains its Newton steps
+  int pp_ready = 0;
+  const void *pp_for = nullptr;
+  long pp_for_n = 0;
+  void *ax_pp = nullptr;        // one combine's partial-sum buffer
   // the Newton step's y and z passes in the z-tile layout (AxisArgs::zt_*):
   // the next per-axis pass reads (ax_zt & 1) / writes (ax_zt & 2) that layout
   int ax_zt = 0;
@@ -384,6 +419,9 @@ int proj_project(fbse_ctx *c, const cplx_t<T> *psi_in, long nstates, double a1, 
 // true when the plane passes can add the projector expansion (one radial
 // projector of each kind, entries by point built)
 bool proj_fusable(const fbse_ctx *c);
+// d (column x atom x 6) from the per-tile projection sums of the last
+// spin-paired combine (fbse_ctx::d_pp_part) for nstates states
+template <typename T> int proj_reduce(fbse_ctx *c, long nstates, double a1, cplx_t<T> *d);
 
 // element-wise helpers (ham_kernels.hip)
 int pack_real(fbse_ctx *c, const double *src, long src_stride, long nstates, double2 *dst);

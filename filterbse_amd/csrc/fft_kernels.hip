@@ -407,6 +407,18 @@ static int fft_pass_lines(fbse_ctx *c, int axis, int kind, const cplx_t<T> *src,
     A.fwd_dst = (cplx_t<T> *)c->ax_fwd;
     // spinors in the automatic mode: tiles of 8 rows of both spins of a state
     A.spin_pair = c->axis_fuse == 1 && c->nspin == 2 && lines == 16 && c->ny % 8 == 0 ? 1 : 0;
+    if (c->ax_pp) {
+      if (!A.spin_pair || !c->ax_pjd)
+        return set_err(c, FBSE_ERR_UNSUPPORTED, "projections in the combine need its spin-paired tiles");
+      A.pp_lane_off = c->d_pp_lane_off;
+      A.pp_lanes = c->d_pp_lanes;
+      A.pp_span = c->d_pp_span;
+      A.pp_meta = c->d_pp_meta;
+      if constexpr (sizeof(T) == 8) A.pp_val = c->d_pp_val;
+      else A.pp_val = c->d_pp_val_f;
+      A.pp_part = (cplx_t<T> *)c->ax_pp;
+      A.pp_nslots = c->pp_nslots;
+    }
   }
   // y and z passes of the Newton step in the z-tile layout (AxisArgs::zt_*)
   if (c->ax_zt) {
@@ -684,6 +696,26 @@ template <typename T> static bool axis_zt(const fbse_ctx *c) {
          c->ksqr_sep && (c->force_lines == 0 || c->force_lines == 1016);
 }
 
+// Projections inside the spin-paired combine (fbse_set_combine_projection):
+// within a filter run's chain of Newton steps, with one radial projector of
+// each kind and the combine in 8-row tiles of both spins.
+template <typename T> static bool pp_usable(const fbse_ctx *c) {
+  return c->pp_on && c->pp_chain && !c->no_alloc && c->d_pp_lanes && c->has_proj && proj_fusable(c) &&
+         c->axis_fuse == 1 && c->nspin == 2 && c->ny % 8 == 0 && fused_lines<T>(c) == 16;
+}
+template <typename T> static int ensure_pp_part(fbse_ctx *c, long nstates) {
+  const size_t need = (size_t)nstates * c->pp_nslots * 12 * sizeof(cplx_t<T>);
+  if (need > c->pp_part_bytes) {
+    if (c->d_pp_part) (void)hipFree(c->d_pp_part);
+    c->d_pp_part = nullptr;
+    c->pp_part_bytes = 0;
+    c->pp_ready = 0;
+    FBSE_HIP(c, hipMalloc(&c->d_pp_part, need));
+    c->pp_part_bytes = need;
+  }
+  return FBSE_OK;
+}
+
 template <typename T>
 static int apply_h_axes_chain(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out, cplx_t<T> *work, long nstates,
                               T a1, T a0, bool first) {
@@ -691,10 +723,20 @@ static int apply_h_axes_chain(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out,
   cplx_t<T> *d = nullptr;
   bool joined = true;
   int rc;
+  const bool pp = pp_usable<T>(c);
   if (c->has_proj) {
     if ((rc = ensure_pjd<T>(c, nstates))) return rc;
     d = (cplx_t<T> *)c->d_pjd;
-    if ((rc = project_forked<T>(c, psi, nstates, (double)a1, d, &joined))) return rc;
+    // the previous step's combine left psi's per-tile projection sums
+    const bool sums = pp && !first && c->pp_ready && c->pp_for == (const void *)psi && c->pp_for_n == nstates;
+    c->pp_ready = 0;
+    if (sums) rc = proj_reduce<T>(c, nstates, (double)a1, d);
+    else rc = project_forked<T>(c, psi, nstates, (double)a1, d, &joined);
+    if (rc) return rc;
+  }
+  if (pp && (rc = ensure_pp_part<T>(c, nstates))) {
+    if (!joined) (void)hipStreamWaitEvent(c->stream, c->ev_join, 0);
+    return rc;
   }
   rc = first ? fft_pass<T>(c, 0, kPassFwd, psi, work, nb, nullptr, 0, 0) : FBSE_OK;
   if (axis_zt<T>(c)) {
@@ -717,9 +759,16 @@ static int apply_h_axes_chain(fbse_ctx *c, const cplx_t<T> *psi, cplx_t<T> *out,
   if (rc) return rc;
   c->ax_pjd = d;
   c->ax_fwd = work;
+  c->ax_pp = pp ? c->d_pp_part : nullptr;
   rc = fft_pass_lines<T>(c, 0, kPassBwdCombine, work, out, nb, psi, a1, a0, 1000 + fused_lines<T>(c));
   c->ax_pjd = nullptr;
   c->ax_fwd = nullptr;
+  c->ax_pp = nullptr;
+  if (pp && !rc) {
+    c->pp_ready = 1;
+    c->pp_for = out;
+    c->pp_for_n = nstates;
+  }
   return rc;
 }
 

@@ -1037,6 +1037,84 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
           const int l = e / N, p = e % N;
           if (l < nl) dst[xoff(e)] = buf[l * LS + p];
         }
+        if constexpr (SP) {
+          if (A.pp_part) {
+            // raw projections of the output rows for the next Newton step: a
+            // lane takes kPpRecs records of one atom (contiguous loads of
+            // their positions and values), the lanes of a segment (one atom's
+            // lanes within the wavefront) are summed by a segmented scan (a
+            // fixed order) over the tile's longest segment, and the segment's
+            // last lane writes its slot
+            constexpr int NR = kPpRecs;
+            const long tau = (long)z * (A.ny / HL) + y0 / HL;
+            const int l0 = A.pp_lane_off[tau], nln = A.pp_lane_off[tau + 1] - l0, span = A.pp_span[tau];
+            C *part = A.pp_part + (item / 2) * A.pp_nslots * 12;
+            const int wl = tid & 63;
+            for (int r0 = 0; r0 < nln; r0 += NT) {
+              const int li = r0 + tid;
+              const bool live = li < nln;
+              int lw = 1;  // lanes past the tile's: segments of their own, no slot
+              int mq[NR];
+              T rv[NR][8];
+              if (live) {
+                lw = A.pp_lanes[l0 + li];
+                const int4 *m4 = reinterpret_cast<const int4 *>(A.pp_meta) + (long)(l0 + li) * (NR / 4);
+#pragma unroll
+                for (int h = 0; h < NR / 4; ++h) {
+                  const int4 t = m4[h];
+                  mq[4 * h] = t.x, mq[4 * h + 1] = t.y, mq[4 * h + 2] = t.z, mq[4 * h + 3] = t.w;
+                }
+#pragma unroll
+                for (int g = 0; g < NR; ++g) load_record<T>(A.pp_val + ((long)(l0 + li) * NR + g) * 8, rv[g]);
+              } else {
+#pragma unroll
+                for (int g = 0; g < NR; ++g) {
+                  mq[g] = 0;
+#pragma unroll
+                  for (int q = 0; q < 8; ++q) rv[g][q] = (T)0;
+                }
+              }
+              // first lane of this lane's segment: the highest head at or below it
+              const unsigned long long heads = __ballot(lw & 1);
+              const unsigned long long upto = wl == 63 ? heads : heads & ((2ull << wl) - 1);
+              const int s0 = 63 - __clzll(upto);
+#pragma unroll
+              for (int spin = 0; spin < 2; ++spin) {
+                // acc[2r], acc[2r+1]: r = SO m (0..2), NL m (3..5)
+                T acc[12];
+#pragma unroll
+                for (int q = 0; q < 12; ++q) acc[q] = (T)0;
+#pragma unroll
+                for (int g = 0; g < NR; ++g) {
+                  const int pos = (spin * HL + (mq[g] >> 16)) * LS + (mq[g] & 0xffff);
+                  const C v = buf[pos];
+                  const T wso = rv[g][0], wnl = rv[g][1];
+#pragma unroll
+                  for (int m = 0; m < 3; ++m) {
+                    const T yx = rv[g][2 + 2 * m], yy = rv[g][3 + 2 * m];
+                    const T ux = yx * v.x + yy * v.y, uy = yx * v.y - yy * v.x;  // conj(Y_m) psi
+                    acc[2 * m] += wso * ux;
+                    acc[2 * m + 1] += wso * uy;
+                    acc[6 + 2 * m] += wnl * ux;
+                    acc[7 + 2 * m] += wnl * uy;
+                  }
+                }
+                for (int o = 1; o < span; o <<= 1) {
+#pragma unroll
+                  for (int q = 0; q < 12; ++q) {
+                    const T f = __shfl_up(acc[q], o);
+                    if (wl - o >= s0) acc[q] += f;
+                  }
+                }
+                if (lw >> 1) {
+                  C *pp = part + (long)((lw >> 1) - 1) * 12 + spin * 6;
+#pragma unroll
+                  for (int r = 0; r < 6; ++r) pp[r] = C{acc[2 * r], acc[2 * r + 1]};
+                }
+              }
+            }
+          }
+        }
       }
       if (A.fwd_dst) {
         // the next Newton step's forward x transform of these rows

@@ -427,6 +427,96 @@ template int proj_expand<float>(fbse_ctx *, const float2 *, float2 *, long);
 template int proj_project<double>(fbse_ctx *, const double2 *, long, double, double2 *);
 template int proj_project<float>(fbse_ctx *, const float2 *, long, double, float2 *);
 
+// The projections of a state batch from the per-tile sums that the
+// spin-paired combine wrote (AxisArgs::pp_*): every atom's 12 raw sums (both
+// spins, SO and NL rows) added over its slots in tile order, then mixed as in
+// the projection pass (L.S across the two spins for SO, the projector sign
+// for NL, times dv * a1) into the coefficients d (column x atom x 6).
+template <typename T> struct PpReduceArgs {
+  const cplx_t<T> *part;
+  const int *atom_off;
+  long nslots;
+  int n_atoms, so, nl;
+  const cplx_t<T> *ls;
+  const int *sign;
+  T scale;
+  cplx_t<T> *d;
+};
+constexpr int kPpAtoms = 16;  // atoms per work-group, 12 sums each
+
+template <typename T>
+__global__ __launch_bounds__(kPpAtoms * 12) void proj_reduce_kernel(PpReduceArgs<T> R) {
+  using C = cplx_t<T>;
+  __shared__ C s[kPpAtoms][12];
+  const int al = threadIdx.x / 12, r = threadIdx.x % 12;
+  const int atom = blockIdx.x * kPpAtoms + al;
+  const long st = blockIdx.y;
+  C acc{(T)0, (T)0};
+  if (atom < R.n_atoms) {
+    const C *p = R.part + st * R.nslots * 12;
+    for (int g = R.atom_off[atom]; g < R.atom_off[atom + 1]; ++g) {
+      const C v = p[(long)g * 12 + r];
+      acc.x += v.x;
+      acc.y += v.y;
+    }
+  }
+  s[al][r] = acc;
+  __syncthreads();
+  if (atom >= R.n_atoms) return;
+  const int spin = r / 6, q = r % 6;
+  C d{(T)0, (T)0};
+  if (q < 3) {
+    if (R.so) {
+      const int j = spin * 3 + q;
+      for (int jp = 0; jp < 6; ++jp) {
+        const C l = R.ls[jp * 6 + j];
+        const C c = s[al][(jp / 3) * 6 + jp % 3];
+        d.x += l.x * c.x - l.y * c.y;
+        d.y += l.x * c.y + l.y * c.x;
+      }
+    }
+  } else if (R.nl) {
+    const T sg = (T)R.sign[atom * 5];
+    d = s[al][r];
+    d.x *= sg;
+    d.y *= sg;
+  }
+  d.x *= R.scale;
+  d.y *= R.scale;
+  R.d[((st * 2 + spin) * R.n_atoms + atom) * 6 + q] = d;
+}
+
+template <typename T> int proj_reduce(fbse_ctx *c, long nstates, double a1, cplx_t<T> *d) {
+  const ProjTables<T> *pt;
+  if constexpr (sizeof(T) == 8) pt = &c->pj;
+  else pt = &c->pj_f;
+  const size_t need = (size_t)nstates * c->pp_nslots * 12 * sizeof(cplx_t<T>);
+  if (!c->d_pp_atom_off || c->nspin != 2 || !c->d_pp_part || c->pp_part_bytes < need)
+    return set_err(c, FBSE_ERR_UNSUPPORTED, "no per-tile projection sums for %ld states", nstates);
+  if (nstates > 65535) return set_err(c, FBSE_ERR_UNSUPPORTED, "%ld states in one projection reduction", nstates);
+  PpReduceArgs<T> R{};
+  R.part = (const cplx_t<T> *)c->d_pp_part;
+  R.atom_off = c->d_pp_atom_off;
+  R.nslots = c->pp_nslots;
+  R.n_atoms = pt->n_atoms;
+  R.so = pt->so && pt->n_so > 0;
+  R.nl = pt->nl && pt->n_nl > 0;
+  R.ls = pt->ls;
+  R.sign = pt->sign;
+  R.scale = (T)(c->dv * a1);
+  R.d = d;
+  if (R.n_atoms == 0 || nstates == 0) return FBSE_OK;
+  hipEvent_t ev = nullptr;
+  prof_begin(c, 2, &ev);
+  const dim3 g((unsigned)((R.n_atoms + kPpAtoms - 1) / kPpAtoms), (unsigned)nstates);
+  hipLaunchKernelGGL((proj_reduce_kernel<T>), g, dim3(kPpAtoms * 12), 0, c->stream, R);
+  FBSE_HIP(c, hipGetLastError());
+  prof_end(c, 2, ev);
+  return FBSE_OK;
+}
+template int proj_reduce<double>(fbse_ctx *, long, double, double2 *);
+template int proj_reduce<float>(fbse_ctx *, long, double, float2 *);
+
 template int proj_apply<double>(fbse_ctx *, const double2 *, double2 *, long, double);
 template int proj_apply<float>(fbse_ctx *, const float2 *, float2 *, long, double);
 

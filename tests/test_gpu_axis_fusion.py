@@ -13,7 +13,12 @@ of both spins of one state by default (fusion 1) or 16 consecutive rows of
 one grid (fusion 16): both must equal the separate passes bit for bit.  On
 160^3 the default mode also runs the step's y and z passes in the z-tile
 layout (y forward into the step's ring slot, z there in contiguous tiles, y
-backward back): again bit for bit."""
+backward back): again bit for bit.  These comparisons run with the
+projections in their own pass; with the default projections inside the
+spin-paired combine (fbse_set_combine_projection: per-tile sums, reduced per
+atom by the next step) only the summation order of the projections changes,
+so the filtered states agree with the separate passes to rounding (relative
+1e-12 in FP64, 1e-5 in FP32), and the FP64 filter matches the oracle."""
 import numpy as np
 import pytest
 
@@ -46,6 +51,7 @@ def test_fused_combine_identical(case, precision):
     g.set_window(4)
     n_starts, ms, M = 3, 4, 40
     psi_rank, stlen, an, zn, Vmin, dE_1 = filter_inputs(s, n_starts, ms, M)
+    g.set_combine_projection(0)
     g.set_axis_fusion(0)
     sep = g.filter(psi_rank, n_starts, ms, an, zn, Vmin, dE_1, precision=precision)
     g.set_axis_fusion(1)
@@ -57,9 +63,19 @@ def test_fused_combine_identical(case, precision):
     rows16 = g.filter(psi_rank, n_starts, ms, an, zn, Vmin, dE_1, precision=precision)
     g.set_axis_fusion(1)
     assert np.array_equal(rows16, sep)
+    # projections summed inside the spin-paired combine (the default)
+    g.set_combine_projection(1)
+    in_combine = g.filter(psi_rank, n_starts, ms, an, zn, Vmin, dE_1, precision=precision)
+    assert g.graph_replays() > 0
+    rel = np.abs(in_combine - sep).max() / np.abs(sep).max()
+    print("projections in the combine vs their own pass: %.2e" % rel)
+    assert rel < (1e-12 if precision == 0 else 1e-5)
+    if CASES[case]["nspin"] == 1:
+        assert np.array_equal(in_combine, sep)  # no spin-paired tiles: the projection pass runs
     if precision == 0 and np.prod(CASES[case]["n"]) <= ORACLE_MAX_POINTS:
         ref = oracle_ham(s).filter_cycles(psi_rank, n_starts, ms, an, zn, Vmin, dE_1)
-        err = np.abs(fused - ref).max() / np.abs(ref).max()
-        print("fused FP64 filter vs oracle: %.2e" % err)
-        assert err < 1e-10
+        for name, got in (("fused", fused), ("projections in the combine", in_combine)):
+            err = np.abs(got - ref).max() / np.abs(ref).max()
+            print("%s FP64 filter vs oracle: %.2e" % (name, err))
+            assert err < 1e-10
     g.close()
