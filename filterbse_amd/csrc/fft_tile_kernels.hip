@@ -447,9 +447,9 @@ __device__ __forceinline__ void fx_pass(int tw_off) {
   // opaque thread index: the LDS addresses below are recomputed in every
   // pass (a few integer operations) instead of being hoisted out of the tile
   // loop, where they would hold registers the butterflies need
-  int tid = threadIdx.x;
+  unsigned tid = threadIdx.x;
   asm volatile("" : "+v"(tid));
-  const int l0 = tid % NL, i0 = tid / NL;
+  const int l0 = (int)(tid % NL), i0 = (int)(tid / NL);  // unsigned: shifts for power-of-two NL
   const int lbase = l0 * LSTR;
   C x[KB][R];
   int dsti[KB], tjk[KB];
@@ -792,9 +792,11 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
   }
   // element e = tid + u*NT of a tile: line l, point p; global offset (within
   // the tile, elements) XAX ? e : p*inner + l, below 2^31 / sizeof(C)
-  int tid = threadIdx.x;
+  unsigned tid = threadIdx.x;
   for (long t = blockIdx.x; t < A.ntiles; t += gridDim.x) {
     asm volatile("" : "+v"(tid));  // per-tile recomputation of the lane's offsets
+    // (unsigned: the element indices below divide by powers of two as shifts)
+    const int itid = (int)tid;
     long base, i0 = 0, line0 = 0, zbase = 0;
     int nl;
     if (XAX) {
@@ -830,19 +832,19 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
     const unsigned inner = (unsigned)A.inner;
     // spin-paired: line l at (l / HL) grids and (l % HL) rows from the tile base
     const unsigned rowsN = (unsigned)A.ny * (unsigned)A.nz * (unsigned)N;
-    auto xoff = [&](int e) -> unsigned {
+    auto xoff = [&](unsigned e) -> unsigned {
       return SP ? (unsigned)((e / N) / HL) * rowsN + (unsigned)(((e / N) % HL) * N + e % N) : (unsigned)e;
     };
-    auto goff = [&](int e) -> unsigned {
+    auto goff = [&](unsigned e) -> unsigned {
       return XAX ? xoff(e) : (unsigned)(e / L) * inner + (unsigned)(e % L);
     };
     // y / z lines: source and result in the reference or the z-tile layout
     const bool zs = !XAX && L == 16 && A.zt_src, zd = !XAX && L == 16 && A.zt_dst;
     const unsigned zinner = (unsigned)A.zt_inner;
-    auto soff = [&](int e) -> unsigned { return zs ? (unsigned)(e / L) * zinner + (unsigned)(e % L) : goff(e); };
-    auto doff = [&](int e) -> unsigned { return zd ? (unsigned)(e / L) * zinner + (unsigned)(e % L) : goff(e); };
-    auto lpos = [&](int e) -> int { return XAX ? (e / N) * LS + e % N : (e % L) * LS + e / L; };
-    auto line = [&](int e) -> int { return XAX ? e / N : e % L; };
+    auto soff = [&](unsigned e) -> unsigned { return zs ? (unsigned)(e / L) * zinner + (unsigned)(e % L) : goff(e); };
+    auto doff = [&](unsigned e) -> unsigned { return zd ? (unsigned)(e / L) * zinner + (unsigned)(e % L) : goff(e); };
+    auto lpos = [&](unsigned e) -> int { return (int)(XAX ? (e / N) * LS + e % N : (e % L) * LS + e / L); };
+    auto line = [&](unsigned e) -> int { return (int)(XAX ? e / N : e % L); };
     const C *src = A.src + (zs ? zbase : base);
     __syncthreads();  // the previous tile's LDS reads are done (first tile: tables written)
 #pragma unroll
@@ -850,7 +852,7 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
       T vr[LB], vi[LB];
 #pragma unroll
       for (int u = 0; u < LB; ++u) {
-        const int e = tid + (ub + u) * NT;
+        const unsigned e = tid + (unsigned)((ub + u) * NT);
         if (line(e) < nl) {
           const C v = src[soff(e)];
           vr[u] = v.x;
@@ -859,12 +861,12 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
       }
 #pragma unroll
       for (int u = 0; u < LB; ++u) {
-        const int e = tid + (ub + u) * NT;
+        const unsigned e = tid + (unsigned)((ub + u) * NT);
         if (line(e) < nl) buf[lpos(e)] = C{vr[u], vi[u]};
       }
     }
     if (KIN) {
-      for (int l = tid; l < nl; l += NT) {
+      for (int l = itid; l < nl; l += NT) {
         const long in = i0 + l;
         const long jy = in / A.nx, jx = in - jy * A.nx;
         kxyl[l] = A.kx[jx] + A.ky[jy];
@@ -878,8 +880,8 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
       // the transforms leave free
 #pragma unroll 4
       for (int u = 0; u < E; ++u) {
-        const int e = tid + u * NT;
-        const int l = e / N, p = e % N;
+        const unsigned e = tid + (unsigned)(u * NT);
+        const int l = (int)(e / N), p = (int)(e % N);
         T k2 = kxyl[l] + kzl[p];
         if (k2 > A.kcap) k2 = A.kcap;
         if (KIND == kPassPrec) k2 = A.pa / ((T)1 + k2 * A.pb);
@@ -899,8 +901,8 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
       const bool keep = expand || A.fwd_dst != nullptr;
 #pragma unroll
       for (int u = 0; u < E; ++u) {
-        const int e = tid + u * NT;
-        const int l = e / N, p = e % N;
+        const unsigned e = tid + (unsigned)(u * NT);
+        const int l = (int)(e / N), p = (int)(e % N);
         if (l < nl) {
           const long row = (line0 + (SP ? (long)(l / HL) * A.ny * A.nz + l % HL : l)) % A.vloc_rows;
           const T vv = A.vloc[row * N + p];
@@ -934,11 +936,11 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
         const int2 *pa = reinterpret_cast<const int2 *>(A.rec_pa);
         const int nr = A.pt_ranks;
         int *seg_start = reinterpret_cast<int *>(kzl), *seg_len = seg_start + nr;
-        if (tid < nr) {
-          const int *ro = A.rec_yoff + ((long)z * nr + tid) * (A.ny + 1) + y0;
+        if (itid < nr) {
+          const int *ro = A.rec_yoff + ((long)z * nr + itid) * (A.ny + 1) + y0;
           const int s0 = ro[0];
-          seg_start[tid] = s0;
-          seg_len[tid] = ro[HL] - s0;
+          seg_start[itid] = s0;
+          seg_len[itid] = ro[HL] - s0;
         }
         __syncthreads();  // also orders the combine's LDS writes before the rounds
         if constexpr (SP) {
@@ -979,7 +981,7 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
               C tv{}, tv2{};
               terms(cq, cv, tv, tv2);
               const int2 q0 = cq;
-              if (k + 1 < nr) load_one(k + 1, tid, cq, cv);
+              if (k + 1 < nr) load_one(k + 1, itid, cq, cv);
               add_one(q0, tv, tv2);
             }
             for (int b0 = NT; b0 < seg_len[k]; b0 += kG * NT) {
@@ -987,7 +989,7 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
               T rv[kG][8];
               C tv[kG], tv2[kG];
 #pragma unroll
-              for (int g = 0; g < kG; ++g) load_one(k, b0 + g * NT + tid, q[g], rv[g]);
+              for (int g = 0; g < kG; ++g) load_one(k, b0 + g * NT + itid, q[g], rv[g]);
 #pragma unroll
               for (int g = 0; g < kG; ++g) terms(q[g], rv[g], tv[g], tv2[g]);
 #pragma unroll
@@ -997,7 +999,7 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
           };
           int2 qa;
           T va[8];
-          if (nr > 0) load_one(0, tid, qa, va);
+          if (nr > 0) load_one(0, itid, qa, va);
           for (int k = 0; k < nr; ++k) round(k, qa, va);
         } else {
           // inside a round a thread takes kG records at a time and issues all
@@ -1010,13 +1012,13 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
               C tv[kG];
 #pragma unroll
               for (int g = 0; g < kG; ++g) {
-                const int i = b0 + g * NT + tid;
+                const int i = b0 + g * NT + itid;
                 q[g] = i < n0 ? pa[s0 + i] : int2{-1, 0};
               }
 #pragma unroll
               for (int g = 0; g < kG; ++g)
                 if (q[g].x >= 0)
-                  tv[g] = record_term<T>(A.rec_val + (long)(s0 + b0 + g * NT + tid) * 8, dcol + (long)q[g].y * 6);
+                  tv[g] = record_term<T>(A.rec_val + (long)(s0 + b0 + g * NT + itid) * 8, dcol + (long)q[g].y * 6);
 #pragma unroll
               for (int g = 0; g < kG; ++g) {
                 if (q[g].x >= 0) {
@@ -1033,8 +1035,8 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
         }
 #pragma unroll 4
         for (int u = 0; u < E; ++u) {
-          const int e = tid + u * NT;
-          const int l = e / N, p = e % N;
+          const unsigned e = tid + (unsigned)(u * NT);
+          const int l = (int)(e / N), p = (int)(e % N);
           if (l < nl) dst[xoff(e)] = buf[l * LS + p];
         }
         if constexpr (SP) {
@@ -1049,9 +1051,9 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
             const long tau = (long)z * (A.ny / HL) + y0 / HL;
             const int l0 = A.pp_lane_off[tau], nln = A.pp_lane_off[tau + 1] - l0, span = A.pp_span[tau];
             C *part = A.pp_part + (item / 2) * A.pp_nslots * 12;
-            const int wl = tid & 63;
+            const int wl = itid & 63;
             for (int r0 = 0; r0 < nln; r0 += NT) {
-              const int li = r0 + tid;
+              const int li = r0 + itid;
               const bool live = li < nln;
               int lw = 1;  // lanes past the tile's: segments of their own, no slot
               int mq[NR];
@@ -1123,15 +1125,15 @@ __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
         C *fd = A.fwd_dst + base;
 #pragma unroll 4
         for (int u = 0; u < E; ++u) {
-          const int e = tid + u * NT;
-          const int l = e / N, p = e % N;
+          const unsigned e = tid + (unsigned)(u * NT);
+          const int l = (int)(e / N), p = (int)(e % N);
           if (l < nl) fd[xoff(e)] = buf[l * LS + p];
         }
       }
     } else {
 #pragma unroll
       for (int u = 0; u < E; ++u) {
-        const int e = tid + u * NT;
+        const unsigned e = tid + (unsigned)(u * NT);
         if (line(e) < nl) dst[doff(e)] = buf[lpos(e)];
       }
     }
