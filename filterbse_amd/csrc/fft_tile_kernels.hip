@@ -771,10 +771,125 @@ FBSE_FX_F32(FBSE_FX_G32)
 // SP (spin-paired combine, x rows): a tile holds rows y0 .. y0+L/2-1 of one
 // plane of both spins of one state (lines l < L/2 spin up, the rest spin
 // down), so the tile's projector records are read once for the two grids.
+// strided (y / z) passes with the next tile's loads in flight (axis_fx_strided)
+constexpr bool kAxisPipelined = true;
+
+// y and z passes (strided lines; forward, backward, k^2 or preconditioner):
+// a work-group's next tile is loaded into registers while the current tile
+// is transformed, so that its HBM reads overlap the LDS sweeps.  Same tiles,
+// same arithmetic, as the plain loop of axis_fx_body.
+template <typename T, int KIND, int N, int R1, int R2, int L, int NT>
+__device__ __forceinline__ void axis_fx_strided(const AxisArgs<T> &A) {
+  using C = cplx_t<T>;
+  constexpr int LS = N | 1, E = N * L / NT;
+  static_assert(NT % L == 0 && E * NT == N * L && R1 * R2 == N, "fixed axis tile shape");
+  constexpr int tw_off = L * LS;
+  C *buf = reinterpret_cast<C *>(tile_lds);
+  C *tw = buf + tw_off;
+  T *kzl = reinterpret_cast<T *>(tw + N);
+  T *kxyl = kzl + N;
+  constexpr bool KIN = KIND == kPassKin || KIND == kPassPrec;
+  constexpr bool INV = KIND == kPassBwd;
+  for (int t = threadIdx.x; t < N; t += NT) {
+    tw[t] = A.tw[t];
+    if (KIN) kzl[t] = A.kz[t];
+  }
+  // element e = tid + u*NT of a tile: line e % L, axis point e / L; source
+  // and result in the reference or the z-tile layout (AxisArgs::zt_*)
+  unsigned tid = threadIdx.x;
+  const bool zs = L == 16 && A.zt_src, zd = L == 16 && A.zt_dst;
+  const unsigned inner = (unsigned)A.inner, zinner = (unsigned)A.zt_inner;
+  struct Geo {
+    long base, zbase, i0;
+    int nl;
+  };
+  auto geo = [&](long t) {
+    Geo g;
+    const long o = t / A.tiles_per_outer;
+    g.i0 = (t - o * A.tiles_per_outer) * L;
+    g.base = o * A.outer_stride + g.i0;
+    g.zbase = (zs || zd) ? (o / A.zt_q) * A.zt_grid + (o % A.zt_q) * 16 + (g.i0 / 16) * A.zt_nz16 : 0;
+    const long rem = A.inner_count - g.i0;
+    g.nl = rem < L ? (int)rem : L;
+    return g;
+  };
+  C pre[E];
+  auto load_tile = [&](const Geo &g) {
+    const C *src = A.src + (zs ? g.zbase : g.base);
+    const unsigned st = zs ? zinner : inner;
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const unsigned e = tid + (unsigned)(u * NT);
+      if ((int)(e % L) < g.nl) pre[u] = src[(e / L) * st + e % L];
+    }
+  };
+  long t = blockIdx.x;
+  Geo cur{};
+  if (t < A.ntiles) {
+    cur = geo(t);
+    load_tile(cur);
+  }
+  for (; t < A.ntiles; t += gridDim.x) {
+    asm volatile("" : "+v"(tid));  // per-tile recomputation of the lane's offsets
+    const int itid = (int)tid;
+    __syncthreads();  // the previous tile's LDS reads are done (first tile: tables written)
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const unsigned e = tid + (unsigned)(u * NT);
+      if ((int)(e % L) < cur.nl) buf[(e % L) * LS + e / L] = pre[u];
+    }
+    if (KIN) {
+      for (int l = itid; l < cur.nl; l += NT) {
+        const long in = cur.i0 + l;
+        const long jy = in / A.nx, jx = in - jy * A.nx;
+        kxyl[l] = A.kx[jx] + A.ky[jy];
+      }
+    }
+    // the next tile's loads are in flight during this tile's transforms
+    const long tn = t + gridDim.x;
+    Geo nxt{};
+    if (tn < A.ntiles) {
+      nxt = geo(tn);
+      load_tile(nxt);
+    }
+    __syncthreads();
+    // lines l >= nl of a last partial tile hold stale values: transformed, never stored
+    fx_fft<T, N, R1, R2, L, 1, LS, NT, INV>(tw_off);
+    if (KIN) {
+#pragma unroll 4
+      for (int u = 0; u < E; ++u) {
+        const unsigned e = tid + (unsigned)(u * NT);
+        const int l = (int)(e / N), p = (int)(e % N);
+        T k2 = kxyl[l] + kzl[p];
+        if (k2 > A.kcap) k2 = A.kcap;
+        if (KIND == kPassPrec) k2 = A.pa / ((T)1 + k2 * A.pb);
+        C v = buf[l * LS + p];
+        v.x *= k2;
+        v.y *= k2;
+        buf[l * LS + p] = v;
+      }
+      __syncthreads();
+      fx_fft<T, N, R1, R2, L, 1, LS, NT, true>(tw_off);
+    }
+    C *dst = A.dst + (zd ? cur.zbase : cur.base);
+    const unsigned dt = zd ? zinner : inner;
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+      const unsigned e = tid + (unsigned)(u * NT);
+      if ((int)(e % L) < cur.nl) dst[(e / L) * dt + e % L] = buf[(e % L) * LS + e / L];
+    }
+    cur = nxt;
+  }
+}
+
 template <typename T, int KIND, int N, int R1, int R2, int L, int NT, bool XAX, int EXP_GROUP, bool SP = false>
 __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
   using C = cplx_t<T>;
   static_assert(!SP || (XAX && KIND == kPassBwdCombine && L % 2 == 0), "spin-paired tiles are combine tiles");
+  if constexpr (!XAX && KIND != kPassBwdCombine && kAxisPipelined) {
+    axis_fx_strided<T, KIND, N, R1, R2, L, NT>(A);
+    return;
+  }
   constexpr int HL = SP ? L / 2 : L;  // rows of one grid in a tile
   constexpr int LS = N | 1, E = N * L / NT;
   constexpr int LB = E % 8 == 0 ? 8 : (E % 5 == 0 ? 5 : E);  // global rows in flight per batch
