@@ -463,8 +463,8 @@ class Context:
 
     def set_combine_projection(self, enable):
         """Projections summed inside the spin-paired combine and reduced per
-        atom by the next Newton step (1, default) or the projection pass every
-        step (0)."""
+        atom by the next Newton step (1) or the projection pass every step (0,
+        default)."""
         self._check(gpu().fbse_set_combine_projection(self.ctx, int(enable)))
 
     def trim(self):

@@ -222,13 +222,13 @@ struct fbse_ctx {
   int axis_fuse = 1;
   const void *ax_pjd = nullptr;
   void *ax_fwd = nullptr;
-  // projections inside the spin-paired combine (fbse_set_combine_projection):
+  // projections inside the spin-paired combine (fbse_set_combine_projection, off by default):
   // per tile of 8 rows of one plane its records grouped by atom (AxisArgs::pp_*),
   // the atoms' slots atom-major (atom a: slots pp_atom_off[a] .. pp_atom_off[a+1]-1);
   // the combine writes the per-slot sums into d_pp_part, and the next Newton
   // step reduces them per atom instead of running the projection pass when
   // pp_ready and its input is pp_for (the last such combine's output)
-  int pp_on = 1;
+  int pp_on = 0;
   int *d_pp_lane_off = nullptr, *d_pp_lanes = nullptr, *d_pp_span = nullptr, *d_pp_meta = nullptr;
   int *d_pp_atom_off = nullptr;
   double *d_pp_val = nullptr;

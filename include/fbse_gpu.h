@@ -260,14 +260,16 @@ int fbse_set_graph(fbse_ctx *ctx, int enable);
  * hamiltonian.c:62-99, phamiltonian.c:140-444). */
 int fbse_set_axis_fusion(fbse_ctx *ctx, int enable);
 
-/* Projections inside the spin-paired combine (default on): in the Newton
+/* Projections inside the spin-paired combine (default off): in the Newton
  * filter's chain of steps over spinors with the combine's 8-row tiles of both
  * spins (fbse_set_axis_fusion 1, ny a multiple of 8, one radial SO and NL
  * projector), the combine also sums the projections of its output rows per
  * tile and atom, and the next step adds those sums per atom in a fixed order
  * instead of running the projection pass, which gathers the sphere points
- * from HBM again; 0 runs the projection pass every step.  Results agree to
- * the rounding of the changed summation order.  No counterpart in the
+ * from HBM again; 0 (default) runs the projection pass every step: it moves
+ * 4 GB less per 8-state launch at 256^3 but the combine grows by more than
+ * the pass it replaces.  Results agree to the rounding of the changed
+ * summation order.  No counterpart in the
  * reference (its projections are a separate loop over the atoms' sphere
  * points, phamiltonian.c:140-444). */
 int fbse_set_combine_projection(fbse_ctx *ctx, int enable);

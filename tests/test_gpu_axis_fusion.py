@@ -14,7 +14,7 @@ one grid (fusion 16): both must equal the separate passes bit for bit.  On
 160^3 the default mode also runs the step's y and z passes in the z-tile
 layout (y forward into the step's ring slot, z there in contiguous tiles, y
 backward back): again bit for bit.  These comparisons run with the
-projections in their own pass; with the default projections inside the
+projections in their own pass (the default); with the projections inside the
 spin-paired combine (fbse_set_combine_projection: per-tile sums, reduced per
 atom by the next step) only the summation order of the projections changes,
 so the filtered states agree with the separate passes to rounding (relative
@@ -63,7 +63,7 @@ def test_fused_combine_identical(case, precision):
     rows16 = g.filter(psi_rank, n_starts, ms, an, zn, Vmin, dE_1, precision=precision)
     g.set_axis_fusion(1)
     assert np.array_equal(rows16, sep)
-    # projections summed inside the spin-paired combine (the default)
+    # projections summed inside the spin-paired combine
     g.set_combine_projection(1)
     in_combine = g.filter(psi_rank, n_starts, ms, an, zn, Vmin, dE_1, precision=precision)
     assert g.graph_replays() > 0
@@ -78,4 +78,5 @@ def test_fused_combine_identical(case, precision):
             err = np.abs(got - ref).max() / np.abs(ref).max()
             print("%s FP64 filter vs oracle: %.2e" % (name, err))
             assert err < 1e-10
+    g.set_combine_projection(0)
     g.close()
