@@ -771,7 +771,8 @@ FBSE_FX_F32(FBSE_FX_G32)
 // SP (spin-paired combine, x rows): a tile holds rows y0 .. y0+L/2-1 of one
 // plane of both spins of one state (lines l < L/2 spin up, the rest spin
 // down), so the tile's projector records are read once for the two grids.
-// strided (y / z) passes with the next tile's loads in flight (axis_fx_strided)
+// strided (y / z) passes with the next tile's loads in flight (axis_fx_strided),
+// FP32 only: the FP64 instances, two work-groups per CU, ran 4% slower with it
 constexpr bool kAxisPipelined = true;
 
 // y and z passes (strided lines; forward, backward, k^2 or preconditioner):
@@ -886,7 +887,7 @@ template <typename T, int KIND, int N, int R1, int R2, int L, int NT, bool XAX, 
 __device__ __forceinline__ void axis_fx_body(const AxisArgs<T> &A) {
   using C = cplx_t<T>;
   static_assert(!SP || (XAX && KIND == kPassBwdCombine && L % 2 == 0), "spin-paired tiles are combine tiles");
-  if constexpr (!XAX && KIND != kPassBwdCombine && kAxisPipelined) {
+  if constexpr (!XAX && KIND != kPassBwdCombine && kAxisPipelined && sizeof(T) == 4) {
     axis_fx_strided<T, KIND, N, R1, R2, L, NT>(A);
     return;
   }
