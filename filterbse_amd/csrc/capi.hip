@@ -221,6 +221,86 @@ template <typename T, typename S> static std::vector<T> cast_vec(const S *p, siz
   return v;
 }
 
+// Tables of the projections inside the spin-paired combine, built on the
+// first fbse_set_combine_projection(ctx, 1) from the rank records (record e:
+// pa[2e] = (y << 16) | x, pa[2e + 1] = atom, values pe[8e ..]; bucket (z, k)
+// rows y .. at yoff[(z * nr + k) * (ny + 1) + y]).
+static int build_pp_tables(fbse_ctx *c, long na, int nr, const std::vector<int> &pa, const std::vector<double> &pe,
+                           const std::vector<int> &yoff) {
+  int rc;
+  // Projections inside the spin-paired combine (AxisArgs::pp_*): per tile
+  // of 8 rows of one plane, its records of every rank grouped by atom
+  // (atoms ascending, a group's records in point order, padded with
+  // zero-weight records to a multiple of kPpRecs), kPpRecs records per lane;
+  // an atom's lanes within one wavefront are one segment, whose sum is one
+  // slot, and pp_span[tile] is the tile's longest segment in lanes, rounded
+  // up to a power of two.  Slots are numbered atom-major, each atom's in
+  // tile order.
+  {
+    const long tpz = c->ny / 8, ntile = c->nz * tpz;
+    struct Piece {
+      int atom, end_lane;
+    };
+    std::vector<Piece> pieces;
+    std::vector<long> piece_off(ntile + 1, 0);
+    std::vector<int> lane_off(ntile + 1, 0), lane_w, meta, span(ntile, 1);
+    std::vector<double> val;
+    std::vector<std::array<int, 3>> tmp;  // (atom, (y << 16) | x, record)
+    for (long tau = 0; tau < ntile; ++tau) {
+      const long z = tau / tpz, y0 = (tau % tpz) * 8;
+      tmp.clear();
+      for (int k = 0; k < nr; ++k) {
+        const size_t b = ((size_t)z * nr + k) * (c->ny + 1);
+        for (int e = yoff[b + y0]; e < yoff[b + y0 + 8]; ++e)
+          tmp.push_back({pa[2 * (size_t)e + 1], pa[2 * (size_t)e], e});
+      }
+      std::sort(tmp.begin(), tmp.end());
+      const int base = (int)lane_w.size();  // tile's first lane
+      for (size_t i = 0; i < tmp.size();) {
+        size_t j = i;
+        while (j < tmp.size() && tmp[j][0] == tmp[i][0]) ++j;
+        const int n = (int)(j - i), nlr = (n + kPpRecs - 1) / kPpRecs;
+        int seg = 0;
+        for (int q = 0; q < nlr; ++q) {
+          const int li = (int)lane_w.size() - base;  // lane within the tile
+          const bool head = q == 0 || li % 64 == 0;
+          lane_w.push_back(head ? 1 : 0);
+          seg = head ? 1 : seg + 1;
+          while (span[tau] < seg) span[tau] *= 2;
+          for (int u = 0; u < kPpRecs; ++u) {
+            const size_t r = i + (size_t)std::min(q * kPpRecs + u, n - 1);
+            const bool pad = q * kPpRecs + u >= n;
+            const int yx = tmp[r][1];
+            meta.push_back((((yx >> 16) - (int)y0) << 16) | (yx & 0xffff));
+            for (int v = 0; v < 8; ++v) val.push_back(pad ? 0.0 : pe[(size_t)tmp[r][2] * 8 + v]);
+          }
+          // the segment ends at the group's last lane or a wavefront's last
+          if (q == nlr - 1 || (li + 1) % 64 == 0) pieces.push_back({tmp[i][0], (int)lane_w.size() - 1});
+        }
+        i = j;
+      }
+      lane_off[tau + 1] = (int)lane_w.size();
+      piece_off[tau + 1] = (long)pieces.size();
+    }
+    std::vector<int> aoff(na + 1, 0);
+    for (const Piece &pc : pieces) ++aoff[pc.atom + 1];
+    for (long a = 0; a < na; ++a) aoff[a + 1] += aoff[a];
+    std::vector<int> seen(aoff.begin(), aoff.end() - 1);
+    for (const Piece &pc : pieces) lane_w[pc.end_lane] |= (seen[pc.atom]++ + 1) << 1;
+    c->pp_nslots = aoff[na];
+    auto valf = cast_vec<float>(val.data(), val.size());
+    if ((rc = upload(c, (void **)&c->d_pp_lane_off, lane_off.data(), lane_off.size() * sizeof(int))) ||
+        (rc = upload(c, (void **)&c->d_pp_lanes, lane_w.data(), lane_w.size() * sizeof(int))) ||
+        (rc = upload(c, (void **)&c->d_pp_span, span.data(), span.size() * sizeof(int))) ||
+        (rc = upload(c, (void **)&c->d_pp_meta, meta.data(), meta.size() * sizeof(int))) ||
+        (rc = upload(c, (void **)&c->d_pp_val, val.data(), val.size() * sizeof(double))) ||
+        (rc = upload(c, (void **)&c->d_pp_val_f, valf.data(), valf.size() * sizeof(float))) ||
+        (rc = upload(c, (void **)&c->d_pp_atom_off, aoff.data(), aoff.size() * sizeof(int))))
+      return rc;
+  }
+  return FBSE_OK;
+}
+
 static int build_projectors(fbse_ctx *c, const fbse_proj_desc *pd) {
   const long na = pd->n_atoms, stride = pd->n_gridpts;
   std::vector<int> off(na), cnt(na), jx, sign(na * 5);
@@ -393,76 +473,6 @@ static int build_projectors(fbse_ctx *c, const fbse_proj_desc *pd) {
         while (e < roff[b + 1] && (pa[2 * (size_t)e] >> 16) < y) ++e;
         yoff[b * (c->ny + 1) + y] = e;
       }
-    }
-    // Projections inside the spin-paired combine (AxisArgs::pp_*): per tile
-    // of 8 rows of one plane, its records of every rank grouped by atom
-    // (atoms ascending, a group's records in point order, padded with
-    // zero-weight records to a multiple of kPpRecs), kPpRecs records per lane;
-    // an atom's lanes within one wavefront are one segment, whose sum is one
-    // slot, and pp_span[tile] is the tile's longest segment in lanes, rounded
-    // up to a power of two.  Slots are numbered atom-major, each atom's in
-    // tile order.
-    if (c->nspin == 2 && c->ny % 8 == 0) {
-      const long tpz = c->ny / 8, ntile = c->nz * tpz;
-      struct Piece {
-        int atom, end_lane;
-      };
-      std::vector<Piece> pieces;
-      std::vector<long> piece_off(ntile + 1, 0);
-      std::vector<int> lane_off(ntile + 1, 0), lane_w, meta, span(ntile, 1);
-      std::vector<double> val;
-      std::vector<std::array<int, 3>> tmp;  // (atom, (y << 16) | x, record)
-      for (long tau = 0; tau < ntile; ++tau) {
-        const long z = tau / tpz, y0 = (tau % tpz) * 8;
-        tmp.clear();
-        for (int k = 0; k < nr; ++k) {
-          const size_t b = ((size_t)z * nr + k) * (c->ny + 1);
-          for (int e = yoff[b + y0]; e < yoff[b + y0 + 8]; ++e)
-            tmp.push_back({pa[2 * (size_t)e + 1], pa[2 * (size_t)e], e});
-        }
-        std::sort(tmp.begin(), tmp.end());
-        const int base = (int)lane_w.size();  // tile's first lane
-        for (size_t i = 0; i < tmp.size();) {
-          size_t j = i;
-          while (j < tmp.size() && tmp[j][0] == tmp[i][0]) ++j;
-          const int n = (int)(j - i), nlr = (n + kPpRecs - 1) / kPpRecs;
-          int seg = 0;
-          for (int q = 0; q < nlr; ++q) {
-            const int li = (int)lane_w.size() - base;  // lane within the tile
-            const bool head = q == 0 || li % 64 == 0;
-            lane_w.push_back(head ? 1 : 0);
-            seg = head ? 1 : seg + 1;
-            while (span[tau] < seg) span[tau] *= 2;
-            for (int u = 0; u < kPpRecs; ++u) {
-              const size_t r = i + (size_t)std::min(q * kPpRecs + u, n - 1);
-              const bool pad = q * kPpRecs + u >= n;
-              const int yx = tmp[r][1];
-              meta.push_back((((yx >> 16) - (int)y0) << 16) | (yx & 0xffff));
-              for (int v = 0; v < 8; ++v) val.push_back(pad ? 0.0 : pe[(size_t)tmp[r][2] * 8 + v]);
-            }
-            // the segment ends at the group's last lane or a wavefront's last
-            if (q == nlr - 1 || (li + 1) % 64 == 0) pieces.push_back({tmp[i][0], (int)lane_w.size() - 1});
-          }
-          i = j;
-        }
-        lane_off[tau + 1] = (int)lane_w.size();
-        piece_off[tau + 1] = (long)pieces.size();
-      }
-      std::vector<int> aoff(na + 1, 0);
-      for (const Piece &pc : pieces) ++aoff[pc.atom + 1];
-      for (long a = 0; a < na; ++a) aoff[a + 1] += aoff[a];
-      std::vector<int> seen(aoff.begin(), aoff.end() - 1);
-      for (const Piece &pc : pieces) lane_w[pc.end_lane] |= (seen[pc.atom]++ + 1) << 1;
-      c->pp_nslots = aoff[na];
-      auto valf = cast_vec<float>(val.data(), val.size());
-      if ((rc = upload(c, (void **)&c->d_pp_lane_off, lane_off.data(), lane_off.size() * sizeof(int))) ||
-          (rc = upload(c, (void **)&c->d_pp_lanes, lane_w.data(), lane_w.size() * sizeof(int))) ||
-          (rc = upload(c, (void **)&c->d_pp_span, span.data(), span.size() * sizeof(int))) ||
-          (rc = upload(c, (void **)&c->d_pp_meta, meta.data(), meta.size() * sizeof(int))) ||
-          (rc = upload(c, (void **)&c->d_pp_val, val.data(), val.size() * sizeof(double))) ||
-          (rc = upload(c, (void **)&c->d_pp_val_f, valf.data(), valf.size() * sizeof(float))) ||
-          (rc = upload(c, (void **)&c->d_pp_atom_off, aoff.data(), aoff.size() * sizeof(int))))
-        return rc;
     }
     auto pef = cast_vec<float>(pe.data(), (long)pe.size());
     if ((rc = upload(c, (void **)&c->d_rec_yoff, yoff.data(), yoff.size() * sizeof(int))) ||
@@ -726,6 +736,18 @@ extern "C" int fbse_set_axis_fusion(fbse_ctx *c, int enable) {
 }
 extern "C" int fbse_set_combine_projection(fbse_ctx *c, int enable) {
   if (!c || enable < 0 || enable > 1) return FBSE_ERR_ARG;
+  if (enable && !c->d_pp_lanes && c->has_proj && c->nspin == 2 && c->ny % 8 == 0) {
+    // the tables are built on first use from the rank records on the device
+    FBSE_HIP(c, hipSetDevice(c->device));
+    const long npts = c->pj.npts, nr = c->pt_ranks;
+    std::vector<int> pa((size_t)npts * 2), yoff((size_t)c->nz * nr * (c->ny + 1));
+    std::vector<double> pe((size_t)npts * 8);
+    FBSE_HIP(c, hipMemcpy(pa.data(), c->d_rec_pa, pa.size() * sizeof(int), hipMemcpyDeviceToHost));
+    FBSE_HIP(c, hipMemcpy(yoff.data(), c->d_rec_yoff, yoff.size() * sizeof(int), hipMemcpyDeviceToHost));
+    FBSE_HIP(c, hipMemcpy(pe.data(), c->d_rec_val, pe.size() * sizeof(double), hipMemcpyDeviceToHost));
+    const int rc = build_pp_tables(c, c->pj.n_atoms, (int)nr, pa, pe, yoff);
+    if (rc) return rc;
+  }
   c->pp_on = enable;
   c->pp_ready = 0;
   return FBSE_OK;
